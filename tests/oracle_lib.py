@@ -1,0 +1,243 @@
+"""ctypes binding of the CPU oracle (oracle/qew_oracle.cpp) for the tests.
+
+Test infrastructure only: tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use it as the checker.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+
+TABLE, SINCOS, CONST, ZERO, HARMONIC, WASHBOARD = range(6)
+LINEAR, ARCLENGTH = 0, 1
+
+
+class Noise(C.Structure):
+    _fields_ = [("kind", C.c_int), ("nx", C.c_int), ("ny", C.c_int),
+                ("dx", C.c_double), ("dy", C.c_double),
+                ("coef", C.POINTER(C.c_double)), ("p", C.c_double * 4)]
+
+
+class Params(C.Structure):
+    _fields_ = [("physical_size", C.c_double), ("line_tension", C.c_double),
+                ("driving_force", C.c_double), ("model", C.c_int)]
+
+
+class LbfgsParams(C.Structure):
+    _fields_ = [("m", C.c_int), ("ftol", C.c_double), ("max_iter", C.c_int),
+                ("c1", C.c_double), ("ls_shrink", C.c_double), ("max_ls", C.c_int),
+                ("curvature_eps", C.c_double), ("precondition", C.c_int),
+                ("precond_shift", C.c_double)]
+
+
+class LbfgsResult(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("max_force", C.c_double),
+                ("converged", C.c_int), ("n_func_evals", C.c_long),
+                ("n_curvature_skips", C.c_long), ("n_ls_restarts", C.c_long)]
+
+
+class RkParams(C.Structure):
+    _fields_ = [("dstep", C.c_double), ("max_advance", C.c_double),
+                ("root_tol", C.c_double), ("sweep_tol", C.c_double),
+                ("runaway", C.c_double), ("max_sweeps", C.c_int)]
+
+
+class RkResult(C.Structure):
+    _fields_ = [("blocked", C.c_int), ("runaway", C.c_int), ("sweeps", C.c_int),
+                ("total_advance", C.c_double), ("mean_h", C.c_double)]
+
+
+_DP = C.POINTER(C.c_double)
+_libs = {}
+
+
+def load(omp=False):
+    name = "liboracle_omp.so" if omp else "liboracle.so"
+    if name in _libs:
+        return _libs[name]
+    path = os.path.join(ORACLE_DIR, name)
+    if not os.path.exists(path):
+        subprocess.run(["make", "-s", "-C", ORACLE_DIR, name], check=True)
+    lib = C.CDLL(path)
+    P = C.POINTER
+    lib.orc_bspline_weights.argtypes = [C.c_double, _DP, _DP]
+    lib.orc_sample_points.argtypes = [P(Noise), C.c_int, _DP, _DP, _DP, _DP, _DP]
+    lib.orc_sample_value.argtypes = [P(Noise), C.c_double, C.c_double]
+    lib.orc_sample_value.restype = C.c_double
+    lib.orc_sample_dy.argtypes = [P(Noise), C.c_double, C.c_double]
+    lib.orc_sample_dy.restype = C.c_double
+    lib.orc_objective.argtypes = [C.c_int, _DP, P(Params), P(Noise)]
+    lib.orc_objective.restype = C.c_double
+    lib.orc_objective_trial.argtypes = [C.c_int, _DP, _DP, C.c_double, P(Params), P(Noise)]
+    lib.orc_objective_trial.restype = C.c_double
+    lib.orc_gradient.argtypes = [C.c_int, _DP, P(Params), P(Noise), _DP]
+    lib.orc_fft.argtypes = [C.c_int, _DP, C.c_int]
+    lib.orc_fft2.argtypes = [C.c_int, C.c_int, _DP, C.c_int]
+    lib.orc_prefilter.argtypes = [C.c_int, C.c_int, _DP, _DP]
+    lib.orc_filtered_noise.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                       C.c_double, C.c_double, C.c_uint64, _DP, _DP]
+    lib.orc_precond_eig.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, _DP]
+    lib.orc_precondition.argtypes = [C.c_int, _DP, _DP, _DP]
+    lib.orc_lbfgs.argtypes = [C.c_int, _DP, P(Params), P(Noise), P(LbfgsParams), P(LbfgsResult)]
+    lib.orc_philox4x32_10.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
+    lib.orc_normal.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
+    lib.orc_normal.restype = C.c_double
+    lib.orc_langevin_step.argtypes = [C.c_int, _DP, P(Params), P(Noise), C.c_double, C.c_double,
+                                      C.c_uint64, C.c_uint64]
+    lib.orc_rk_site_velocity.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                         C.c_int, C.c_double, C.c_double, C.c_double, P(Noise)]
+    lib.orc_rk_site_velocity.restype = C.c_double
+    lib.orc_rk_block.argtypes = [C.c_int, _DP, P(Params), P(Noise), P(RkParams), P(RkResult)]
+    lib.orc_rk_block.restype = C.c_int
+    lib.orc_num_threads.restype = C.c_int
+    _libs[name] = lib
+    return lib
+
+
+def dp(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_DP)
+
+
+class OracleNoise:
+    """Owns the arrays an orc_noise points at."""
+
+    def __init__(self, kind, coef=None, nx=0, ny=0, dx=1.0, dy=1.0, p=(0, 0, 0, 0)):
+        self.coef = None if coef is None else np.ascontiguousarray(coef, dtype=np.float64)
+        self.s = Noise(kind, nx, ny, dx, dy,
+                       dp(self.coef) if self.coef is not None else _DP(),
+                       (C.c_double * 4)(*p))
+
+    @classmethod
+    def table(cls, coef, Lx, Ly):
+        nx, ny = coef.shape
+        return cls(TABLE, coef, nx, ny, Lx / nx, Ly / ny)
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+def params(L, lt, f, model):
+    return Params(L, lt, f, model)
+
+
+def lbfgs_params(m=8, ftol=1e-6, max_iter=10000, c1=1e-4, ls_shrink=0.5, max_ls=30,
+                 curvature_eps=1e-12, precondition=False, precond_shift=0.0):
+    return LbfgsParams(m, ftol, max_iter, c1, ls_shrink, max_ls, curvature_eps,
+                       int(precondition), precond_shift)
+
+
+def rk_params(dstep=0.0, max_advance=0.0, root_tol=1e-10, sweep_tol=1e-9, runaway=0.0,
+              max_sweeps=200000):
+    return RkParams(dstep, max_advance, root_tol, sweep_tol, runaway, max_sweeps)
+
+
+# ---- convenience wrappers ------------------------------------------------
+
+def sample_points(noise, x, y):
+    lib = load()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    v, dvx, dvy = (np.empty_like(x) for _ in range(3))
+    lib.orc_sample_points(noise.ref, len(x), dp(x), dp(y), dp(v), dp(dvx), dp(dvy))
+    return v, dvx, dvy
+
+
+def objective(h, p, noise, omp=False):
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    return load(omp).orc_objective(len(h), dp(h), C.byref(p), noise.ref)
+
+
+def objective_trial(x0, d, alpha, p, noise):
+    x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    return load().orc_objective_trial(len(x0), dp(x0), dp(d), alpha, C.byref(p), noise.ref)
+
+
+def gradient(h, p, noise, omp=False):
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    g = np.empty_like(h)
+    load(omp).orc_gradient(len(h), dp(h), C.byref(p), noise.ref, dp(g))
+    return g
+
+
+def fft(x, sign=-1):
+    x = np.asarray(x, dtype=np.complex128)
+    buf = np.ascontiguousarray(np.stack([x.real, x.imag], axis=-1).reshape(-1))
+    load().orc_fft(len(x), dp(buf), sign)
+    return buf[0::2] + 1j * buf[1::2]
+
+
+def fft2(a, sign=-1):
+    a = np.asarray(a, dtype=np.complex128)
+    nx, ny = a.shape
+    buf = np.ascontiguousarray(np.stack([a.real, a.imag], axis=-1).reshape(-1))
+    load().orc_fft2(nx, ny, dp(buf), sign)
+    return (buf[0::2] + 1j * buf[1::2]).reshape(nx, ny)
+
+
+def prefilter(field):
+    field = np.ascontiguousarray(field, dtype=np.float64)
+    coef = np.empty_like(field)
+    load().orc_prefilter(field.shape[0], field.shape[1], dp(field), dp(coef))
+    return coef
+
+
+def filtered_noise(nx, ny, Lx, Ly, amp, xi_x, xi_y, seed):
+    field = np.empty((nx, ny))
+    coef = np.empty((nx, ny))
+    load().orc_filtered_noise(nx, ny, Lx, Ly, amp, xi_x, xi_y, seed, dp(field), dp(coef))
+    return field, coef
+
+
+def precond_eig(n, kappa, L, shift):
+    eig = np.empty(n // 2 + 1)
+    load().orc_precond_eig(n, kappa, L, shift, dp(eig))
+    return eig
+
+
+def precondition(q, eig):
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    d = np.empty_like(q)
+    load().orc_precondition(len(q), dp(q), dp(np.ascontiguousarray(eig)), dp(d))
+    return d
+
+
+def lbfgs(h, p, noise, lp, omp=False):
+    h = np.array(h, dtype=np.float64)
+    r = LbfgsResult()
+    load(omp).orc_lbfgs(len(h), dp(h), C.byref(p), noise.ref, C.byref(lp), C.byref(r))
+    return h, r
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    load().orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def normal(seed, step, site):
+    return load().orc_normal(seed, step, site)
+
+
+def langevin_step(h, p, noise, a, b, seed, step):
+    load().orc_langevin_step(len(h), dp(h), C.byref(p), noise.ref, a, b, seed, step)
+
+
+def rk_site_velocity(i, n, dx, lt, f, model, hi, hl, hr, noise):
+    return load().orc_rk_site_velocity(i, n, dx, lt, f, model, hi, hl, hr, noise.ref)
+
+
+def rk_block(h, p, noise, rp, omp=False):
+    r = RkResult()
+    rc = load(omp).orc_rk_block(len(h), dp(h), C.byref(p), noise.ref, C.byref(rp), C.byref(r))
+    if rc != 0:
+        raise ValueError("rk_block: line length must be even for the red-black sweep")
+    return r
