@@ -1,0 +1,229 @@
+"""Benchmark of the qEW gradient/objective hot path on one MI355X.
+
+Workload (the gradient/objective throughput configuration): a line of
+L = 65536 sites on a 65536 x 8192 FilteredNoise table (xi = 8 grid units,
+unit amplitude, seed 1, grid units dx = dy = 1), arclength elasticity,
+line tension 1, no driving force. The line is a seeded random walk with
+unit-variance Gaussian steps started at Nh/2. One step = 10^4 fused
+gradient + energy evaluations issued as one batched launch; evaluation r sees
+the walk shifted by 8 r cells (mod Nh), so the evaluations in flight read
+disjoint table rows and the 1024-shift cycle spans the whole table (no L2/MALL
+reuse across evaluations). Line states and gradients of all 10^4 evaluations
+are distinct buffers in HBM.
+
+`value` is FP32-compensated evaluations per second (FP32 line offsets from a
+per-line anchor, FP32 table, two-sum energy); the FP64 path (bit-exact to the
+oracle) is reported beside it. The roofline uses the compulsory bytes per
+evaluation, s_c * sum_r U_r + L * (s_h + s_g), with U_r the number of distinct
+coefficients x-row r supplies (computed exactly from the walk).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+L = 65536
+NY = 8192
+XI = 8.0
+EVALS = 10000
+SHIFT = 8
+NSHIFT = NY // SHIFT
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+
+
+def walk(seed=2):
+    rng = np.random.default_rng(seed)
+    return NY / 2 + np.cumsum(rng.standard_normal(L))
+
+
+def distinct_coefficients(h, n, nx, ny):
+    """sum_r U_r: distinct (x-row, y-cell) pairs the edges of line h touch."""
+    i = np.arange(n)
+    ip = (i + 1) % n
+    gx = (i + ip) * 0.5 * (nx / n)
+    ix = np.floor(gx).astype(np.int64) % nx
+    hc = (h[ip] + h) * 0.5
+    iy = np.floor(np.mod(hc, ny)).astype(np.int64)
+    keys = []
+    for a in (-1, 0, 1, 2):
+        for b in (-1, 0, 1, 2):
+            keys.append(((ix + a) % nx) * ny + (iy + b) % ny)
+    return int(np.unique(np.concatenate(keys)).size)
+
+
+def traffic_from_profiles():
+    p = os.path.join(ROOT, "profiles", "eval_traffic.json")
+    if os.path.exists(p):
+        try:
+            return json.load(open(p))
+        except Exception:
+            return None
+    return None
+
+
+def cpu_baseline(coef_rowmajor, h0, budget_s=15.0):
+    """The oracle's OpenMP build (restatement of the reference's two-pass
+    gradient + separate objective reduce) on this host, on a bounded sample of
+    the same evaluations."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    lib = O.load(omp=True)
+    nz = O.OracleNoise.table(coef_rowmajor, float(L), float(NY))
+    p = O.params(float(L), 1.0, 0.0, O.ARCLENGTH)
+    t0 = time.perf_counter()
+    k = 0
+    while True:
+        h = h0 + SHIFT * (k % NSHIFT)
+        O.gradient(h, p, nz, omp=True)
+        O.objective(h, p, nz, omp=True)
+        k += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": k / dt, "unit": "grad evals/s", "cores": int(lib.orc_num_threads()),
+            "kind": "port",
+            "sample": f"{k} gradient+objective evaluations (FP64, L={L}, {L}x{NY} table, "
+                      f"OpenMP restatement of the reference two-pass gradient) in {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--evals", type=int, default=EVALS)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["both", "f32", "f64"], default="both")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(this build has no multi-GPU path)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import qew_amd as q
+
+    dev = torch.device("cuda:0")
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    h0 = walk()
+    B = args.evals
+    shifts = torch.tensor((np.arange(B) % NSHIFT) * SHIFT, dtype=torch.float64, device=dev)
+    u_sum = distinct_coefficients(h0, L, L, NY)
+    log(f"sum_r U_r = {u_sum} ({u_sum / L:.3f} per row)")
+
+    results = {}
+    params = q.Params(float(L), 1.0, 0.0, q.Model.Arclength)
+
+    def run(dtype):
+        t_gen = time.perf_counter()
+        noise = q.FilteredNoise(L, NY, float(L), float(NY), 1.0, XI, XI, 1, dtype=dtype)
+        torch.cuda.synchronize()
+        log(f"table {'f32' if dtype else 'f64'} generated in {time.perf_counter() - t_gen:.1f} s")
+        batch = q.Batch([L] * B, [noise] * B, [params] * B)
+        h0d = torch.tensor(h0, dtype=torch.float64, device=dev)
+        if dtype == q.F64:
+            h = (h0d[None, :] + shifts[:, None]).reshape(-1).contiguous()
+            anchor = None
+            grad = torch.empty_like(h)
+        else:
+            off = (h0d - NY / 2).to(torch.float32)
+            h = off[None, :].expand(B, L).contiguous().reshape(-1)
+            anchor = (NY / 2 + shifts).contiguous()
+            grad = torch.empty_like(h)
+        energy = torch.empty(B, dtype=torch.float64, device=dev)
+        for _ in range(args.warmup):
+            batch.eval(h, grad, energy, dtype=dtype, anchor=anchor)
+        torch.cuda.synchronize()
+        starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+        ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            starts[k].record()
+            batch.eval(h, grad, energy, dtype=dtype, anchor=anchor)
+            ends[k].record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
+        s_c = 4 if dtype == q.F32 else 8
+        bytes_eval = s_c * u_sum + L * 2 * s_c
+        res = {
+            "evals_per_s": B * args.steps / wall,
+            "ms_per_step": 1e3 * wall / args.steps,
+            "kernel_ms": kms,
+            "bytes_per_eval": bytes_eval,
+            "achieved_GBs": bytes_eval * B / (kms * 1e-3) / 1e9,
+        }
+        first = (h[:L].clone(), grad[:L].clone(), float(energy[0]))
+        if dtype == q.F64:
+            results["table64"] = noise
+        del batch, h, grad, energy
+        if dtype == q.F32:
+            del noise
+        torch.cuda.empty_cache()
+        return res, first
+
+    r32 = r64 = None
+    if args.only in ("both", "f64"):
+        r64, first64 = run(q.F64)
+        log(f"f64: {json.dumps(r64)}")
+    if args.only in ("both", "f32"):
+        r32, first32 = run(q.F32)
+        log(f"f32: {json.dumps(r32)}")
+
+    err = None
+    if r32 and r64:
+        g64 = first64[1].cpu().numpy()
+        g32 = first32[1].cpu().numpy().astype(np.float64)
+        err = {"grad_max_abs_err_over_max_grad": float(np.max(np.abs(g32 - g64)) / np.max(np.abs(g64))),
+               "energy_rel_err": abs(first32[2] - first64[2]) / abs(first64[2])}
+
+    cpu = None
+    if not args.no_cpu and "table64" in results:
+        coef = results["table64"].coefficients()
+        del results["table64"]
+        cpu = cpu_baseline(coef, h0)
+        log(f"cpu: {json.dumps(cpu)}")
+
+    head = r32 if r32 else r64
+    head_dtype = "f32" if r32 else "f64"
+    traffic = traffic_from_profiles()
+    line = {
+        "metric": "grad evals/s + % HBM peak at L=65536; batched L-BFGS minima/s, FP32 vs FP64",
+        "value": head["evals_per_s"],
+        "unit": "grad evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": head_dtype,
+        "data": "synthetic (seeded FilteredNoise table, seeded random-walk line)",
+        "config": {"workload": f"fused gradient+energy, L={L}, table {L}x{NY}, xi={XI:g}, arclength, "
+                               f"{B} evaluations per step (one batched launch)",
+                   "L": L, "noise_grid": [L, NY], "xi": XI, "evals_per_step": B},
+        "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": head["achieved_GBs"] / HBM_PEAK_GBS,
+                     "traffic": traffic.get(head_dtype) if traffic else None},
+        "cpu_baseline": cpu,
+    }
+    if r64:
+        line["f64"] = {"value": r64["evals_per_s"], "ms_per_step": r64["ms_per_step"],
+                       "roofline_achieved_GBs": r64["achieved_GBs"],
+                       "roofline_frac": r64["achieved_GBs"] / HBM_PEAK_GBS,
+                       "bytes_per_eval": r64["bytes_per_eval"]}
+    if r32:
+        line["f32_bytes_per_eval"] = r32["bytes_per_eval"]
+    if err:
+        line["fp32_vs_fp64_error"] = err
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

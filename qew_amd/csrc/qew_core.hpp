@@ -1,0 +1,421 @@
+// Device-side core of the qEW engine for gfx950: descriptors, the bicubic
+// B-spline sampler (exact FP64 restatement and an FP32 fast path), elastic
+// stencils and wave64 reductions. Included by every .hip translation unit.
+//
+// All HIP code is compiled with -ffp-contract=off: the FP64 path reproduces
+// the reference's operation order (src/filtered_noise.h:33-176,
+// src/qew_model.h:53-194) without fused multiply-adds, so per-site results are
+// bit-identical to a sequential FP64 CPU evaluation. The FP32 path asks for
+// fmaf explicitly where it wants one.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace qew_dev {
+
+constexpr int kWave = 64;
+constexpr int kTile = 256;  // sites per workgroup tile in the eval kernels
+
+enum Kind { TABLE = 0, SINCOS = 1, CONST = 2, ZERO = 3, HARMONIC = 4, WASHBOARD = 5 };
+
+// A potential as seen by a kernel. The table is stored x-contiguous,
+// coef[iy * nx + ix]: the 4 x-rows an edge needs at one y level are adjacent,
+// and neighbouring edges (adjacent lanes) read adjacent x, so a wavefront's
+// 16 gathers coalesce along x.
+struct NoiseDesc {
+    int kind;
+    int dtype;  // 0 FP64 table, 1 FP32 table
+    int nx, ny;
+    double dx, dy;
+    const void *coef;
+    double p[4];
+};
+
+// One line of a batch: sites [off, off + n) of the batch's line buffers.
+struct LineDesc {
+    int64_t off;
+    int n;
+    int noise;  // index into the batch's NoiseDesc array
+    int model;  // 0 linear, 1 arclength
+    int tile0;  // first eval tile of this line
+    int ntiles;
+    int pad;
+    double L;   // physical_size
+    double lt;  // line_tension
+    double f;   // driving_force
+};
+
+// ---------------------------------------------------------------------------
+// FP64 restatement of the reference sampler (src/filtered_noise.h:33-176).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void bspline_w(double t, double w[4]) {
+    const double t2 = t * t;
+    const double t3 = t2 * t;
+    const double omt = 1.0 - t;
+    w[0] = omt * omt * omt / 6.0;
+    w[1] = (4.0 - 6.0 * t2 + 3.0 * t3) / 6.0;
+    w[2] = (1.0 + 3.0 * t + 3.0 * t2 - 3.0 * t3) / 6.0;
+    w[3] = t3 / 6.0;
+}
+
+__device__ __forceinline__ void bspline_dw(double t, double dw[4]) {
+    const double t2 = t * t;
+    const double omt = 1.0 - t;
+    dw[0] = -omt * omt / 2.0;
+    dw[1] = (-12.0 * t + 9.0 * t2) / 6.0;
+    dw[2] = (3.0 + 6.0 * t - 9.0 * t2) / 6.0;
+    dw[3] = 3.0 * t2 / 6.0;
+}
+
+// DeviceNoise::locate (src/filtered_noise.h:77-101). Division by a spacing of
+// exactly 1 is skipped: x / 1.0 == x bit for bit.
+__device__ __forceinline__ void locate64(const NoiseDesc &nd, double x, double y, int ixn[4],
+                                         int iyn[4], double &fx, double &fy) {
+    const int nx = nd.nx, ny = nd.ny;
+    double gx = (nd.dx == 1.0) ? x : x / nd.dx;
+    double gy = (nd.dy == 1.0) ? y : y / nd.dy;
+    gx -= floor(gx / (double)nx) * (double)nx;
+    gy -= floor(gy / (double)ny) * (double)ny;
+    int ix = (int)floor(gx);
+    int iy = (int)floor(gy);
+    if (ix >= nx) { ix -= nx; gx -= (double)nx; }
+    if (iy >= ny) { iy -= ny; gy -= (double)ny; }
+    fx = gx - (double)ix;
+    fy = gy - (double)iy;
+    ixn[0] = (ix == 0) ? nx - 1 : ix - 1;
+    ixn[1] = ix;
+    ixn[2] = (ix + 1 == nx) ? 0 : ix + 1;
+    ixn[3] = (ixn[2] + 1 == nx) ? 0 : ixn[2] + 1;
+    iyn[0] = (iy == 0) ? ny - 1 : iy - 1;
+    iyn[1] = iy;
+    iyn[2] = (iy + 1 == ny) ? 0 : iy + 1;
+    iyn[3] = (iyn[2] + 1 == ny) ? 0 : iyn[2] + 1;
+}
+
+template <typename C>
+__device__ __forceinline__ double coef_at(const NoiseDesc &nd, int ix, int iy) {
+    return (double)static_cast<const C *>(nd.coef)[(size_t)iy * nd.nx + ix];
+}
+
+// Analytic potentials: the reference tests' functors, evaluated with the
+// device math library (ocml).
+__device__ inline void analytic_sample(const NoiseDesc &nd, double x, double y, double &v,
+                                       double &dvdx, double &dvdy) {
+    const double *p = nd.p;
+    switch (nd.kind) {
+    case SINCOS:
+        v = p[0] * sin(p[1] * x) * cos(p[2] * y);
+        dvdx = p[0] * p[1] * cos(p[1] * x) * cos(p[2] * y);
+        dvdy = -p[0] * p[2] * sin(p[1] * x) * sin(p[2] * y);
+        return;
+    case CONST:
+        v = p[0]; dvdx = 0; dvdy = 0;
+        return;
+    case HARMONIC: {
+        const double d = y - p[1];
+        v = 0.5 * p[0] * d * d; dvdx = 0; dvdy = p[0] * d;
+        return;
+    }
+    case WASHBOARD:
+        v = p[0] * cos(p[1] * y); dvdx = 0; dvdy = -p[0] * p[1] * sin(p[1] * y);
+        return;
+    default:
+        v = 0; dvdx = 0; dvdy = 0;
+        return;
+    }
+}
+
+// V and dV/dy at one point from one 4x4 gather: the same arithmetic as
+// sample_value (src/filtered_noise.h:138-155) and sample_dy (:159-176).
+template <typename C>
+__device__ __forceinline__ void sample_v_dy64(const NoiseDesc &nd, double x, double y,
+                                              double &v, double &dvdy) {
+    if (nd.kind != TABLE) {
+        double dvdx;
+        analytic_sample(nd, x, y, v, dvdx, dvdy);
+        return;
+    }
+    int ixn[4], iyn[4];
+    double fx, fy;
+    locate64(nd, x, y, ixn, iyn, fx, fy);
+    double wx[4], wy[4], dwy[4];
+    bspline_w(fx, wx);
+    bspline_w(fy, wy);
+    bspline_dw(fy, dwy);
+    double c[4][4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int a = 0; a < 4; ++a) c[a][b] = coef_at<C>(nd, ixn[a], iyn[b]);
+    double vv = 0, dd = 0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        double row = 0, row_dy = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            row += c[a][b] * wy[b];
+            row_dy += c[a][b] * dwy[b];
+        }
+        vv += wx[a] * row;
+        dd += wx[a] * row_dy;
+    }
+    v = vv;
+    dvdy = (nd.dy == 1.0) ? dd : dd / nd.dy;
+}
+
+template <typename C>
+__device__ __forceinline__ double sample_dy64(const NoiseDesc &nd, double x, double y) {
+    if (nd.kind != TABLE) {
+        double v, dvdx, dvdy;
+        analytic_sample(nd, x, y, v, dvdx, dvdy);
+        return dvdy;
+    }
+    int ixn[4], iyn[4];
+    double fx, fy;
+    locate64(nd, x, y, ixn, iyn, fx, fy);
+    double wx[4], dwy[4];
+    bspline_w(fx, wx);
+    bspline_dw(fy, dwy);
+    double c[4][4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int a = 0; a < 4; ++a) c[a][b] = coef_at<C>(nd, ixn[a], iyn[b]);
+    double d = 0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        double row_dy = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) row_dy += c[a][b] * dwy[b];
+        d += wx[a] * row_dy;
+    }
+    return (nd.dy == 1.0) ? d : d / nd.dy;
+}
+
+template <typename C>
+__device__ __forceinline__ double sample_v64(const NoiseDesc &nd, double x, double y) {
+    double v, dvdy;
+    if (nd.kind != TABLE) {
+        double dvdx;
+        analytic_sample(nd, x, y, v, dvdx, dvdy);
+        return v;
+    }
+    int ixn[4], iyn[4];
+    double fx, fy;
+    locate64(nd, x, y, ixn, iyn, fx, fy);
+    double wx[4], wy[4];
+    bspline_w(fx, wx);
+    bspline_w(fy, wy);
+    double vv = 0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        double row = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) row += coef_at<C>(nd, ixn[a], iyn[b]) * wy[b];
+        vv += wx[a] * row;
+    }
+    return vv;
+}
+
+// Full sample with both derivatives (src/filtered_noise.h:103-134).
+template <typename C>
+__device__ inline void sample_full64(const NoiseDesc &nd, double x, double y, double &v,
+                                     double &dvdx, double &dvdy) {
+    if (nd.kind != TABLE) {
+        analytic_sample(nd, x, y, v, dvdx, dvdy);
+        return;
+    }
+    int ixn[4], iyn[4];
+    double fx, fy;
+    locate64(nd, x, y, ixn, iyn, fx, fy);
+    double wx[4], dwx[4], wy[4], dwy[4];
+    bspline_w(fx, wx);
+    bspline_dw(fx, dwx);
+    bspline_w(fy, wy);
+    bspline_dw(fy, dwy);
+    double vv = 0, dgx = 0, dgy = 0;
+    for (int a = 0; a < 4; ++a) {
+        double row = 0, row_dy = 0;
+        for (int b = 0; b < 4; ++b) {
+            const double c = coef_at<C>(nd, ixn[a], iyn[b]);
+            row += c * wy[b];
+            row_dy += c * dwy[b];
+        }
+        vv += wx[a] * row;
+        dgx += dwx[a] * row;
+        dgy += wx[a] * row_dy;
+    }
+    v = vv;
+    dvdx = dgx / nd.dx;
+    dvdy = dgy / nd.dy;
+}
+
+// ---------------------------------------------------------------------------
+// FP32 fast sampler. The line height enters as an FP32 offset from a per-line
+// anchor; the anchor's cell index is exact (integer) and only the small
+// in-band offset is FP32, so the fraction keeps ~1e-7 cell resolution
+// regardless of where the line sits in the table.
+// ---------------------------------------------------------------------------
+struct YAnchor {
+    int cell;    // floor(anchor / dy) reduced into [0, ny)
+    float frac;  // anchor / dy - floor(anchor / dy)
+};
+
+__device__ __forceinline__ YAnchor make_anchor(double anchor, double dy, int ny) {
+    const double g = anchor / dy;
+    const double fl = floor(g);
+    YAnchor a;
+    double c = fl - floor(fl / (double)ny) * (double)ny;
+    int ci = (int)c;
+    if (ci >= ny) ci -= ny;
+    a.cell = ci;
+    a.frac = (float)(g - fl);
+    return a;
+}
+
+__device__ __forceinline__ void bspline_wf(float t, float w[4]) {
+    const float t2 = t * t, t3 = t2 * t, omt = 1.0f - t;
+    const float s = 1.0f / 6.0f;
+    w[0] = omt * omt * omt * s;
+    w[1] = __builtin_fmaf(3.0f, t3, __builtin_fmaf(-6.0f, t2, 4.0f)) * s;
+    w[2] = __builtin_fmaf(-3.0f, t3, __builtin_fmaf(3.0f, t2, __builtin_fmaf(3.0f, t, 1.0f))) * s;
+    w[3] = t3 * s;
+}
+
+__device__ __forceinline__ void bspline_dwf(float t, float dw[4]) {
+    const float t2 = t * t, omt = 1.0f - t;
+    dw[0] = -0.5f * omt * omt;
+    dw[1] = __builtin_fmaf(1.5f, t2, -2.0f * t);
+    dw[2] = __builtin_fmaf(-1.5f, t2, __builtin_fmaf(1.0f, t, 0.5f));
+    dw[3] = 0.5f * t2;
+}
+
+// ---------------------------------------------------------------------------
+// wave64 / workgroup reductions
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const T u = __shfl_xor(v, o, kWave);
+        v = (u > v) ? u : v;
+    }
+    return v;
+}
+
+// Error-free sum of two floats (Knuth TwoSum): s + e == a + b exactly.
+__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
+    s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+// Compensated pair (sum, correction) combine.
+struct KSum {
+    float s, c;
+};
+
+__device__ __forceinline__ KSum ksum_add(KSum x, KSum y) {
+    float s, e;
+    two_sum(x.s, y.s, s, e);
+    KSum r;
+    const float c = x.c + y.c + e;
+    two_sum(s, c, r.s, r.c);
+    return r;
+}
+
+__device__ __forceinline__ KSum wave_ksum(KSum v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        KSum u;
+        u.s = __shfl_xor(v.s, o, kWave);
+        u.c = __shfl_xor(v.c, o, kWave);
+        v = ksum_add(v, u);
+    }
+    return v;
+}
+
+// Sum over a workgroup of blockDim.x threads (a multiple of 64, <= 1024).
+// Every thread gets the result. `red` is LDS scratch of >= 16 entries.
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T *red) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    v = wave_sum(v);
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    T r = 0;
+    for (int w = 0; w < nw; ++w) r += red[w];
+    return r;
+}
+
+template <typename T>
+__device__ __forceinline__ T block_max(T v, T *red) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    v = wave_max(v);
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    T r = red[0];
+    for (int w = 1; w < nw; ++w) r = (red[w] > r) ? red[w] : r;
+    return r;
+}
+
+__device__ __forceinline__ KSum block_ksum(KSum v, KSum *red) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    v = wave_ksum(v);
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    KSum r = red[0];
+    for (int w = 1; w < nw; ++w) r = ksum_add(r, red[w]);
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) and the per-(seed, step,
+// site) standard normal: counter = (site/2, step_lo, step_hi, 0), key = seed,
+// two 53-bit uniforms, Box-Muller (cos for even sites, sin for odd sites).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
+        const uint32_t lo0 = 0xD2511F53u * c[0];
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
+        const uint32_t lo1 = 0xCD9E8D57u * c[2];
+        const uint32_t n0 = hi1 ^ c[1] ^ k0;
+        const uint32_t n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+__device__ __forceinline__ double philox_normal(uint64_t seed, uint64_t step, uint64_t site) {
+    uint32_t c[4] = {(uint32_t)(site >> 1), (uint32_t)step, (uint32_t)(step >> 32), 0u};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const uint64_t k1 = ((((uint64_t)c[0]) << 32) | c[1]) >> 11;
+    const uint64_t k2 = ((((uint64_t)c[2]) << 32) | c[3]) >> 11;
+    const double u1 = (double)(k1 + 1) * 0x1.0p-53;
+    const double u2 = (double)k2 * 0x1.0p-53;
+    const double rad = sqrt(-2.0 * log(u1));
+    const double th = 6.283185307179586 * u2;
+    return (site & 1) ? rad * sin(th) : rad * cos(th);
+}
+
+}  // namespace qew_dev

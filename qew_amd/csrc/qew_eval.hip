@@ -1,0 +1,500 @@
+// Fused gradient + energy of ragged batches of lines (the qew_model.h hot
+// path: src/qew_model.h:53-201). One workgroup = one tile of 256 consecutive
+// sites of one line; each thread owns one site and its right edge, so every
+// edge's 4x4 gather runs exactly once. The left-neighbour edge comes through
+// LDS; thread 0 also evaluates the tile's left halo edge. The energy is
+// reduced per tile, and the last tile of each line to finish (completion
+// counter) sums that line's tile partials in a fixed order, so results are
+// deterministic and need no second launch.
+#include "qew_internal.hpp"
+
+using namespace qew_dev;
+
+namespace {
+
+template <typename T>
+__device__ __forceinline__ T ld_volatile(const T *p) {
+    return *reinterpret_cast<const volatile T *>(p);
+}
+
+// Last-workgroup reduction of one line's tile partials (FP64).
+__device__ void finish_line_f64(double bsum, int tile, int tile0, int ntl, int line,
+                                double *partial, unsigned *counter, double *energy,
+                                double *red) {
+    __shared__ int s_last;
+    if (threadIdx.x == 0) {
+        partial[tile] = bsum;
+        __threadfence();
+        const unsigned prev = atomicAdd(&counter[line], 1u);
+        s_last = (prev == (unsigned)ntl - 1u);
+    }
+    __syncthreads();
+    if (s_last) {
+        __threadfence();
+        double acc = 0;
+        for (int t = threadIdx.x; t < ntl; t += blockDim.x) acc += ld_volatile(&partial[tile0 + t]);
+        acc = block_sum(acc, red);
+        if (threadIdx.x == 0) {
+            energy[line] = acc;
+            counter[line] = 0u;
+        }
+    }
+}
+
+template <typename C, bool TRIAL, bool GRAD, bool ENERGY>
+__global__ __launch_bounds__(kTile) void eval64_kernel(qew_host::EvalArgs a, LineDesc one_line,
+                                                       NoiseDesc one_noise, int single) {
+    __shared__ double s_dvdy[kTile + 1];
+    __shared__ double s_slope[kTile + 1];
+    __shared__ double s_h[kTile + 1];
+    __shared__ double red[16];
+    const int tid = threadIdx.x;
+    const int tile = blockIdx.x;
+    int line;
+    LineDesc L;
+    NoiseDesc N;
+    if (single) {
+        line = 0;
+        L = one_line;
+        N = one_noise;
+        L.tile0 = 0;
+    } else {
+        line = a.tile_line[tile];
+        L = a.lines[line];
+        N = a.noises[L.noise];
+    }
+    const int n = L.n;
+    const int start = (tile - L.tile0) * kTile;
+    const double *h = static_cast<const double *>(a.h) + L.off;
+    const double *dd = TRIAL ? static_cast<const double *>(a.d) + L.off : nullptr;
+    const double alpha = a.alpha;
+    auto at = [&](int i) -> double { return TRIAL ? h[i] + alpha * dd[i] : h[i]; };
+    const double dx = L.L / (double)n;
+    const double lt = L.lt, f = L.f;
+    const bool arc = (L.model == 1);
+    const int i = start + tid;
+    const bool active = i < n;
+    double e = 0, hi = 0, hip = 0;
+    if (active) {
+        const int ip = (i + 1 == n) ? 0 : i + 1;
+        hi = at(i);
+        hip = at(ip);
+        const double xc = (double)(i + ip) * 0.5 * dx;
+        const double hc = (hip + hi) * 0.5;
+        double v = 0, dvdy = 0;
+        if (ENERGY && GRAD) sample_v_dy64<C>(N, xc, hc, v, dvdy);
+        else if (ENERGY) v = sample_v64<C>(N, xc, hc);
+        else dvdy = sample_dy64<C>(N, xc, hc);
+        const double dh = (hip - hi) / dx;
+        if (ENERGY) {
+            double le;
+            if (!arc) le = lt * dh * dh * 0.5;
+            else le = lt * (sqrt(1.0 + dh * dh) - 1.0);
+            e = dx * (le + v - f * hi);
+        }
+        if (GRAD) {
+            s_dvdy[tid + 1] = dvdy;
+            s_h[tid + 1] = hi;
+            if (arc) s_slope[tid + 1] = dh / sqrt(1.0 + dh * dh);
+            if (tid == 0) {
+                // left halo: edge (im, i)
+                const int im = (i == 0) ? n - 1 : i - 1;
+                const double hm = at(im);
+                const double xcm = (double)(im + i) * 0.5 * dx;
+                const double hcm = (hi + hm) * 0.5;
+                s_dvdy[0] = sample_dy64<C>(N, xcm, hcm);
+                s_h[0] = hm;
+                if (arc) {
+                    const double dhm = (hi - hm) / dx;
+                    s_slope[0] = dhm / sqrt(1.0 + dhm * dhm);
+                }
+            }
+        }
+    }
+    if (GRAD) {
+        __syncthreads();
+        if (active) {
+            double line_force;
+            if (!arc) {
+                const double d2h = s_h[tid] + hip - 2.0 * hi;
+                line_force = -lt * d2h / (dx * dx);
+            } else {
+                line_force = lt * (s_slope[tid] - s_slope[tid + 1]) / dx;
+            }
+            static_cast<double *>(a.grad)[L.off + i] =
+                dx * (line_force + 0.5 * (s_dvdy[tid] + s_dvdy[tid + 1]) - f);
+        }
+    }
+    if (ENERGY) {
+        const double bsum = block_sum(e, red);
+        const int ntl = single ? (int)gridDim.x : L.ntiles;
+        finish_line_f64(bsum, tile, L.tile0, ntl, line, static_cast<double *>(a.partial),
+                        a.counter, a.energy, red);
+    }
+}
+
+// FP32 fast path: FP32 line offsets from a per-line anchor, FP32 table, FP32
+// arithmetic, energy accumulated as compensated (sum, error) pairs.
+__global__ __launch_bounds__(kTile) void eval32_kernel(qew_host::EvalArgs a) {
+    __shared__ float s_dvdy[kTile + 1];
+    __shared__ float s_slope[kTile + 1];
+    __shared__ float s_h[kTile + 1];
+    __shared__ KSum red[16];
+    __shared__ int s_last;
+    const int tid = threadIdx.x;
+    const int tile = blockIdx.x;
+    const int line = a.tile_line[tile];
+    const LineDesc L = a.lines[line];
+    const NoiseDesc N = a.noises[L.noise];
+    const int n = L.n;
+    const int start = (tile - L.tile0) * kTile;
+    const float *h = static_cast<const float *>(a.h) + L.off;
+    const double anchor = a.anchor ? a.anchor[line] : 0.0;
+    const double dxl = L.L / (double)n;
+    const float dxf = (float)dxl, inv_dx = (float)(1.0 / dxl);
+    const float lt = (float)L.lt, f = (float)L.f;
+    const bool arc = (L.model == 1);
+    const float *coef = static_cast<const float *>(N.coef);
+    const int nx = N.nx, ny = N.ny;
+    const double cx = 0.5 * dxl / N.dx;
+    const float inv_dy = (float)(1.0 / N.dy);
+    const YAnchor ya = make_anchor(anchor, N.dy, ny);
+
+    auto edge = [&](int i, int ip, float hi, float hip, float &v, float &dvdy) {
+        double gx = (double)(i + ip) * cx;
+        if (gx >= (double)nx) gx -= floor(gx / (double)nx) * (double)nx;
+        int ix = (int)gx;
+        const float fx = (float)(gx - (double)ix);
+        if (ix >= nx) ix -= nx;
+        const float t = __builtin_fmaf((hi + hip) * 0.5f, inv_dy, ya.frac);
+        const float kf = floorf(t);
+        const float fy = t - kf;
+        int iy = ya.cell + (int)kf;
+        if (iy < 0 || iy >= ny) iy = ((iy % ny) + ny) % ny;
+        float wx[4], wy[4], dwy[4];
+        bspline_wf(fx, wx);
+        bspline_wf(fy, wy);
+        bspline_dwf(fy, dwy);
+        const int x0 = (ix == 0) ? nx - 1 : ix - 1;
+        const int x2 = (ix + 1 == nx) ? 0 : ix + 1;
+        const int x3 = (x2 + 1 == nx) ? 0 : x2 + 1;
+        const int y0 = (iy == 0) ? ny - 1 : iy - 1;
+        const int y2 = (iy + 1 == ny) ? 0 : iy + 1;
+        const int y3 = (y2 + 1 == ny) ? 0 : y2 + 1;
+        const int ys[4] = {y0, iy, y2, y3};
+        const int xs[4] = {x0, ix, x2, x3};
+        float c[4][4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const float *r = coef + (size_t)ys[b] * nx;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) c[q][b] = __builtin_nontemporal_load(r + xs[q]);
+        }
+        float vv = 0.f, dd = 0.f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float row = c[q][0] * wy[0], rdy = c[q][0] * dwy[0];
+#pragma unroll
+            for (int b = 1; b < 4; ++b) {
+                row = __builtin_fmaf(c[q][b], wy[b], row);
+                rdy = __builtin_fmaf(c[q][b], dwy[b], rdy);
+            }
+            vv = __builtin_fmaf(wx[q], row, vv);
+            dd = __builtin_fmaf(wx[q], rdy, dd);
+        }
+        v = vv;
+        dvdy = dd * inv_dy;
+    };
+
+    const int i = start + tid;
+    const bool active = i < n;
+    float e = 0.f, hi = 0.f, hip = 0.f;
+    if (active) {
+        const int ip = (i + 1 == n) ? 0 : i + 1;
+        hi = h[i];
+        hip = h[ip];
+        float v, dvdy;
+        edge(i, ip, hi, hip, v, dvdy);
+        const float dh = (hip - hi) * inv_dx;
+        const float s2 = dh * dh;
+        const float root = sqrtf(1.0f + s2);
+        const float le = arc ? lt * (s2 / (root + 1.0f)) : 0.5f * lt * s2;
+        e = dxf * (le + v - f * hi);
+        s_dvdy[tid + 1] = dvdy;
+        s_h[tid + 1] = hi;
+        s_slope[tid + 1] = dh / root;
+        if (tid == 0) {
+            const int im = (i == 0) ? n - 1 : i - 1;
+            const float hm = h[im];
+            float vm, dvm;
+            edge(im, i, hm, hi, vm, dvm);
+            s_dvdy[0] = dvm;
+            s_h[0] = hm;
+            const float dhm = (hi - hm) * inv_dx;
+            s_slope[0] = dhm / sqrtf(1.0f + dhm * dhm);
+        }
+    }
+    __syncthreads();
+    if (active && a.grad) {
+        float lf;
+        if (!arc) lf = -lt * ((s_h[tid] + hip) - 2.0f * hi) * (inv_dx * inv_dx);
+        else lf = lt * (s_slope[tid] - s_slope[tid + 1]) * inv_dx;
+        static_cast<float *>(a.grad)[L.off + i] =
+            dxf * (lf + 0.5f * (s_dvdy[tid] + s_dvdy[tid + 1]) - f);
+    }
+    if (a.energy) {
+        KSum k{e, 0.f};
+        k = block_ksum(k, red);
+        KSum *part = static_cast<KSum *>(a.partial);
+        if (tid == 0) {
+            part[tile] = k;
+            __threadfence();
+            const unsigned prev = atomicAdd(&a.counter[line], 1u);
+            s_last = (prev == (unsigned)L.ntiles - 1u);
+        }
+        __syncthreads();
+        if (s_last) {
+            __threadfence();
+            KSum acc{0.f, 0.f};
+            for (int t = tid; t < L.ntiles; t += blockDim.x) {
+                KSum p;
+                p.s = ld_volatile(&part[L.tile0 + t].s);
+                p.c = ld_volatile(&part[L.tile0 + t].c);
+                acc = ksum_add(acc, p);
+            }
+            acc = block_ksum(acc, red);
+            if (tid == 0) {
+                // -f * anchor * L: the anchor's share of the driving-force term
+                a.energy[line] = ((double)acc.s + (double)acc.c) - L.f * anchor * L.L;
+                a.counter[line] = 0u;
+            }
+        }
+    }
+}
+
+template <bool TRIAL, bool GRAD, bool ENERGY>
+void launch64(hipStream_t s, const qew_host::EvalArgs &a, const LineDesc *one,
+              const NoiseDesc *one_noise, int table_dtype, int grid) {
+    LineDesc l{};
+    NoiseDesc nd{};
+    if (one) {
+        l = *one;
+        nd = *one_noise;
+    }
+    const int single = one ? 1 : 0;
+    if (table_dtype == 1)
+        hipLaunchKernelGGL((eval64_kernel<float, TRIAL, GRAD, ENERGY>), dim3(grid), dim3(kTile), 0, s,
+                           a, l, nd, single);
+    else
+        hipLaunchKernelGGL((eval64_kernel<double, TRIAL, GRAD, ENERGY>), dim3(grid), dim3(kTile), 0,
+                           s, a, l, nd, single);
+}
+
+__global__ void sample_kernel(NoiseDesc nd, int m, const double *x, const double *y, double *v,
+                              double *dvdx, double *dvdy) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    double a, b, c;
+    if (nd.dtype == 1) sample_full64<float>(nd, x[k], y[k], a, b, c);
+    else sample_full64<double>(nd, x[k], y[k], a, b, c);
+    v[k] = a;
+    dvdx[k] = b;
+    dvdy[k] = c;
+}
+
+}  // namespace
+
+namespace qew_host {
+
+void launch_eval_f64(hipStream_t s, const EvalArgs &a, const LineDesc *one_line,
+                     const NoiseDesc *one_noise) {
+    // the table dtype of a batch must be uniform for the FP64 path; the batch
+    // records it in noise 0 (checked at batch creation)
+    const int dtype = one_noise ? one_noise->dtype : 0;
+    const bool trial = a.d != nullptr, grad = a.grad != nullptr, energy = a.energy != nullptr;
+    const int grid = a.ntiles;
+    if (trial) {
+        launch64<true, false, true>(s, a, one_line, one_noise, dtype, grid);
+    } else if (grad && energy) {
+        launch64<false, true, true>(s, a, one_line, one_noise, dtype, grid);
+    } else if (grad) {
+        launch64<false, true, false>(s, a, one_line, one_noise, dtype, grid);
+    } else {
+        launch64<false, false, true>(s, a, one_line, one_noise, dtype, grid);
+    }
+    QEW_HIP(hipGetLastError());
+}
+
+void launch_eval_f32(hipStream_t s, const EvalArgs &a) {
+    hipLaunchKernelGGL(eval32_kernel, dim3(a.ntiles), dim3(kTile), 0, s, a);
+    QEW_HIP(hipGetLastError());
+}
+
+void ctx_reserve_eval(qew_ctx ctx, int ntiles, int nlines) {
+    ctx->partial.ensure(sizeof(double) * (size_t)std::max(ntiles, 1));
+    const size_t cb = sizeof(unsigned) * (size_t)std::max(nlines, 1);
+    if (ctx->counter.bytes < cb) {
+        ctx->counter.ensure(cb);
+        QEW_HIP(hipMemsetAsync(ctx->counter.ptr, 0, ctx->counter.bytes, ctx->stream));
+    }
+    ctx->result.ensure(sizeof(double) * 64);
+    ctx->pinned.ensure(sizeof(double) * 64);
+}
+
+LineDesc single_line(int n, const qew_params *p) {
+    LineDesc l{};
+    l.off = 0;
+    l.n = n;
+    l.noise = 0;
+    l.model = p->model;
+    l.tile0 = 0;
+    l.ntiles = (n + kTile - 1) / kTile;
+    l.L = p->physical_size;
+    l.lt = p->line_tension;
+    l.f = p->driving_force;
+    return l;
+}
+
+}  // namespace qew_host
+
+using namespace qew_host;
+
+namespace {
+
+void check_line(qew_ctx ctx, qew_noise noise, const qew_params *p, int n) {
+    ctx_check(ctx);
+    QEW_REQUIRE(noise != nullptr, "null noise");
+    QEW_REQUIRE(p != nullptr, "null params");
+    QEW_REQUIRE(n >= 1, "line length must be >= 1");
+    QEW_REQUIRE(p->model == 0 || p->model == 1, "model must be QEW_LINEAR or QEW_ARCLENGTH");
+}
+
+double single_eval(qew_ctx ctx, qew_noise noise, const qew_params *p, int n, const double *h,
+                   const double *d, double alpha, double *grad, bool want_energy) {
+    use_device(ctx);
+    const LineDesc l = single_line(n, p);
+    ctx_reserve_eval(ctx, l.ntiles, 1);
+    EvalArgs a{};
+    a.ntiles = l.ntiles;
+    a.h = h;
+    a.d = d;
+    a.alpha = alpha;
+    a.grad = grad;
+    a.energy = want_energy ? ctx->result.as<double>() : nullptr;
+    a.partial = ctx->partial.ptr;
+    a.counter = ctx->counter.as<unsigned>();
+    launch_eval_f64(ctx->stream, a, &l, &noise->desc);
+    if (!want_energy) return 0.0;
+    QEW_HIP(hipMemcpyAsync(ctx->pinned.ptr, ctx->result.ptr, sizeof(double), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    QEW_HIP(hipStreamSynchronize(ctx->stream));
+    return *ctx->pinned.as<double>();
+}
+
+}  // namespace
+
+extern "C" {
+
+qew_status qew_objective(qew_ctx ctx, qew_noise noise, const qew_params *p, int n,
+                         const double *h, double *energy) {
+    return guard([&] {
+        check_line(ctx, noise, p, n);
+        QEW_REQUIRE(h && energy, "null pointer");
+        *energy = single_eval(ctx, noise, p, n, h, nullptr, 0.0, nullptr, true);
+    });
+}
+
+qew_status qew_objective_trial(qew_ctx ctx, qew_noise noise, const qew_params *p, int n,
+                               const double *x0, const double *d, double alpha,
+                               double *energy) {
+    return guard([&] {
+        check_line(ctx, noise, p, n);
+        QEW_REQUIRE(x0 && d && energy, "null pointer");
+        *energy = single_eval(ctx, noise, p, n, x0, d, alpha, nullptr, true);
+    });
+}
+
+qew_status qew_gradient(qew_ctx ctx, qew_noise noise, const qew_params *p, int n,
+                        const double *h, double *grad) {
+    return guard([&] {
+        check_line(ctx, noise, p, n);
+        QEW_REQUIRE(h && grad, "null pointer");
+        single_eval(ctx, noise, p, n, h, nullptr, 0.0, grad, false);
+    });
+}
+
+qew_status qew_eval(qew_ctx ctx, qew_noise noise, const qew_params *p, int n,
+                    const double *h, double *grad, double *energy) {
+    return guard([&] {
+        check_line(ctx, noise, p, n);
+        QEW_REQUIRE(h && grad, "null pointer");
+        const double e = single_eval(ctx, noise, p, n, h, nullptr, 0.0, grad, energy != nullptr);
+        if (energy) *energy = e;
+    });
+}
+
+qew_status qew_batch_eval(qew_batch b, int dtype, const void *h, const double *anchor,
+                          void *grad, double *energy) {
+    return guard([&] {
+        QEW_REQUIRE(b != nullptr, "null batch");
+        QEW_REQUIRE(h != nullptr, "null line buffer");
+        QEW_REQUIRE(grad || energy, "nothing to compute");
+        QEW_REQUIRE(dtype == QEW_F64 || dtype == QEW_F32, "dtype must be QEW_F64 or QEW_F32");
+        qew_ctx ctx = b->ctx;
+        use_device(ctx);
+        EvalArgs a{};
+        a.lines = b->d_lines.as<LineDesc>();
+        a.noises = b->d_noises.as<NoiseDesc>();
+        a.tile_line = b->d_tile_line.as<int>();
+        a.ntiles = b->ntiles;
+        a.h = h;
+        a.anchor = anchor;
+        a.grad = grad;
+        a.energy = energy;
+        a.partial = b->d_partial.ptr;
+        a.counter = b->d_counter.as<unsigned>();
+        if (dtype == QEW_F32) {
+            for (const auto &nd : b->noises)
+                QEW_REQUIRE(nd.kind == TABLE && nd.dtype == 1,
+                            "QEW_F32 evaluation needs FP32 coefficient tables");
+            launch_eval_f32(ctx->stream, a);
+        } else {
+            QEW_REQUIRE(!anchor, "anchors apply to QEW_F32 line state only");
+            NoiseDesc dtype_probe{};
+            dtype_probe.dtype = b->noises.empty() ? 0 : b->noises[0].dtype;
+            for (const auto &nd : b->noises)
+                QEW_REQUIRE(nd.kind != TABLE || nd.dtype == dtype_probe.dtype,
+                            "all tables of a batch must share one storage type");
+            // batch mode: descriptors come from device arrays; pass the dtype only
+            const bool trial = false;
+            (void)trial;
+            if (dtype_probe.dtype == 1) {
+                if (grad && energy) launch64<false, true, true>(ctx->stream, a, nullptr, nullptr, 1, a.ntiles);
+                else if (grad) launch64<false, true, false>(ctx->stream, a, nullptr, nullptr, 1, a.ntiles);
+                else launch64<false, false, true>(ctx->stream, a, nullptr, nullptr, 1, a.ntiles);
+            } else {
+                if (grad && energy) launch64<false, true, true>(ctx->stream, a, nullptr, nullptr, 0, a.ntiles);
+                else if (grad) launch64<false, true, false>(ctx->stream, a, nullptr, nullptr, 0, a.ntiles);
+                else launch64<false, false, true>(ctx->stream, a, nullptr, nullptr, 0, a.ntiles);
+            }
+            QEW_HIP(hipGetLastError());
+        }
+    });
+}
+
+qew_status qew_noise_sample(qew_ctx ctx, qew_noise noise, int m, const double *x,
+                            const double *y, double *v, double *dvdx, double *dvdy) {
+    return guard([&] {
+        ctx_check(ctx);
+        QEW_REQUIRE(noise != nullptr, "null noise");
+        QEW_REQUIRE(m >= 0, "negative point count");
+        if (m == 0) return;
+        QEW_REQUIRE(x && y && v && dvdx && dvdy, "null pointer");
+        use_device(ctx);
+        hipLaunchKernelGGL(sample_kernel, dim3((m + 255) / 256), dim3(256), 0, ctx->stream,
+                           noise->desc, m, x, y, v, dvdx, dvdy);
+        QEW_HIP(hipGetLastError());
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,176 @@
+// Host-side internals shared by the library's translation units: the opaque
+// handle types behind the C ABI, error plumbing and the stream-ordered scratch
+// buffers a context keeps for single-line calls.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/qew_amd.h"
+#include "qew_core.hpp"
+
+namespace qew_host {
+
+// Thrown inside the library, turned into a qew_status at the ABI boundary.
+struct Error : std::runtime_error {
+    qew_status code;
+    Error(qew_status c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string &msg);
+
+#define QEW_HIP(expr)                                                                  \
+    do {                                                                               \
+        hipError_t qew_e_ = (expr);                                                    \
+        if (qew_e_ != hipSuccess)                                                      \
+            throw ::qew_host::Error(                                                   \
+                qew_e_ == hipErrorOutOfMemory ? QEW_ERR_OUT_OF_MEMORY : QEW_ERR_HIP,   \
+                std::string(#expr) + ": " + hipGetErrorString(qew_e_));               \
+    } while (0)
+
+#define QEW_REQUIRE(cond, msg)                                                         \
+    do {                                                                               \
+        if (!(cond)) throw ::qew_host::Error(QEW_ERR_INVALID_ARGUMENT, (msg));        \
+    } while (0)
+
+// Runs f and maps exceptions to a status.
+template <class F>
+qew_status guard(F &&f) {
+    try {
+        f();
+        return QEW_OK;
+    } catch (const Error &e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_last_error("host allocation failed");
+        return QEW_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception &e) {
+        set_last_error(e.what());
+        return QEW_ERR_HIP;
+    }
+}
+
+// A device buffer that grows on demand (never shrinks) and frees on destroy.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t need) {
+        if (need <= bytes) return;
+        if (ptr) QEW_HIP(hipFree(ptr));
+        ptr = nullptr;
+        bytes = 0;
+        QEW_HIP(hipMalloc(&ptr, need));
+        bytes = need;
+    }
+    template <class T>
+    T *as() const { return static_cast<T *>(ptr); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+};
+
+// Pinned host staging for small results (host scalars returned by
+// objective(), dot products in the host-driven L-BFGS, ...).
+struct PinnedBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t need) {
+        if (need <= bytes) return;
+        if (ptr) QEW_HIP(hipHostFree(ptr));
+        ptr = nullptr;
+        QEW_HIP(hipHostMalloc(&ptr, need, hipHostMallocDefault));
+        bytes = need;
+    }
+    template <class T>
+    T *as() const { return static_cast<T *>(ptr); }
+    void release() {
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+};
+
+}  // namespace qew_host
+
+struct qew_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    // scratch for single-line calls: per-tile partial sums and the per-line
+    // completion counter of the last-workgroup reduction (kept zeroed)
+    qew_host::DevBuf partial, counter, result, work;
+    qew_host::PinnedBuf pinned;
+    int num_cus = 0;
+};
+
+struct qew_noise_s {
+    qew_ctx ctx = nullptr;
+    qew_dev::NoiseDesc desc{};
+    void *table = nullptr;             // device table, coef[iy * nx + ix]
+    std::vector<double> field;         // noise_on_grid (generated / from-field tables)
+};
+
+struct qew_batch_s {
+    qew_ctx ctx = nullptr;
+    int nlines = 0;
+    int64_t total = 0;
+    int ntiles = 0;
+    int max_n = 0;
+    std::vector<qew_dev::LineDesc> lines;
+    std::vector<qew_dev::NoiseDesc> noises;
+    std::vector<int64_t> offsets;
+    qew_host::DevBuf d_lines, d_noises, d_tile_line, d_partial, d_counter;
+    qew_host::DevBuf d_scratch;  // solver workspaces (grown on demand)
+};
+
+namespace qew_host {
+
+inline void ctx_check(qew_ctx ctx) { QEW_REQUIRE(ctx != nullptr, "null context"); }
+
+inline void use_device(qew_ctx ctx) { QEW_HIP(hipSetDevice(ctx->device)); }
+
+// Makes sure the context's partial/counter scratch covers `ntiles` tiles and
+// `nlines` lines. Counters are zeroed when (re)allocated; the kernels reset
+// them after use.
+void ctx_reserve_eval(qew_ctx ctx, int ntiles, int nlines);
+
+// Single-line descriptor helper.
+qew_dev::LineDesc single_line(int n, const qew_params *p);
+
+// ---- kernels exposed to other translation units ----
+// Fused gradient/energy over `nlines` lines. lines/noises: device arrays
+// (batch) or, when single != nullptr, the one line passed by value.
+struct EvalArgs {
+    const qew_dev::LineDesc *lines;
+    const qew_dev::NoiseDesc *noises;
+    const int *tile_line;
+    int ntiles;
+    const void *h;
+    const void *d;      // trial direction (objective at h + alpha d), or null
+    double alpha;
+    const double *anchor;
+    void *grad;         // null: energy only
+    double *energy;     // device, per line; null: gradient only
+    void *partial;
+    unsigned *counter;
+};
+
+void launch_eval_f64(hipStream_t s, const EvalArgs &a, const qew_dev::LineDesc *one_line,
+                     const qew_dev::NoiseDesc *one_noise);
+void launch_eval_f32(hipStream_t s, const EvalArgs &a);
+
+// Global-memory Stockham FFT (batched 1-D complex transforms of FP64 pairs).
+// data[b * bstride + k * stride] for b < batch, k < n; sign -1 forward,
+// +1 backward (unnormalised). tmp must hold as many elements as data spans.
+void fft_c2c(hipStream_t s, double2 *data, double2 *tmp, int n, int batch, int64_t stride,
+             int64_t bstride, int sign);
+
+}  // namespace qew_host

@@ -1,0 +1,271 @@
+// Overdamped Langevin step fused with the gradient (src/dynamics.h:34-64):
+//   h_i <- h_i + (-drift * dE/dh_i + diff * N(0,1))
+// One launch per step over every line of a batch; the step reads the old line
+// state and writes a second buffer (the stencil needs the old neighbours), and
+// the buffers swap between steps. The normal for (seed, step, site) comes from
+// Philox4x32-10 computed in registers: no RNG state in memory, no pool locks,
+// and the same stream on any grid shape. diff == 0 skips the draw entirely.
+#include <algorithm>
+#include <vector>
+
+#include "qew_internal.hpp"
+
+using namespace qew_dev;
+
+namespace {
+
+struct StepArgs {
+    const LineDesc *lines;
+    const NoiseDesc *noises;
+    const int *tile_line;
+    const void *h_in;
+    void *h_out;
+    const double *anchor;
+    double drift, diff;
+    uint64_t seed;  // line l uses seed + l
+    uint64_t step;
+};
+
+template <typename C>
+__global__ __launch_bounds__(kTile) void langevin64_kernel(StepArgs a, LineDesc one_line,
+                                                           NoiseDesc one_noise, int single) {
+    __shared__ double s_dvdy[kTile + 1];
+    __shared__ double s_slope[kTile + 1];
+    __shared__ double s_h[kTile + 1];
+    const int tid = threadIdx.x;
+    const int tile = blockIdx.x;
+    int line;
+    LineDesc L;
+    NoiseDesc N;
+    if (single) {
+        line = 0;
+        L = one_line;
+        N = one_noise;
+        L.tile0 = 0;
+    } else {
+        line = a.tile_line[tile];
+        L = a.lines[line];
+        N = a.noises[L.noise];
+    }
+    const int n = L.n;
+    const int i = (tile - L.tile0) * kTile + tid;
+    const bool active = i < n;
+    const double *h = static_cast<const double *>(a.h_in) + L.off;
+    const double dx = L.L / (double)n;
+    const double lt = L.lt, f = L.f;
+    const bool arc = (L.model == 1);
+    double hi = 0, hip = 0;
+    if (active) {
+        const int ip = (i + 1 == n) ? 0 : i + 1;
+        hi = h[i];
+        hip = h[ip];
+        const double xc = (double)(i + ip) * 0.5 * dx;
+        const double hc = (hip + hi) * 0.5;
+        s_dvdy[tid + 1] = sample_dy64<C>(N, xc, hc);
+        s_h[tid + 1] = hi;
+        if (arc) {
+            const double dh = (hip - hi) / dx;
+            s_slope[tid + 1] = dh / sqrt(1.0 + dh * dh);
+        }
+        if (tid == 0) {
+            const int im = (i == 0) ? n - 1 : i - 1;
+            const double hm = h[im];
+            s_dvdy[0] = sample_dy64<C>(N, (double)(im + i) * 0.5 * dx, (hi + hm) * 0.5);
+            s_h[0] = hm;
+            if (arc) {
+                const double dhm = (hi - hm) / dx;
+                s_slope[0] = dhm / sqrt(1.0 + dhm * dhm);
+            }
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    double line_force;
+    if (!arc) {
+        const double d2h = s_h[tid] + hip - 2.0 * hi;
+        line_force = -lt * d2h / (dx * dx);
+    } else {
+        line_force = lt * (s_slope[tid] - s_slope[tid + 1]) / dx;
+    }
+    const double g = dx * (line_force + 0.5 * (s_dvdy[tid] + s_dvdy[tid + 1]) - f);
+    double hn;
+    if (a.diff > 0)
+        hn = hi + (-a.drift * g + a.diff * philox_normal(a.seed + (uint64_t)line, a.step, (uint64_t)i));
+    else
+        hn = hi + -a.drift * g;
+    static_cast<double *>(a.h_out)[L.off + i] = hn;
+}
+
+// FP32 line state (offsets from the per-line anchor) on FP32 tables.
+__global__ __launch_bounds__(kTile) void langevin32_kernel(StepArgs a) {
+    __shared__ float s_dvdy[kTile + 1];
+    __shared__ float s_slope[kTile + 1];
+    __shared__ float s_h[kTile + 1];
+    const int tid = threadIdx.x;
+    const int tile = blockIdx.x;
+    const int line = a.tile_line[tile];
+    const LineDesc L = a.lines[line];
+    const NoiseDesc N = a.noises[L.noise];
+    const int n = L.n;
+    const int i = (tile - L.tile0) * kTile + tid;
+    const bool active = i < n;
+    const float *h = static_cast<const float *>(a.h_in) + L.off;
+    const double anchor = a.anchor ? a.anchor[line] : 0.0;
+    const double dxl = L.L / (double)n;
+    const float dxf = (float)dxl, inv_dx = (float)(1.0 / dxl);
+    const float lt = (float)L.lt, f = (float)L.f;
+    const bool arc = (L.model == 1);
+    const float *coef = static_cast<const float *>(N.coef);
+    const int nx = N.nx, ny = N.ny;
+    const double cx = 0.5 * dxl / N.dx;
+    const float inv_dy = (float)(1.0 / N.dy);
+    const YAnchor ya = make_anchor(anchor, N.dy, ny);
+
+    auto edge_dy = [&](int e, int ep, float he, float hep) -> float {
+        double gx = (double)(e + ep) * cx;
+        if (gx >= (double)nx) gx -= floor(gx / (double)nx) * (double)nx;
+        int ix = (int)gx;
+        const float fx = (float)(gx - (double)ix);
+        if (ix >= nx) ix -= nx;
+        const float t = __builtin_fmaf((he + hep) * 0.5f, inv_dy, ya.frac);
+        const float kf = floorf(t);
+        const float fy = t - kf;
+        int iy = ya.cell + (int)kf;
+        if (iy < 0 || iy >= ny) iy = ((iy % ny) + ny) % ny;
+        float wx[4], dwy[4];
+        bspline_wf(fx, wx);
+        bspline_dwf(fy, dwy);
+        const int x2 = (ix + 1 == nx) ? 0 : ix + 1;
+        const int xs[4] = {(ix == 0) ? nx - 1 : ix - 1, ix, x2, (x2 + 1 == nx) ? 0 : x2 + 1};
+        const int y2 = (iy + 1 == ny) ? 0 : iy + 1;
+        const int ys[4] = {(iy == 0) ? ny - 1 : iy - 1, iy, y2, (y2 + 1 == ny) ? 0 : y2 + 1};
+        float dd = 0.f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float r = 0.f;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) r = __builtin_fmaf(coef[(size_t)ys[b] * nx + xs[q]], dwy[b], r);
+            dd = __builtin_fmaf(wx[q], r, dd);
+        }
+        return dd * inv_dy;
+    };
+
+    float hi = 0.f, hip = 0.f;
+    if (active) {
+        const int ip = (i + 1 == n) ? 0 : i + 1;
+        hi = h[i];
+        hip = h[ip];
+        s_dvdy[tid + 1] = edge_dy(i, ip, hi, hip);
+        s_h[tid + 1] = hi;
+        const float dh = (hip - hi) * inv_dx;
+        s_slope[tid + 1] = dh * rsqrtf(1.0f + dh * dh);
+        if (tid == 0) {
+            const int im = (i == 0) ? n - 1 : i - 1;
+            const float hm = h[im];
+            s_dvdy[0] = edge_dy(im, i, hm, hi);
+            s_h[0] = hm;
+            const float dhm = (hi - hm) * inv_dx;
+            s_slope[0] = dhm * rsqrtf(1.0f + dhm * dhm);
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    float lf;
+    if (!arc) lf = -lt * ((s_h[tid] + hip) - 2.0f * hi) * (inv_dx * inv_dx);
+    else lf = lt * (s_slope[tid] - s_slope[tid + 1]) * inv_dx;
+    const float g = dxf * (lf + 0.5f * (s_dvdy[tid] + s_dvdy[tid + 1]) - f);
+    float hn = hi - (float)a.drift * g;
+    if (a.diff > 0)
+        hn += (float)(a.diff * philox_normal(a.seed + (uint64_t)line, a.step, (uint64_t)i));
+    static_cast<float *>(a.h_out)[L.off + i] = hn;
+}
+
+}  // namespace
+
+using namespace qew_host;
+
+extern "C" {
+
+qew_status qew_langevin_step(qew_ctx ctx, qew_noise noise, const qew_params *p, int n,
+                             double *h, double drift, double diff, uint64_t seed,
+                             uint64_t step) {
+    return guard([&] {
+        ctx_check(ctx);
+        QEW_REQUIRE(noise && p && h, "null pointer");
+        QEW_REQUIRE(n >= 1, "line length must be >= 1");
+        use_device(ctx);
+        const LineDesc l = single_line(n, p);
+        ctx->work.ensure(sizeof(double) * (size_t)n);
+        StepArgs a{};
+        a.h_in = h;
+        a.h_out = ctx->work.ptr;
+        a.drift = drift;
+        a.diff = diff;
+        a.seed = seed;
+        a.step = step;
+        if (noise->desc.dtype == 1)
+            hipLaunchKernelGGL(langevin64_kernel<float>, dim3(l.ntiles), dim3(kTile), 0, ctx->stream, a,
+                               l, noise->desc, 1);
+        else
+            hipLaunchKernelGGL(langevin64_kernel<double>, dim3(l.ntiles), dim3(kTile), 0, ctx->stream,
+                               a, l, noise->desc, 1);
+        QEW_HIP(hipGetLastError());
+        QEW_HIP(hipMemcpyAsync(h, ctx->work.ptr, sizeof(double) * (size_t)n,
+                               hipMemcpyDeviceToDevice, ctx->stream));
+    });
+}
+
+qew_status qew_batch_langevin(qew_batch b, int dtype, void *h, const double *anchor, int nsteps,
+                              const double *drift, const double *diff, uint64_t seed,
+                              uint64_t step0) {
+    return guard([&] {
+        QEW_REQUIRE(b != nullptr, "null batch");
+        QEW_REQUIRE(h && drift && diff, "null pointer");
+        QEW_REQUIRE(nsteps >= 0, "negative step count");
+        QEW_REQUIRE(dtype == QEW_F64 || dtype == QEW_F32, "dtype must be QEW_F64 or QEW_F32");
+        if (nsteps == 0) return;
+        qew_ctx ctx = b->ctx;
+        use_device(ctx);
+        const size_t es = dtype == QEW_F32 ? sizeof(float) : sizeof(double);
+        if (dtype == QEW_F32) {
+            for (const auto &nd : b->noises)
+                QEW_REQUIRE(nd.kind == TABLE && nd.dtype == 1,
+                            "QEW_F32 dynamics needs FP32 coefficient tables");
+        } else {
+            QEW_REQUIRE(!anchor, "anchors apply to QEW_F32 line state only");
+            for (const auto &nd : b->noises)
+                QEW_REQUIRE(nd.kind != TABLE || nd.dtype == b->noises[0].dtype,
+                            "all tables of a batch must share one storage type");
+        }
+        b->d_scratch.ensure(es * (size_t)b->total);
+        void *bufs[2] = {h, b->d_scratch.ptr};
+        StepArgs a{};
+        a.lines = b->d_lines.as<LineDesc>();
+        a.noises = b->d_noises.as<NoiseDesc>();
+        a.tile_line = b->d_tile_line.as<int>();
+        a.anchor = anchor;
+        a.seed = seed;
+        const int table_dtype = b->noises[0].dtype;
+        for (int k = 0; k < nsteps; ++k) {
+            a.h_in = bufs[k & 1];
+            a.h_out = bufs[(k + 1) & 1];
+            a.drift = drift[k];
+            a.diff = diff[k];
+            a.step = step0 + (uint64_t)k;
+            if (dtype == QEW_F32)
+                hipLaunchKernelGGL(langevin32_kernel, dim3(b->ntiles), dim3(kTile), 0, ctx->stream, a);
+            else if (table_dtype == 1)
+                hipLaunchKernelGGL(langevin64_kernel<float>, dim3(b->ntiles), dim3(kTile), 0,
+                                   ctx->stream, a, LineDesc{}, NoiseDesc{}, 0);
+            else
+                hipLaunchKernelGGL(langevin64_kernel<double>, dim3(b->ntiles), dim3(kTile), 0,
+                                   ctx->stream, a, LineDesc{}, NoiseDesc{}, 0);
+            QEW_HIP(hipGetLastError());
+        }
+        if (nsteps & 1)
+            QEW_HIP(hipMemcpyAsync(h, b->d_scratch.ptr, es * (size_t)b->total,
+                                   hipMemcpyDeviceToDevice, ctx->stream));
+    });
+}
+
+}  // extern "C"

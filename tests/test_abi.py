@@ -1,0 +1,52 @@
+"""CPU-side checks of the drop-in boundary: the built library loads, exports
+every entry point include/qew_amd.h declares, and the Python layer declares
+them all (no compute calls: no GPU here)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "qew_amd.h")
+LIB = os.path.join(ROOT, "qew_amd", "libqew_amd.so")
+
+
+def declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const char \*|qew_status|qew_lbfgs_params)\s*(qew_\w+)\s*\(",
+                                 text, flags=re.M)))
+
+
+def test_header_declares_entry_points():
+    names = declared()
+    assert "qew_eval" in names and "qew_batch_lbfgs" in names and "qew_rk_block" in names
+    assert len(names) >= 30
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libqew_amd.so not built")
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(LIB)
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libqew_amd.so not built")
+def test_python_layer_binds_every_declared_symbol():
+    from qew_amd import _lib
+    bound = {n for n, _ in _lib.SIGNATURES} | set(_lib._RESTYPE)
+    missing = [n for n in declared() if n not in bound]
+    assert not missing, missing
+    _lib.load()  # argtypes for every symbol resolve
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libqew_amd.so not built")
+def test_errors_without_device_are_reported_not_raised_across_abi():
+    from qew_amd import _lib
+    lib = _lib.load()
+    # a null context is rejected with QEW_ERR_INVALID_ARGUMENT and a message
+    st = lib.qew_ctx_synchronize(None)
+    assert st == _lib.QEW_ERR_INVALID_ARGUMENT
+    assert b"null context" in lib.qew_last_error()
+    p = lib.qew_lbfgs_default_params()
+    assert (p.m, p.ftol, p.max_iter, p.max_ls) == (8, 1e-6, 10000, 30)

@@ -1,0 +1,235 @@
+"""GPU parity of the solvers (HIP path through the C ABI) against the CPU
+oracle: L-BFGS (host-driven FP64 path), Langevin steps (Philox normals) and
+the Rosso-Krauth construction, plus the reference's own solver tests
+restated on the GPU (tests/test_lbfgs.cpp, test_dynamics.cpp,
+test_rosso_krauth.cpp)."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from gpu_util import dev, host, rel, torch_cuda
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def q():
+    torch_cuda()
+    import qew_amd
+    return qew_amd
+
+
+# ---- L-BFGS ------------------------------------------------------------------
+
+def test_lbfgs_golden_fixture(q):
+    g = json.load(open(os.path.join(GOLD, "lbfgs_golden.json")))
+    nx, ny = int(g["nx"]), int(g["ny"])
+    field = np.array(g["field"]["data"]).reshape(nx, ny)
+    gn = q.FilteredNoise.from_field(field, g["Lx"], g["Ly"])
+    p = q.Params(g["Lx"], g["line_tension"], 0.0, q.Model.Linear)
+    h = dev(np.array(g["h0"]["data"]))
+    r = q.lbfgs_minimize(h, p, gn, q.LbfgsParams(ftol=1e-8, precondition=True))
+    assert r.converged
+    e = q.objective(h, p, gn)
+    assert abs(e - g["e_min"]) < 1e-5 * abs(g["e_min"]) + 1e-6
+
+
+@pytest.mark.parametrize("precondition", [False, True])
+@pytest.mark.parametrize("model", [0, 1])
+def test_lbfgs_matches_oracle(q, precondition, model):
+    n, ny = 256, 64
+    field, coef = O.filtered_noise(n, ny, 2.56, 1.0, 0.3, 0.2, 0.2, 5)
+    on = O.OracleNoise.table(coef, 2.56, 1.0)
+    gn = q.FilteredNoise.from_coefficients(coef, 0.01, 1.0 / ny)
+    lt = 50.0 if precondition else 1.0
+    p_o = O.params(2.56, lt, 0.0, model)
+    p = q.Params(2.56, lt, 0.0, q.Model(model))
+    lpo = O.lbfgs_params(m=10, ftol=1e-10, max_iter=5000, precondition=precondition,
+                         precond_shift=0.5 * 0.01 * 0.3 / 0.04 if precondition else 0.0)
+    lp = q.LbfgsParams(m=10, ftol=1e-10, max_iter=5000, precondition=precondition,
+                       precond_shift=lpo.precond_shift)
+    h_ref, r_ref = O.lbfgs(np.full(n, 0.5), p_o, on, lpo)
+    h = dev(np.full(n, 0.5))
+    r = q.lbfgs_minimize(h, p, gn, lp)
+    assert r.converged == bool(r_ref.converged)
+    e_ref = O.objective(h_ref, p_o, on)
+    assert rel(q.objective(h, p, gn), e_ref) < 1e-12
+    assert np.max(np.abs(host(h) - h_ref)) < 1e-8
+
+
+def test_lbfgs_branches(q):
+    well = q.HarmonicWell(2.0, 0.5)
+    lin = lambda n: q.Params(float(n), 1.0, 0.0, q.Model.Linear)
+    # relaxes to flat / harmonic minimum (tests/test_lbfgs.cpp:67-124)
+    h = dev(0.5 + 0.3 * np.cos(4 * np.pi * np.arange(128) / 128))
+    assert q.lbfgs_minimize(h, lin(128), q.ZeroNoise(), q.LbfgsParams(ftol=1e-9)).converged
+    assert np.ptp(host(h)) < 1e-6
+    for model in (q.Model.Linear, q.Model.Arclength):
+        h = dev(np.zeros(96))
+        p = q.Params(96.0, 1.0, 0.0, model)
+        assert q.lbfgs_minimize(h, p, well, q.LbfgsParams(ftol=1e-9)).converged
+        assert np.max(np.abs(host(h) - 0.5)) < 1e-6
+    # non-convergence, curvature skip, no-history give-up (:126-188)
+    r = q.lbfgs_minimize(dev(np.zeros(96)), lin(96), well, q.LbfgsParams(ftol=1e-12, max_iter=1))
+    assert not r.converged and r.max_force >= 1e-12
+    h = dev(np.zeros(16))
+    r = q.lbfgs_minimize(h, lin(16), well, q.LbfgsParams(ftol=1e-8, max_iter=20000, curvature_eps=1e30))
+    assert r.n_curvature_skips > 0 and r.converged
+    r = q.lbfgs_minimize(dev(np.zeros(32)), lin(32), well, q.LbfgsParams(ftol=1e-9, max_ls=0))
+    assert not r.converged and r.iterations == 0 and r.n_ls_restarts == 0
+
+
+def test_lbfgs_restart_branch(q):
+    # tests/test_lbfgs.cpp:200-228 on the device-generated table
+    gn = q.FilteredNoise(64, 64, 3.2, 1.0, 2.0, 0.15, 0.15, 3)
+    p = q.Params(3.2, 0.5, 0.0, q.Model.Linear)
+    r = q.lbfgs_minimize(dev(np.full(64, 0.5)), p, gn, q.LbfgsParams(ftol=1e-6, max_iter=50000, max_ls=3))
+    assert r.n_ls_restarts > 0
+    r2 = q.lbfgs_minimize(dev(np.full(64, 0.5)), p, gn, q.LbfgsParams(ftol=1e-6, max_iter=50000))
+    assert r2.converged and r2.n_ls_restarts == 0
+
+
+# ---- Langevin ----------------------------------------------------------------
+
+def test_langevin_steps_match_oracle(q):
+    n = 1000
+    field, coef = O.filtered_noise(n, 64, 10.0, 1.0, 0.5, 0.2, 0.2, 8)
+    on = O.OracleNoise.table(coef, 10.0, 1.0)
+    gn = q.FilteredNoise.from_coefficients(coef, 0.01, 1.0 / 64)
+    for model in (0, 1):
+        p_o = O.params(10.0, 0.7, 0.2, model)
+        p = q.Params(10.0, 0.7, 0.2, q.Model(model))
+        h_ref = np.random.default_rng(3).uniform(0, 1, n)
+        h = dev(h_ref)
+        dyn = q.LangevinDynamics(n, 77)
+        for s in range(20):
+            O.langevin_step(h_ref, p_o, on, 0.003, 0.01, 77, s)
+            dyn.step(h, p, gn, 0.003, 0.01)
+        # normals use device log/sin/cos: agreement to a few ulp per step
+        assert np.max(np.abs(host(h) - h_ref)) < 1e-13
+
+
+def test_langevin_zero_diffusion_bitwise_and_seed_free(q):
+    n = 48
+    field, coef = O.filtered_noise(n, 32, 4.8, 1.0, 0.5, 0.3, 0.3, 2)
+    on = O.OracleNoise.table(coef, 4.8, 1.0)
+    gn = q.FilteredNoise.from_coefficients(coef, 0.1, 1.0 / 32)
+    p_o = O.params(4.8, 1.0, 0.3, 0)
+    p = q.Params(4.8, 1.0, 0.3, q.Model.Linear)
+    h0 = np.random.default_rng(5).uniform(-0.5, 0.5, n)
+    h1, h2, h_ref = dev(h0), dev(h0), h0.copy()
+    d1, d2 = q.LangevinDynamics(n, 111), q.LangevinDynamics(n, 999)
+    for s in range(50):
+        d1.step(h1, p, gn, 0.05, 0.0)
+        d2.step(h2, p, gn, 0.05, 0.0)
+        O.langevin_step(h_ref, p_o, on, 0.05, 0.0, 0, s)
+    np.testing.assert_array_equal(host(h1), host(h2))
+    np.testing.assert_array_equal(host(h1), h_ref)
+
+
+def test_langevin_force_shifts_equilibrium(q):
+    # tests/test_dynamics.cpp:106-130
+    k, y0, f = 4.0, 0.2, 0.6
+    h = dev(np.zeros(32))
+    dyn = q.LangevinDynamics(32, 0)
+    p = q.Params(32.0, 1.0, f, q.Model.Linear)
+    well = q.HarmonicWell(k, y0)
+    for _ in range(5000):
+        dyn.step(h, p, well, 0.05, 0.0)
+    assert np.max(np.abs(host(h) - (y0 + f / k))) < 1e-6
+
+
+def test_batch_langevin_matches_single(q):
+    torch = torch_cuda()
+    n = 512
+    field, coef = O.filtered_noise(n, 64, 5.12, 1.0, 0.5, 0.2, 0.2, 4)
+    gn = q.FilteredNoise.from_coefficients(coef, 0.01, 1.0 / 64)
+    p = q.Params(5.12, 0.9, 0.1, q.Model.Arclength)
+    h0 = np.random.default_rng(9).uniform(0, 1, n)
+    b = q.Batch([n, n], [gn, gn], [p, p])
+    hb = dev(np.concatenate([h0, h0]))
+    steps = 7
+    b.langevin(hb, [0.002] * steps, [0.01] * steps, seed=40)
+    hs = dev(h0)
+    dyn = q.LangevinDynamics(n, 40)
+    for _ in range(steps):
+        dyn.step(hs, p, gn, 0.002, 0.01)
+    np.testing.assert_array_equal(host(hb)[:n], host(hs))
+    assert np.max(np.abs(host(hb)[n:] - host(hs))) > 0  # line 1 uses seed + 1
+
+
+# ---- Rosso-Krauth --------------------------------------------------------------
+
+def test_rk_block_matches_oracle_bitwise(q):
+    for model, seed, amp, lt, f in [(0, 7, 0.5, 1.0, 0.0), (1, 29, 0.6, 1.2, 0.1), (1, 9, 0.5, 1.5, 0.3)]:
+        field, coef = O.filtered_noise(64, 64, 3.2, 1.0, amp, 0.2, 0.2, seed)
+        on = O.OracleNoise.table(coef, 3.2, 1.0)
+        gn = q.FilteredNoise.from_coefficients(coef, 0.05, 1.0 / 64)
+        rpo = O.rk_params(dstep=0.5 / 64, max_advance=1.0, root_tol=1e-11, sweep_tol=1e-9, runaway=3.0)
+        rp = q.RkParams(0.5 / 64, 1.0, 1e-11, 1e-9, 3.0)
+        h_ref = np.full(64, 0.5)
+        r_ref = O.rk_block(h_ref, O.params(3.2, lt, f, model), on, rpo)
+        h = dev(np.full(64, 0.5))
+        r = q.rk_block(h, q.Params(3.2, lt, f, q.Model(model)), gn, rp)
+        assert (r.blocked, r.runaway, r.sweeps) == (bool(r_ref.blocked), bool(r_ref.runaway), r_ref.sweeps)
+        np.testing.assert_array_equal(host(h), h_ref)
+        assert abs(r.mean_h - r_ref.mean_h) < 1e-12
+
+
+def test_rk_washboard_critical_force(q):
+    # tests/test_rosso_krauth.cpp:49-89 (10 bisection steps)
+    wash = q.SinusoidNoise(0.1, 2 * math.pi)
+    rp = q.RkParams(1 / 200, 1.0, 1e-12, 1e-11, 3.0, 100000)
+    fc = 0.2 * math.pi
+
+    def blocks(f):
+        return q.rk_block(dev(np.zeros(16)), q.Params(16.0, 1.0, f, q.Model.Linear), wash, rp).blocked
+
+    lo, hi = 0.0, 2 * fc
+    assert blocks(lo) and not blocks(hi)
+    for _ in range(10):
+        m = 0.5 * (lo + hi)
+        lo, hi = (m, hi) if blocks(m) else (lo, m)
+    assert abs(0.5 * (lo + hi) - fc) < 0.02 * fc
+    r = q.rk_block(dev(np.zeros(16)), q.Params(16.0, 1.0, 2 * fc, q.Model.Linear), wash, rp)
+    assert r.runaway and not r.blocked and r.mean_h > 3.0
+
+
+def test_rk_odd_length_rejected(q):
+    with pytest.raises(ValueError):
+        q.rk_block(dev(np.zeros(17)), q.Params(17.0, 1.0, 0.0, q.Model.Linear), q.SinusoidNoise(0.1, 6.28),
+                   q.RkParams(0.01, 1.0, 1e-10, 1e-9, 3.0))
+
+
+def test_rk_site_velocity_is_minus_gradient(q):
+    torch = torch_cuda()
+    field, coef = O.filtered_noise(64, 64, 3.2, 1.0, 0.7, 0.2, 0.2, 13)
+    gn = q.FilteredNoise.from_coefficients(coef, 0.05, 1.0 / 64)
+    for model in (q.Model.Linear, q.Model.Arclength):
+        p = q.Params(3.2, 1.3, 0.4, model)
+        h = np.random.default_rng(21).uniform(0.1, 0.9, 64)
+        g = host(q.gradient(dev(h), p, gn))
+        idx = torch.arange(64, dtype=torch.int32, device="cuda:0")
+        v = q.rk_site_velocity(idx, 64, p, gn, dev(h), dev(np.roll(h, 1)), dev(np.roll(h, -1)))
+        assert np.max(np.abs(host(v) + g)) < 1e-12
+
+
+def test_batch_rk_matches_single_lines(q):
+    field, coef = O.filtered_noise(64, 64, 3.2, 1.0, 0.5, 0.2, 0.2, 7)
+    gn = q.FilteredNoise.from_coefficients(coef, 0.05, 1.0 / 64)
+    rp = q.RkParams(0.5 / 64, 1.0, 1e-11, 1e-9, 3.0)
+    forces = [0.0, 0.05, 0.1, 5.0]
+    params = [q.Params(3.2, 1.0, f, q.Model.Linear) for f in forces]
+    b = q.Batch([64] * 4, [gn] * 4, params)
+    hb = dev(np.full(256, 0.5))
+    res = b.rk_block(hb, rp)
+    for k, f in enumerate(forces):
+        h = dev(np.full(64, 0.5))
+        r = q.rk_block(h, params[k], gn, rp)
+        assert (res[k].blocked, res[k].runaway, res[k].sweeps) == (r.blocked, r.runaway, r.sweeps)
+        np.testing.assert_array_equal(host(hb)[64 * k:64 * (k + 1)], host(h))
