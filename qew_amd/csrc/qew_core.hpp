@@ -419,3 +419,83 @@ __device__ __forceinline__ double philox_normal(uint64_t seed, uint64_t step, ui
 }
 
 }  // namespace qew_dev
+
+namespace qew_dev {
+
+// ---------------------------------------------------------------------------
+// Per-line geometry for kernels that work on one whole line per workgroup.
+// ---------------------------------------------------------------------------
+struct LineGeom {
+    int n;
+    int model;
+    double dx, lt, f, L;
+    NoiseDesc nd;
+    // FP32 path
+    YAnchor ya;
+    double anchor;
+    double cx;
+    float inv_dy, inv_dx, dxf;
+};
+
+__device__ __forceinline__ LineGeom make_geom(const LineDesc &L, const NoiseDesc &nd,
+                                              double anchor) {
+    LineGeom g;
+    g.n = L.n;
+    g.model = L.model;
+    g.dx = L.L / (double)L.n;
+    g.lt = L.lt;
+    g.f = L.f;
+    g.L = L.L;
+    g.nd = nd;
+    g.anchor = anchor;
+    g.ya = make_anchor(anchor, nd.dy, nd.ny);
+    g.cx = 0.5 * g.dx / nd.dx;
+    g.inv_dy = (float)(1.0 / nd.dy);
+    g.inv_dx = (float)(1.0 / g.dx);
+    g.dxf = (float)g.dx;
+    return g;
+}
+
+// FP32 edge (e, ep) of a line whose heights are FP32 offsets from the
+// anchor: value and/or y-derivative of the potential at the edge midpoint.
+template <bool WANT_V, bool WANT_DY>
+__device__ __forceinline__ void edge32(const LineGeom &G, int e, int ep, float he, float hep,
+                                       float &v, float &dvdy) {
+    const int nx = G.nd.nx, ny = G.nd.ny;
+    double gx = (double)(e + ep) * G.cx;
+    if (gx >= (double)nx) gx -= floor(gx / (double)nx) * (double)nx;
+    int ix = (int)gx;
+    const float fx = (float)(gx - (double)ix);
+    if (ix >= nx) ix -= nx;
+    const float t = __builtin_fmaf((he + hep) * 0.5f, G.inv_dy, G.ya.frac);
+    const float kf = floorf(t);
+    const float fy = t - kf;
+    int iy = G.ya.cell + (int)kf;
+    if (iy < 0 || iy >= ny) iy = ((iy % ny) + ny) % ny;
+    float wx[4], wy[4], dwy[4];
+    bspline_wf(fx, wx);
+    if (WANT_V) bspline_wf(fy, wy);
+    if (WANT_DY) bspline_dwf(fy, dwy);
+    const int x2 = (ix + 1 == nx) ? 0 : ix + 1;
+    const int xs[4] = {(ix == 0) ? nx - 1 : ix - 1, ix, x2, (x2 + 1 == nx) ? 0 : x2 + 1};
+    const int y2 = (iy + 1 == ny) ? 0 : iy + 1;
+    const int ys[4] = {(iy == 0) ? ny - 1 : iy - 1, iy, y2, (y2 + 1 == ny) ? 0 : y2 + 1};
+    const float *coef = static_cast<const float *>(G.nd.coef);
+    float vv = 0.f, dd = 0.f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        float r = 0.f, rd = 0.f;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const float c = coef[(size_t)ys[b] * nx + xs[q]];
+            if (WANT_V) r = __builtin_fmaf(c, wy[b], r);
+            if (WANT_DY) rd = __builtin_fmaf(c, dwy[b], rd);
+        }
+        if (WANT_V) vv = __builtin_fmaf(wx[q], r, vv);
+        if (WANT_DY) dd = __builtin_fmaf(wx[q], rd, dd);
+    }
+    v = vv;
+    dvdy = dd * G.inv_dy;
+}
+
+}  // namespace qew_dev

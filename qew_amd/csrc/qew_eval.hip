@@ -188,7 +188,7 @@ __global__ __launch_bounds__(kTile) void eval32_kernel(qew_host::EvalArgs a) {
         for (int b = 0; b < 4; ++b) {
             const float *r = coef + (size_t)ys[b] * nx;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) c[q][b] = __builtin_nontemporal_load(r + xs[q]);
+            for (int q = 0; q < 4; ++q) c[q][b] = r[xs[q]];
         }
         float vv = 0.f, dd = 0.f;
 #pragma unroll

@@ -1,0 +1,106 @@
+"""GPU checks of the persistent batched L-BFGS (one workgroup per line,
+device-side control flow) against the CPU oracle's sequential lbfgs_minimize.
+
+FP64 mode: same algorithm and arithmetic as the oracle, with workgroup tree
+sums instead of sequential sums, so every converged line must reach the
+oracle's minimum: energy within 1e-12 relative, heights within 1e-6.
+FP32 mode (FP32 line offsets and tables, Kahan dots, FP64 Armijo and curvature
+pair): stated tolerance 1e-6 relative energy against the FP64 oracle minimum.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from gpu_util import dev, host, rel, torch_cuda
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def q():
+    torch_cuda()
+    import qew_amd
+    return qew_amd
+
+
+def make_problem(q, lengths, ny, xis, models, seeds, dtype):
+    noises, oracle, params, h0 = [], [], [], []
+    for n, xi, model, seed in zip(lengths, xis, models, seeds):
+        field, coef = O.filtered_noise(n, ny, float(n), float(ny), 1.0, xi, xi, seed)
+        oracle.append((O.OracleNoise.table(coef, float(n), float(ny)),
+                       O.params(float(n), 1.0, 0.0, model)))
+        noises.append(q.FilteredNoise.from_coefficients(coef, 1.0, 1.0, dtype=dtype))
+        params.append(q.Params(float(n), 1.0, 0.0, q.Model(model)))
+        h0.append(np.full(n, ny / 2.0))
+    return noises, oracle, params, h0
+
+
+@pytest.mark.parametrize("precondition", [True, False])
+def test_batch_lbfgs_fp64_matches_oracle(q, precondition):
+    lengths = [256, 64, 512, 128]
+    xis = [2.0, 4.0, 8.0, 4.0]
+    models = [0, 1, 0, 1]
+    noises, oracle, params, h0 = make_problem(q, lengths, 128, xis, models, [1, 2, 3, 4], q.F64)
+    lp_kw = dict(m=10, ftol=1e-8, max_iter=4000, precondition=precondition)
+    b = q.Batch(lengths, noises, params)
+    h = dev(np.concatenate(h0))
+    res = b.lbfgs(h, q.LbfgsParams(**lp_kw, precond_shift=0.5 / 16.0), dtype=q.F64)
+    hh = host(h)
+    for k, n in enumerate(lengths):
+        on, po = oracle[k]
+        h_ref, r_ref = O.lbfgs(h0[k], po, on, O.lbfgs_params(**lp_kw, precond_shift=0.5 / 16.0))
+        assert res[k].converged and r_ref.converged, (k, res[k], r_ref.iterations)
+        line = hh[b.offsets[k]:b.offsets[k + 1]]
+        e = O.objective(line, po, on)
+        e_ref = O.objective(h_ref, po, on)
+        assert rel(e, e_ref) < 1e-12, (k, e, e_ref)
+        assert np.max(np.abs(line - h_ref)) < 1e-6
+
+
+def test_batch_lbfgs_fp32_energy_error(q):
+    lengths = [1024, 2048, 512, 1024, 4096, 256]
+    xis = [2.0, 4.0, 8.0, 4.0, 8.0, 2.0]
+    models = [0, 1, 0, 1, 1, 0]
+    noises, oracle, params, h0 = make_problem(q, lengths, 256, xis, models, [5, 6, 7, 8, 9, 10], q.F32)
+    b = q.Batch(lengths, noises, params)
+    h = dev(np.concatenate(h0))
+    res = b.lbfgs(h, q.LbfgsParams(m=10, ftol=1e-5, max_iter=4000, precondition=True,
+                                   precond_shift=0.5 / 16.0), dtype=q.F32)
+    hh = host(h)
+    for k, n in enumerate(lengths):
+        on, po = oracle[k]
+        h_ref, r_ref = O.lbfgs(h0[k], po, on, O.lbfgs_params(m=10, ftol=1e-8, max_iter=4000,
+                                                             precondition=True,
+                                                             precond_shift=0.5 / 16.0))
+        line = hh[b.offsets[k]:b.offsets[k + 1]]
+        e = O.objective(line, po, on)
+        e_ref = O.objective(h_ref, po, on)
+        assert rel(e, e_ref) < 1e-6, (k, res[k], e, e_ref)
+
+
+def test_batch_lbfgs_harmonic_and_branches(q):
+    # tests/test_lbfgs.cpp:87-188 restated on the batched kernel (analytic
+    # potentials, FP64)
+    well = q.HarmonicWell(2.0, 0.5)
+    p = q.Params(96.0, 1.0, 0.0, q.Model.Arclength)
+    b = q.Batch([96, 96], [well, well], [p, q.Params(96.0, 1.0, 0.0, q.Model.Linear)])
+    h = dev(np.zeros(192))
+    res = b.lbfgs(h, q.LbfgsParams(ftol=1e-9))
+    assert all(r.converged for r in res)
+    assert np.max(np.abs(host(h) - 0.5)) < 1e-6
+    # max_ls = 0: gives up at iteration 0 without history
+    h = dev(np.zeros(192))
+    res = b.lbfgs(h, q.LbfgsParams(ftol=1e-9, max_ls=0))
+    assert all((not r.converged) and r.iterations == 0 and r.n_ls_restarts == 0 for r in res)
+    # curvature pairs all rejected: still converges
+    h = dev(np.zeros(192))
+    res = b.lbfgs(h, q.LbfgsParams(ftol=1e-8, max_iter=20000, curvature_eps=1e30))
+    assert all(r.converged and r.n_curvature_skips > 0 for r in res)
+
+
+def test_batch_lbfgs_rejects_bad_lengths_for_fft(q):
+    well = q.HarmonicWell(2.0, 0.5)
+    p = q.Params(100.0, 1.0, 0.0, q.Model.Linear)
+    b = q.Batch([100], [well], [p])
+    with pytest.raises(ValueError):
+        b.lbfgs(dev(np.zeros(100)), q.LbfgsParams(precondition=True))

@@ -49,17 +49,21 @@ def test_lbfgs_matches_oracle(q, precondition, model):
     lt = 50.0 if precondition else 1.0
     p_o = O.params(2.56, lt, 0.0, model)
     p = q.Params(2.56, lt, 0.0, q.Model(model))
-    lpo = O.lbfgs_params(m=10, ftol=1e-10, max_iter=5000, precondition=precondition,
+    # ftol 1e-8 stays above the Armijo roundoff floor of this problem, so both
+    # sides converge; the minima then agree to the energy's rounding and to
+    # ftol / (smallest curvature) in position
+    lpo = O.lbfgs_params(m=10, ftol=1e-8, max_iter=5000, precondition=precondition,
                          precond_shift=0.5 * 0.01 * 0.3 / 0.04 if precondition else 0.0)
-    lp = q.LbfgsParams(m=10, ftol=1e-10, max_iter=5000, precondition=precondition,
+    lp = q.LbfgsParams(m=10, ftol=1e-8, max_iter=5000, precondition=precondition,
                        precond_shift=lpo.precond_shift)
     h_ref, r_ref = O.lbfgs(np.full(n, 0.5), p_o, on, lpo)
     h = dev(np.full(n, 0.5))
     r = q.lbfgs_minimize(h, p, gn, lp)
-    assert r.converged == bool(r_ref.converged)
+    assert r.converged and r_ref.converged
+    assert abs(r.iterations - r_ref.iterations) <= max(2, r_ref.iterations // 10)
     e_ref = O.objective(h_ref, p_o, on)
     assert rel(q.objective(h, p, gn), e_ref) < 1e-12
-    assert np.max(np.abs(host(h) - h_ref)) < 1e-8
+    assert np.max(np.abs(host(h) - h_ref)) < 1e-6
 
 
 def test_lbfgs_branches(q):
@@ -118,7 +122,7 @@ def test_langevin_zero_diffusion_bitwise_and_seed_free(q):
     n = 48
     field, coef = O.filtered_noise(n, 32, 4.8, 1.0, 0.5, 0.3, 0.3, 2)
     on = O.OracleNoise.table(coef, 4.8, 1.0)
-    gn = q.FilteredNoise.from_coefficients(coef, 0.1, 1.0 / 32)
+    gn = q.FilteredNoise.from_coefficients(coef, 4.8 / n, 1.0 / 32)
     p_o = O.params(4.8, 1.0, 0.3, 0)
     p = q.Params(4.8, 1.0, 0.3, q.Model.Linear)
     h0 = np.random.default_rng(5).uniform(-0.5, 0.5, n)
