@@ -1,0 +1,249 @@
+// Run drivers behind the command-line tools: the reference's static sweep,
+// annealed dynamics and force ramp (src/qew_runs.h:45-245), plus the f_c
+// bisection the depinning workload needs (absent from the reference drivers;
+// its only bisection is tests/test_rosso_krauth.cpp:76-88).
+//
+// Kept identical to the reference because the outputs must match:
+//   * line tension from lambda_p / xi: Gamma = A (lambda_p/xi)^{3/2}
+//     (src/qew_runs.h:77,143-144,210-211);
+//   * flat start h = Ly / 2;
+//   * static: preconditioned L-BFGS with shift 0.5 dx A / xi^2 (:87-91);
+//   * dynamic: dt = min(0.1, tau/10 min(fac, fac^2)), fac = xi / max(tl_start,
+//     tl_end) (:139-142); drift tl dt/(tau A), diffusion tl sqrt(2 dt/tau)
+//     (:165-166); tl updated after each step from the pre-increment time
+//     (:167-170); snapshots at 10 dt, 100 dt, ... (:171-174); Langevin seed is
+//     seed + 1 (:154);
+//   * ramp: f = k f_step with an integer k, warm start from the previous
+//     configuration, stop at the first non-blocked force (:224-241).
+#ifndef QEW_RUNS_HPP
+#define QEW_RUNS_HPP
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "qew.hpp"
+
+namespace qew {
+
+struct StaticConfig {
+    Model model = Model::Arclength;
+    real_t amplitude = 1.0;
+    real_t xi = 0.1;
+    real_t Lx = 32.0;
+    real_t Ly = 1.0;
+    int nx = 4096;
+    int ny = 512;
+    real_t driving_force = 0.0;
+    std::vector<double> pinning_lengths_over_xi = {0.01, 0.1, 1.0, 10.0, 100.0};
+    real_t ftol = 1e-6;
+    int max_iter = 5000;
+    int m = 8;  // history length (the reference fixes the LbfgsParams default)
+};
+
+struct StaticProfile {
+    double pinning_length_over_xi = 0;
+    real_t line_tension = 0;
+    std::vector<double> h;
+    LbfgsResult result;
+};
+
+inline real_t larkin_line_tension(double lp_over_xi, real_t amplitude) {
+    return std::pow(lp_over_xi, static_cast<real_t>(1.5)) * amplitude;
+}
+
+template <class Noise>
+std::vector<StaticProfile> static_sweep(const StaticConfig &c, const Noise &noise) {
+    std::vector<StaticProfile> out;
+    for (double lp : c.pinning_lengths_over_xi) {
+        Params p;
+        p.physical_size = c.Lx;
+        p.line_tension = larkin_line_tension(lp, c.amplitude);
+        p.driving_force = c.driving_force;
+        p.model = c.model;
+        View1D h("h", c.nx);
+        deep_copy(h, static_cast<real_t>(0.5) * c.Ly);
+        LbfgsParams opt;
+        opt.m = c.m;
+        opt.ftol = c.ftol;
+        opt.max_iter = c.max_iter;
+        opt.precondition = true;
+        opt.precond_shift = static_cast<real_t>(0.5) * (c.Lx / c.nx) * c.amplitude / (c.xi * c.xi);
+        StaticProfile prof;
+        prof.result = lbfgs_minimize(h, p, noise, opt);
+        prof.pinning_length_over_xi = lp;
+        prof.line_tension = p.line_tension;
+        prof.h = to_host_vec(h);
+        out.push_back(std::move(prof));
+    }
+    return out;
+}
+
+struct DynamicConfig {
+    Model model = Model::Arclength;
+    real_t amplitude = 1.0;
+    real_t xi = 0.1;
+    real_t pinning_length_over_xi = 0.1;
+    real_t driving_force = 0.0;
+    real_t Lx = 2.0;
+    real_t Ly = 1.0;
+    int nx = 2048;
+    int ny = 1024;
+    std::uint64_t seed = 1;
+    real_t tau = 1.0;
+    real_t tl_start = 0.01;
+    real_t tl_end = 0.0001;
+    real_t total_time = 1000.0;
+    real_t dt = 0;  // > 0 overrides the timestep heuristic (no reference counterpart)
+};
+
+struct DynamicSnapshot {
+    real_t t = 0;
+    std::vector<double> h;
+};
+
+struct DynamicResult {
+    real_t dt = 0;
+    real_t total_time = 0;
+    real_t line_tension = 0;
+    std::vector<DynamicSnapshot> snapshots;
+};
+
+inline real_t dynamic_timestep(const DynamicConfig &c) {
+    if (c.dt > 0) return c.dt;
+    const real_t fac = c.xi / std::max(c.tl_start, c.tl_end);
+    return std::min(static_cast<real_t>(0.1), c.tau / 10 * std::min(fac, fac * fac));
+}
+
+template <class Noise>
+DynamicResult dynamic_anneal(const DynamicConfig &c, const Noise &noise) {
+    const real_t tau = c.tau;
+    const real_t dt = dynamic_timestep(c);
+    const real_t line_tension = larkin_line_tension(c.pinning_length_over_xi, c.amplitude);
+    Params p;
+    p.physical_size = c.Lx;
+    p.line_tension = line_tension;
+    p.driving_force = c.driving_force;
+    p.model = c.model;
+    View1D h("h", c.nx);
+    deep_copy(h, static_cast<real_t>(0.5) * c.Ly);
+    LangevinDynamics dyn(c.nx, c.seed + 1);
+    const DeviceNoise dn = as_device(noise);
+    DynamicResult res;
+    res.dt = dt;
+    res.total_time = c.total_time;
+    res.line_tension = line_tension;
+    real_t t = 0;
+    real_t next_save = 10 * dt;
+    real_t tl = c.tl_start;
+    while (t < c.total_time) {
+        const real_t drift = tl * dt / (tau * c.amplitude);
+        const real_t diff = tl * std::sqrt(2 * dt / tau);
+        dyn.step(h, p, dn, drift, diff);
+        tl = c.tl_start + (c.tl_end - c.tl_start) * (t / c.total_time);
+        t += dt;
+        if (t > next_save - dt / 2) {
+            res.snapshots.push_back(DynamicSnapshot{t, to_host_vec(h)});
+            next_save *= 10;
+        }
+    }
+    return res;
+}
+
+struct DepinningConfig {
+    Model model = Model::Arclength;
+    real_t amplitude = 1.0;
+    real_t xi = 0.1;
+    real_t pinning_length_over_xi = 5.0;
+    real_t Lx = 8.0;
+    real_t Ly = 1.0;
+    int nx = 1024;
+    int ny = 256;
+    real_t f_step = 1.0;
+    real_t f_max = 40.0;
+    RkParams rp;
+    // > 0: after the ramp brackets f_c, bisect [last blocked, first unblocked]
+    // to this relative width, warm-starting every probe from the highest
+    // blocked configuration found so far (no-passing keeps it below the
+    // critical line). No reference counterpart.
+    real_t fc_rel_tol = 0;
+};
+
+struct DepinningStep {
+    real_t f = 0;
+    RkResult result;
+    std::vector<double> h;
+};
+
+struct DepinningResult {
+    real_t line_tension = 0;
+    real_t f_c_upper_bound = -1;
+    std::vector<DepinningStep> steps;
+    // bisection (fc_rel_tol > 0 and a bracket was found)
+    real_t f_c = -1;
+    real_t f_c_lower = -1;
+    real_t f_c_upper = -1;
+    int bisection_probes = 0;
+    std::vector<double> h_critical;  // blocked configuration at f_c_lower
+};
+
+template <class Noise>
+DepinningResult depinning_ramp(const DepinningConfig &c, const Noise &noise) {
+    const real_t line_tension = larkin_line_tension(c.pinning_length_over_xi, c.amplitude);
+    Params p;
+    p.physical_size = c.Lx;
+    p.line_tension = line_tension;
+    p.model = c.model;
+    View1D h("h", c.nx);
+    deep_copy(h, static_cast<real_t>(0.5) * c.Ly);
+    const DeviceNoise dn = as_device(noise);
+    DepinningResult res;
+    res.line_tension = line_tension;
+    std::vector<double> last_blocked;
+    real_t f_lo = -1;
+    for (int k = 0; static_cast<real_t>(k) * c.f_step <= c.f_max; ++k) {
+        const real_t f = static_cast<real_t>(k) * c.f_step;
+        p.driving_force = f;
+        const RkResult r = rk_block(h, p, dn, c.rp);
+        DepinningStep st;
+        st.f = f;
+        st.result = r;
+        if (r.blocked) {
+            st.h = to_host_vec(h);
+            last_blocked = st.h;
+            f_lo = f;
+        } else {
+            res.f_c_upper_bound = f;
+        }
+        res.steps.push_back(std::move(st));
+        if (!r.blocked) break;
+    }
+    if (c.fc_rel_tol > 0 && res.f_c_upper_bound >= 0 && f_lo >= 0) {
+        real_t lo = f_lo, hi = res.f_c_upper_bound;
+        std::vector<double> h_lo = last_blocked;
+        while (hi - lo > c.fc_rel_tol * std::max(std::abs(hi), std::abs(lo))) {
+            const real_t mid = 0.5 * (lo + hi);
+            if (!(mid > lo && mid < hi)) break;  // bracket at FP64 resolution
+            deep_copy(h, h_lo);
+            p.driving_force = mid;
+            const RkResult r = rk_block(h, p, dn, c.rp);
+            ++res.bisection_probes;
+            if (r.blocked) {
+                lo = mid;
+                h_lo = to_host_vec(h);
+            } else {
+                hi = mid;
+            }
+        }
+        res.f_c_lower = lo;
+        res.f_c_upper = hi;
+        res.f_c = 0.5 * (lo + hi);
+        res.h_critical = h_lo;
+    }
+    return res;
+}
+
+}  // namespace qew
+
+#endif  // QEW_RUNS_HPP

@@ -2,42 +2,44 @@
 // path: src/qew_model.h:53-201). One workgroup = one tile of 256 consecutive
 // sites of one line; each thread owns one site and its right edge, so every
 // edge's 4x4 gather runs exactly once. The left-neighbour edge comes through
-// LDS; thread 0 also evaluates the tile's left halo edge. The energy is
-// reduced per tile, and the last tile of each line to finish (completion
-// counter) sums that line's tile partials in a fixed order, so results are
-// deterministic and need no second launch.
+// LDS; thread 0 also evaluates the tile's left halo edge. Each tile writes
+// its energy partial; a second, small launch sums every line's partials in a
+// fixed order (deterministic). The kernel boundary orders the two: no
+// device-scope fence or atomic is needed per tile, which matters on a
+// multi-XCD part where a device-scope release writes back the L2.
 #include "qew_internal.hpp"
 
 using namespace qew_dev;
 
 namespace {
 
-template <typename T>
-__device__ __forceinline__ T ld_volatile(const T *p) {
-    return *reinterpret_cast<const volatile T *>(p);
+// energy[line] = sum of the line's tile partials (FP64)
+__global__ __launch_bounds__(256) void line_energy64(const LineDesc *lines, LineDesc one,
+                                                     int single, const double *partial,
+                                                     double *energy) {
+    __shared__ double red[16];
+    const int line = blockIdx.x;
+    const LineDesc L = single ? one : lines[line];
+    const int t0 = single ? 0 : L.tile0;
+    double acc = 0;
+    for (int t = threadIdx.x; t < L.ntiles; t += blockDim.x) acc += partial[t0 + t];
+    acc = block_sum(acc, red);
+    if (threadIdx.x == 0) energy[line] = acc;
 }
 
-// Last-workgroup reduction of one line's tile partials (FP64).
-__device__ void finish_line_f64(double bsum, int tile, int tile0, int ntl, int line,
-                                double *partial, unsigned *counter, double *energy,
-                                double *red) {
-    __shared__ int s_last;
+// FP32 path: compensated partials; -f * anchor * L adds the anchor's share of
+// the driving-force term
+__global__ __launch_bounds__(256) void line_energy32(const LineDesc *lines, const double *anchor,
+                                                     const KSum *partial, double *energy) {
+    __shared__ KSum red[16];
+    const int line = blockIdx.x;
+    const LineDesc L = lines[line];
+    KSum acc{0.f, 0.f};
+    for (int t = threadIdx.x; t < L.ntiles; t += blockDim.x) acc = ksum_add(acc, partial[L.tile0 + t]);
+    acc = block_ksum(acc, red);
     if (threadIdx.x == 0) {
-        partial[tile] = bsum;
-        __threadfence();
-        const unsigned prev = atomicAdd(&counter[line], 1u);
-        s_last = (prev == (unsigned)ntl - 1u);
-    }
-    __syncthreads();
-    if (s_last) {
-        __threadfence();
-        double acc = 0;
-        for (int t = threadIdx.x; t < ntl; t += blockDim.x) acc += ld_volatile(&partial[tile0 + t]);
-        acc = block_sum(acc, red);
-        if (threadIdx.x == 0) {
-            energy[line] = acc;
-            counter[line] = 0u;
-        }
+        const double a = anchor ? anchor[line] : 0.0;
+        energy[line] = ((double)acc.s + (double)acc.c) - L.f * a * L.L;
     }
 }
 
@@ -127,9 +129,7 @@ __global__ __launch_bounds__(kTile) void eval64_kernel(qew_host::EvalArgs a, Lin
     }
     if (ENERGY) {
         const double bsum = block_sum(e, red);
-        const int ntl = single ? (int)gridDim.x : L.ntiles;
-        finish_line_f64(bsum, tile, L.tile0, ntl, line, static_cast<double *>(a.partial),
-                        a.counter, a.energy, red);
+        if (tid == 0) static_cast<double *>(a.partial)[tile] = bsum;
     }
 }
 
@@ -140,7 +140,6 @@ __global__ __launch_bounds__(kTile) void eval32_kernel(qew_host::EvalArgs a) {
     __shared__ float s_slope[kTile + 1];
     __shared__ float s_h[kTile + 1];
     __shared__ KSum red[16];
-    __shared__ int s_last;
     const int tid = threadIdx.x;
     const int tile = blockIdx.x;
     const int line = a.tile_line[tile];
@@ -243,32 +242,8 @@ __global__ __launch_bounds__(kTile) void eval32_kernel(qew_host::EvalArgs a) {
             dxf * (lf + 0.5f * (s_dvdy[tid] + s_dvdy[tid + 1]) - f);
     }
     if (a.energy) {
-        KSum k{e, 0.f};
-        k = block_ksum(k, red);
-        KSum *part = static_cast<KSum *>(a.partial);
-        if (tid == 0) {
-            part[tile] = k;
-            __threadfence();
-            const unsigned prev = atomicAdd(&a.counter[line], 1u);
-            s_last = (prev == (unsigned)L.ntiles - 1u);
-        }
-        __syncthreads();
-        if (s_last) {
-            __threadfence();
-            KSum acc{0.f, 0.f};
-            for (int t = tid; t < L.ntiles; t += blockDim.x) {
-                KSum p;
-                p.s = ld_volatile(&part[L.tile0 + t].s);
-                p.c = ld_volatile(&part[L.tile0 + t].c);
-                acc = ksum_add(acc, p);
-            }
-            acc = block_ksum(acc, red);
-            if (tid == 0) {
-                // -f * anchor * L: the anchor's share of the driving-force term
-                a.energy[line] = ((double)acc.s + (double)acc.c) - L.f * anchor * L.L;
-                a.counter[line] = 0u;
-            }
-        }
+        const KSum k = block_ksum(KSum{e, 0.f}, red);
+        if (tid == 0) static_cast<KSum *>(a.partial)[tile] = k;
     }
 }
 
@@ -323,21 +298,33 @@ void launch_eval_f64(hipStream_t s, const EvalArgs &a, const LineDesc *one_line,
         launch64<false, false, true>(s, a, one_line, one_noise, dtype, grid);
     }
     QEW_HIP(hipGetLastError());
+    if (energy) finish_energy_f64(s, a, one_line, 1);
 }
 
-void launch_eval_f32(hipStream_t s, const EvalArgs &a) {
+void finish_energy_f64(hipStream_t s, const EvalArgs &a, const LineDesc *one_line, int nlines) {
+    LineDesc l{};
+    if (one_line) {
+        l = *one_line;
+        l.ntiles = a.ntiles;
+    }
+    hipLaunchKernelGGL(line_energy64, dim3(one_line ? 1 : nlines), dim3(256), 0, s, a.lines, l,
+                       one_line ? 1 : 0, static_cast<const double *>(a.partial), a.energy);
+    QEW_HIP(hipGetLastError());
+}
+
+void launch_eval_f32(hipStream_t s, const EvalArgs &a, int nlines) {
     hipLaunchKernelGGL(eval32_kernel, dim3(a.ntiles), dim3(kTile), 0, s, a);
     QEW_HIP(hipGetLastError());
+    if (a.energy) {
+        hipLaunchKernelGGL(line_energy32, dim3(nlines), dim3(256), 0, s, a.lines, a.anchor,
+                           static_cast<const KSum *>(a.partial), a.energy);
+        QEW_HIP(hipGetLastError());
+    }
 }
 
 void ctx_reserve_eval(qew_ctx ctx, int ntiles, int nlines) {
     ctx->partial.ensure(sizeof(double) * (size_t)std::max(ntiles, 1));
-    const size_t cb = sizeof(unsigned) * (size_t)std::max(nlines, 1);
-    if (ctx->counter.bytes < cb) {
-        ctx->counter.ensure(cb);
-        QEW_HIP(hipMemsetAsync(ctx->counter.ptr, 0, ctx->counter.bytes, ctx->stream));
-    }
-    ctx->result.ensure(sizeof(double) * 64);
+    ctx->result.ensure(sizeof(double) * (size_t)std::max(nlines, 64));
     ctx->pinned.ensure(sizeof(double) * 64);
 }
 
@@ -382,7 +369,6 @@ double single_eval(qew_ctx ctx, qew_noise noise, const qew_params *p, int n, con
     a.grad = grad;
     a.energy = want_energy ? ctx->result.as<double>() : nullptr;
     a.partial = ctx->partial.ptr;
-    a.counter = ctx->counter.as<unsigned>();
     launch_eval_f64(ctx->stream, a, &l, &noise->desc);
     if (!want_energy) return 0.0;
     QEW_HIP(hipMemcpyAsync(ctx->pinned.ptr, ctx->result.ptr, sizeof(double), hipMemcpyDeviceToHost,
@@ -452,12 +438,11 @@ qew_status qew_batch_eval(qew_batch b, int dtype, const void *h, const double *a
         a.grad = grad;
         a.energy = energy;
         a.partial = b->d_partial.ptr;
-        a.counter = b->d_counter.as<unsigned>();
         if (dtype == QEW_F32) {
             for (const auto &nd : b->noises)
                 QEW_REQUIRE(nd.kind == TABLE && nd.dtype == 1,
                             "QEW_F32 evaluation needs FP32 coefficient tables");
-            launch_eval_f32(ctx->stream, a);
+            launch_eval_f32(ctx->stream, a, b->nlines);
         } else {
             QEW_REQUIRE(!anchor, "anchors apply to QEW_F32 line state only");
             NoiseDesc dtype_probe{};
@@ -478,6 +463,7 @@ qew_status qew_batch_eval(qew_batch b, int dtype, const void *h, const double *a
                 else launch64<false, false, true>(ctx->stream, a, nullptr, nullptr, 0, a.ntiles);
             }
             QEW_HIP(hipGetLastError());
+            if (energy) finish_energy_f64(ctx->stream, a, nullptr, b->nlines);
         }
     });
 }

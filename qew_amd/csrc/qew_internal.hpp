@@ -104,9 +104,9 @@ struct qew_ctx_s {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    // scratch for single-line calls: per-tile partial sums and the per-line
-    // completion counter of the last-workgroup reduction (kept zeroed)
-    qew_host::DevBuf partial, counter, result, work;
+    // scratch for single-line calls: per-tile energy partials, the energy
+    // result slot and solver work buffers
+    qew_host::DevBuf partial, result, work;
     qew_host::PinnedBuf pinned;
     int num_cus = 0;
 };
@@ -127,7 +127,7 @@ struct qew_batch_s {
     std::vector<qew_dev::LineDesc> lines;
     std::vector<qew_dev::NoiseDesc> noises;
     std::vector<int64_t> offsets;
-    qew_host::DevBuf d_lines, d_noises, d_tile_line, d_partial, d_counter;
+    qew_host::DevBuf d_lines, d_noises, d_tile_line, d_partial;
     qew_host::DevBuf d_scratch;  // solver workspaces (grown on demand)
 };
 
@@ -137,9 +137,8 @@ inline void ctx_check(qew_ctx ctx) { QEW_REQUIRE(ctx != nullptr, "null context")
 
 inline void use_device(qew_ctx ctx) { QEW_HIP(hipSetDevice(ctx->device)); }
 
-// Makes sure the context's partial/counter scratch covers `ntiles` tiles and
-// `nlines` lines. Counters are zeroed when (re)allocated; the kernels reset
-// them after use.
+// Makes sure the context's scratch covers the energy partials of `ntiles`
+// tiles and the results of `nlines` lines.
 void ctx_reserve_eval(qew_ctx ctx, int ntiles, int nlines);
 
 // Single-line descriptor helper.
@@ -160,12 +159,15 @@ struct EvalArgs {
     void *grad;         // null: energy only
     double *energy;     // device, per line; null: gradient only
     void *partial;
-    unsigned *counter;
 };
 
 void launch_eval_f64(hipStream_t s, const EvalArgs &a, const qew_dev::LineDesc *one_line,
                      const qew_dev::NoiseDesc *one_noise);
-void launch_eval_f32(hipStream_t s, const EvalArgs &a);
+void launch_eval_f32(hipStream_t s, const EvalArgs &a, int nlines);
+// Sums the tile partials of an energy evaluation into energy[] (one line
+// passed by value, or nlines batch lines).
+void finish_energy_f64(hipStream_t s, const EvalArgs &a, const qew_dev::LineDesc *one_line,
+                       int nlines);
 
 // Global-memory Stockham FFT (batched 1-D complex transforms of FP64 pairs).
 // data[b * bstride + k * stride] for b < batch, k < n; sign -1 forward,

@@ -230,7 +230,6 @@ qew_status qew_lbfgs_minimize(qew_ctx ctx, qew_noise noise, const qew_params *p,
             a.grad = grad;
             a.energy = energy ? ctx->result.as<double>() : nullptr;
             a.partial = ctx->partial.ptr;
-            a.counter = ctx->counter.as<unsigned>();
             launch_eval_f64(s, a, &line, &noise->desc);
             if (!energy) return 0.0;
             QEW_HIP(hipMemcpyAsync(P.host, ctx->result.ptr, sizeof(double), hipMemcpyDeviceToHost, s));

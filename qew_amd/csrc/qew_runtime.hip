@@ -229,7 +229,6 @@ qew_status qew_ctx_destroy(qew_ctx ctx) {
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
         ctx->partial.release();
-        ctx->counter.release();
         ctx->result.release();
         ctx->work.release();
         ctx->pinned.release();
@@ -552,7 +551,6 @@ qew_status qew_batch_create(qew_ctx ctx, int nlines, const int *lengths,
             b->d_noises.ensure(sizeof(NoiseDesc) * b->noises.size());
             b->d_tile_line.ensure(sizeof(int) * tile_line.size());
             b->d_partial.ensure(sizeof(double) * tile_line.size());
-            b->d_counter.ensure(sizeof(unsigned) * nlines);
             QEW_HIP(hipMemcpyAsync(b->d_lines.ptr, b->lines.data(), sizeof(LineDesc) * nlines,
                                    hipMemcpyHostToDevice, ctx->stream));
             QEW_HIP(hipMemcpyAsync(b->d_noises.ptr, b->noises.data(),
@@ -561,14 +559,12 @@ qew_status qew_batch_create(qew_ctx ctx, int nlines, const int *lengths,
             QEW_HIP(hipMemcpyAsync(b->d_tile_line.ptr, tile_line.data(),
                                    sizeof(int) * tile_line.size(), hipMemcpyHostToDevice,
                                    ctx->stream));
-            QEW_HIP(hipMemsetAsync(b->d_counter.ptr, 0, sizeof(unsigned) * nlines, ctx->stream));
             QEW_HIP(hipStreamSynchronize(ctx->stream));
         } catch (...) {
             b->d_lines.release();
             b->d_noises.release();
             b->d_tile_line.release();
             b->d_partial.release();
-            b->d_counter.release();
             delete b;
             throw;
         }
@@ -585,7 +581,6 @@ qew_status qew_batch_destroy(qew_batch b) {
         b->d_noises.release();
         b->d_tile_line.release();
         b->d_partial.release();
-        b->d_counter.release();
         b->d_scratch.release();
         delete b;
     });

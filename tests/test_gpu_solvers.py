@@ -49,12 +49,14 @@ def test_lbfgs_matches_oracle(q, precondition, model):
     lt = 50.0 if precondition else 1.0
     p_o = O.params(2.56, lt, 0.0, model)
     p = q.Params(2.56, lt, 0.0, q.Model(model))
-    # ftol 1e-8 stays above the Armijo roundoff floor of this problem, so both
-    # sides converge; the minima then agree to the energy's rounding and to
-    # ftol / (smallest curvature) in position
-    lpo = O.lbfgs_params(m=10, ftol=1e-8, max_iter=5000, precondition=precondition,
+    # ftol stays above the Armijo roundoff floor of each problem (below it
+    # the sufficient-decrease test compares rounding noise and both sides
+    # stall), so both sides converge; the minima then agree to the energy's
+    # rounding and to ftol / (smallest curvature) in position
+    ftol = 1e-8 if precondition else 1e-6
+    lpo = O.lbfgs_params(m=10, ftol=ftol, max_iter=5000, precondition=precondition,
                          precond_shift=0.5 * 0.01 * 0.3 / 0.04 if precondition else 0.0)
-    lp = q.LbfgsParams(m=10, ftol=1e-8, max_iter=5000, precondition=precondition,
+    lp = q.LbfgsParams(m=10, ftol=ftol, max_iter=5000, precondition=precondition,
                        precond_shift=lpo.precond_shift)
     h_ref, r_ref = O.lbfgs(np.full(n, 0.5), p_o, on, lpo)
     h = dev(np.full(n, 0.5))
@@ -62,8 +64,8 @@ def test_lbfgs_matches_oracle(q, precondition, model):
     assert r.converged and r_ref.converged
     assert abs(r.iterations - r_ref.iterations) <= max(2, r_ref.iterations // 10)
     e_ref = O.objective(h_ref, p_o, on)
-    assert rel(q.objective(h, p, gn), e_ref) < 1e-12
-    assert np.max(np.abs(host(h) - h_ref)) < 1e-6
+    assert rel(q.objective(h, p, gn), e_ref) < 1e-11
+    assert np.max(np.abs(host(h) - h_ref)) < 1e-5
 
 
 def test_lbfgs_branches(q):
