@@ -99,7 +99,19 @@ def main():
     ap.add_argument("--evals", type=int, default=EVALS)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["both", "f32", "f64"], default="both")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (reference point for rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE scaling) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": x.numel() * 4, "launches": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(this build has no multi-GPU path)", file=sys.stderr)

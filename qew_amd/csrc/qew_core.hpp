@@ -19,14 +19,19 @@ constexpr int kTile = 256;  // sites per workgroup tile in the eval kernels
 
 enum Kind { TABLE = 0, SINCOS = 1, CONST = 2, ZERO = 3, HARMONIC = 4, WASHBOARD = 5 };
 
-// A potential as seen by a kernel. The table is stored x-contiguous,
-// coef[iy * nx + ix]: the 4 x-rows an edge needs at one y level are adjacent,
-// and neighbouring edges (adjacent lanes) read adjacent x, so a wavefront's
-// 16 gathers coalesce along x.
+// A potential as seen by a kernel. The table is stored x-contiguous with a
+// periodic x-halo: row iy holds pitch >= nx + 3 entries,
+//   table[iy * pitch + j] = coef((j - 1) mod nx, iy),  j = 0 .. nx + 2,
+// so the 4 x-neighbours ix-1 .. ix+2 of any cell ix are the contiguous
+// entries j = ix .. ix+3 -- one 16-byte (FP32) or two (FP64) vector loads per
+// y level, no wrap test -- and neighbouring edges (adjacent lanes) read
+// adjacent x, so a wavefront's gathers coalesce along x. pitch is a multiple
+// of 4 so rows start 16-byte aligned.
 struct NoiseDesc {
     int kind;
     int dtype;  // 0 FP64 table, 1 FP32 table
     int nx, ny;
+    int pitch;
     double dx, dy;
     const void *coef;
     double p[4];
@@ -93,9 +98,38 @@ __device__ __forceinline__ void locate64(const NoiseDesc &nd, double x, double y
     iyn[3] = (iyn[2] + 1 == ny) ? 0 : iyn[2] + 1;
 }
 
+// Vector types with element alignment only: a window starts at any element.
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef double d2u __attribute__((ext_vector_type(2), aligned(8)));
+#define QEW_GLOBAL __attribute__((address_space(1)))
+
+// The 4 coefficients (ix-1 .. ix+2 mod nx, iy) as one global vector load
+// (the table pointer travels in a descriptor, so the address space is stated
+// explicitly; a generic pointer would compile to flat loads).
+__device__ __forceinline__ void row4(const float *t, int pitch, int ix, int iy, float out[4]) {
+    const f4u r = *(const QEW_GLOBAL f4u *)(t + (size_t)iy * pitch + ix);
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+    out[3] = r.w;
+}
+
 template <typename C>
-__device__ __forceinline__ double coef_at(const NoiseDesc &nd, int ix, int iy) {
-    return (double)static_cast<const C *>(nd.coef)[(size_t)iy * nd.nx + ix];
+__device__ __forceinline__ void row4_64(const NoiseDesc &nd, int ix, int iy, double out[4]) {
+    if constexpr (sizeof(C) == 8) {
+        const double *p = static_cast<const double *>(nd.coef) + (size_t)iy * nd.pitch + ix;
+        const d2u a = *(const QEW_GLOBAL d2u *)p;
+        const d2u b = *(const QEW_GLOBAL d2u *)(p + 2);
+        out[0] = a.x;
+        out[1] = a.y;
+        out[2] = b.x;
+        out[3] = b.y;
+    } else {
+        float f[4];
+        row4(static_cast<const float *>(nd.coef), nd.pitch, ix, iy, f);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[k] = (double)f[k];
+    }
 }
 
 // Analytic potentials: the reference tests' functors, evaluated with the
@@ -145,9 +179,12 @@ __device__ __forceinline__ void sample_v_dy64(const NoiseDesc &nd, double x, dou
     bspline_dw(fy, dwy);
     double c[4][4];
 #pragma unroll
-    for (int b = 0; b < 4; ++b)
+    for (int b = 0; b < 4; ++b) {
+        double r[4];
+        row4_64<C>(nd, ixn[1], iyn[b], r);
 #pragma unroll
-        for (int a = 0; a < 4; ++a) c[a][b] = coef_at<C>(nd, ixn[a], iyn[b]);
+        for (int a = 0; a < 4; ++a) c[a][b] = r[a];
+    }
     double vv = 0, dd = 0;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
@@ -179,9 +216,12 @@ __device__ __forceinline__ double sample_dy64(const NoiseDesc &nd, double x, dou
     bspline_dw(fy, dwy);
     double c[4][4];
 #pragma unroll
-    for (int b = 0; b < 4; ++b)
+    for (int b = 0; b < 4; ++b) {
+        double r[4];
+        row4_64<C>(nd, ixn[1], iyn[b], r);
 #pragma unroll
-        for (int a = 0; a < 4; ++a) c[a][b] = coef_at<C>(nd, ixn[a], iyn[b]);
+        for (int a = 0; a < 4; ++a) c[a][b] = r[a];
+    }
     double d = 0;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
@@ -207,12 +247,20 @@ __device__ __forceinline__ double sample_v64(const NoiseDesc &nd, double x, doub
     double wx[4], wy[4];
     bspline_w(fx, wx);
     bspline_w(fy, wy);
+    double c[4][4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        double r[4];
+        row4_64<C>(nd, ixn[1], iyn[b], r);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) c[a][b] = r[a];
+    }
     double vv = 0;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
         double row = 0;
 #pragma unroll
-        for (int b = 0; b < 4; ++b) row += coef_at<C>(nd, ixn[a], iyn[b]) * wy[b];
+        for (int b = 0; b < 4; ++b) row += c[a][b] * wy[b];
         vv += wx[a] * row;
     }
     return vv;
@@ -234,11 +282,17 @@ __device__ inline void sample_full64(const NoiseDesc &nd, double x, double y, do
     bspline_dw(fx, dwx);
     bspline_w(fy, wy);
     bspline_dw(fy, dwy);
+    double cc[4][4];
+    for (int b = 0; b < 4; ++b) {
+        double r[4];
+        row4_64<C>(nd, ixn[1], iyn[b], r);
+        for (int a = 0; a < 4; ++a) cc[a][b] = r[a];
+    }
     double vv = 0, dgx = 0, dgy = 0;
     for (int a = 0; a < 4; ++a) {
         double row = 0, row_dy = 0;
         for (int b = 0; b < 4; ++b) {
-            const double c = coef_at<C>(nd, ixn[a], iyn[b]);
+            const double c = cc[a][b];
             row += c * wy[b];
             row_dy += c * dwy[b];
         }
@@ -476,18 +530,24 @@ __device__ __forceinline__ void edge32(const LineGeom &G, int e, int ep, float h
     bspline_wf(fx, wx);
     if (WANT_V) bspline_wf(fy, wy);
     if (WANT_DY) bspline_dwf(fy, dwy);
-    const int x2 = (ix + 1 == nx) ? 0 : ix + 1;
-    const int xs[4] = {(ix == 0) ? nx - 1 : ix - 1, ix, x2, (x2 + 1 == nx) ? 0 : x2 + 1};
     const int y2 = (iy + 1 == ny) ? 0 : iy + 1;
     const int ys[4] = {(iy == 0) ? ny - 1 : iy - 1, iy, y2, (y2 + 1 == ny) ? 0 : y2 + 1};
     const float *coef = static_cast<const float *>(G.nd.coef);
+    float cf[4][4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        float r[4];
+        row4(coef, G.nd.pitch, ix, ys[b], r);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) cf[q][b] = r[q];
+    }
     float vv = 0.f, dd = 0.f;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         float r = 0.f, rd = 0.f;
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const float c = coef[(size_t)ys[b] * nx + xs[q]];
+            const float c = cf[q][b];
             if (WANT_V) r = __builtin_fmaf(c, wy[b], r);
             if (WANT_DY) rd = __builtin_fmaf(c, dwy[b], rd);
         }

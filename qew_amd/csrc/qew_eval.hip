@@ -149,60 +149,12 @@ __global__ __launch_bounds__(kTile) void eval32_kernel(qew_host::EvalArgs a) {
     const int start = (tile - L.tile0) * kTile;
     const float *h = static_cast<const float *>(a.h) + L.off;
     const double anchor = a.anchor ? a.anchor[line] : 0.0;
-    const double dxl = L.L / (double)n;
-    const float dxf = (float)dxl, inv_dx = (float)(1.0 / dxl);
+    const LineGeom G = make_geom(L, N, anchor);
+    const float dxf = G.dxf, inv_dx = G.inv_dx;
     const float lt = (float)L.lt, f = (float)L.f;
     const bool arc = (L.model == 1);
-    const float *coef = static_cast<const float *>(N.coef);
-    const int nx = N.nx, ny = N.ny;
-    const double cx = 0.5 * dxl / N.dx;
-    const float inv_dy = (float)(1.0 / N.dy);
-    const YAnchor ya = make_anchor(anchor, N.dy, ny);
-
-    auto edge = [&](int i, int ip, float hi, float hip, float &v, float &dvdy) {
-        double gx = (double)(i + ip) * cx;
-        if (gx >= (double)nx) gx -= floor(gx / (double)nx) * (double)nx;
-        int ix = (int)gx;
-        const float fx = (float)(gx - (double)ix);
-        if (ix >= nx) ix -= nx;
-        const float t = __builtin_fmaf((hi + hip) * 0.5f, inv_dy, ya.frac);
-        const float kf = floorf(t);
-        const float fy = t - kf;
-        int iy = ya.cell + (int)kf;
-        if (iy < 0 || iy >= ny) iy = ((iy % ny) + ny) % ny;
-        float wx[4], wy[4], dwy[4];
-        bspline_wf(fx, wx);
-        bspline_wf(fy, wy);
-        bspline_dwf(fy, dwy);
-        const int x0 = (ix == 0) ? nx - 1 : ix - 1;
-        const int x2 = (ix + 1 == nx) ? 0 : ix + 1;
-        const int x3 = (x2 + 1 == nx) ? 0 : x2 + 1;
-        const int y0 = (iy == 0) ? ny - 1 : iy - 1;
-        const int y2 = (iy + 1 == ny) ? 0 : iy + 1;
-        const int y3 = (y2 + 1 == ny) ? 0 : y2 + 1;
-        const int ys[4] = {y0, iy, y2, y3};
-        const int xs[4] = {x0, ix, x2, x3};
-        float c[4][4];
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const float *r = coef + (size_t)ys[b] * nx;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) c[q][b] = r[xs[q]];
-        }
-        float vv = 0.f, dd = 0.f;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            float row = c[q][0] * wy[0], rdy = c[q][0] * dwy[0];
-#pragma unroll
-            for (int b = 1; b < 4; ++b) {
-                row = __builtin_fmaf(c[q][b], wy[b], row);
-                rdy = __builtin_fmaf(c[q][b], dwy[b], rdy);
-            }
-            vv = __builtin_fmaf(wx[q], row, vv);
-            dd = __builtin_fmaf(wx[q], rdy, dd);
-        }
-        v = vv;
-        dvdy = dd * inv_dy;
+    auto edge = [&](int e, int ep, float he, float hep, float &v, float &dvdy) {
+        edge32<true, true>(G, e, ep, he, hep, v, dvdy);
     };
 
     const int i = start + tid;

@@ -114,7 +114,7 @@ struct qew_ctx_s {
 struct qew_noise_s {
     qew_ctx ctx = nullptr;
     qew_dev::NoiseDesc desc{};
-    void *table = nullptr;             // device table, coef[iy * nx + ix]
+    void *table = nullptr;  // device table, see NoiseDesc for the layout
     std::vector<double> field;         // noise_on_grid (generated / from-field tables)
 };
 

@@ -111,43 +111,14 @@ __global__ __launch_bounds__(kTile) void langevin32_kernel(StepArgs a) {
     const bool active = i < n;
     const float *h = static_cast<const float *>(a.h_in) + L.off;
     const double anchor = a.anchor ? a.anchor[line] : 0.0;
-    const double dxl = L.L / (double)n;
-    const float dxf = (float)dxl, inv_dx = (float)(1.0 / dxl);
+    const LineGeom G = make_geom(L, N, anchor);
+    const float dxf = G.dxf, inv_dx = G.inv_dx;
     const float lt = (float)L.lt, f = (float)L.f;
     const bool arc = (L.model == 1);
-    const float *coef = static_cast<const float *>(N.coef);
-    const int nx = N.nx, ny = N.ny;
-    const double cx = 0.5 * dxl / N.dx;
-    const float inv_dy = (float)(1.0 / N.dy);
-    const YAnchor ya = make_anchor(anchor, N.dy, ny);
-
     auto edge_dy = [&](int e, int ep, float he, float hep) -> float {
-        double gx = (double)(e + ep) * cx;
-        if (gx >= (double)nx) gx -= floor(gx / (double)nx) * (double)nx;
-        int ix = (int)gx;
-        const float fx = (float)(gx - (double)ix);
-        if (ix >= nx) ix -= nx;
-        const float t = __builtin_fmaf((he + hep) * 0.5f, inv_dy, ya.frac);
-        const float kf = floorf(t);
-        const float fy = t - kf;
-        int iy = ya.cell + (int)kf;
-        if (iy < 0 || iy >= ny) iy = ((iy % ny) + ny) % ny;
-        float wx[4], dwy[4];
-        bspline_wf(fx, wx);
-        bspline_dwf(fy, dwy);
-        const int x2 = (ix + 1 == nx) ? 0 : ix + 1;
-        const int xs[4] = {(ix == 0) ? nx - 1 : ix - 1, ix, x2, (x2 + 1 == nx) ? 0 : x2 + 1};
-        const int y2 = (iy + 1 == ny) ? 0 : iy + 1;
-        const int ys[4] = {(iy == 0) ? ny - 1 : iy - 1, iy, y2, (y2 + 1 == ny) ? 0 : y2 + 1};
-        float dd = 0.f;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            float r = 0.f;
-#pragma unroll
-            for (int b = 0; b < 4; ++b) r = __builtin_fmaf(coef[(size_t)ys[b] * nx + xs[q]], dwy[b], r);
-            dd = __builtin_fmaf(wx[q], r, dd);
-        }
-        return dd * inv_dy;
+        float v, dy;
+        edge32<false, true>(G, e, ep, he, hep, v, dy);
+        return dy;
     };
 
     float hi = 0.f, hip = 0.f;

@@ -101,10 +101,11 @@ __global__ void prefilter_divide(double2 *A, int nx, int ny) {
     A[k] = make_double2(A[k].x / den, A[k].y / den);
 }
 
-// table[j * nx + i] = (T) src[i * ny + j]; src is the real part of a complex
-// array (stride 2) or a real array (stride 1).
+// table[j * pitch + i + 1] = (T) src[i * ny + j]; src is the real part of a
+// complex array (stride 2) or a real array (stride 1).
 template <typename T>
-__global__ void transpose_to_table(const double *src, int sstride, T *table, int nx, int ny) {
+__global__ void transpose_to_table(const double *src, int sstride, T *table, int nx, int ny,
+                                   int pitch) {
     __shared__ double tile[32][33];
     const int i0 = blockIdx.y * 32, j0 = blockIdx.x * 32;
     for (int r = threadIdx.y; r < 32; r += blockDim.y) {
@@ -114,17 +115,29 @@ __global__ void transpose_to_table(const double *src, int sstride, T *table, int
     __syncthreads();
     for (int r = threadIdx.y; r < 32; r += blockDim.y) {
         const int j = j0 + r, i = i0 + threadIdx.x;
-        if (i < nx && j < ny) table[(int64_t)j * nx + i] = (T)tile[threadIdx.x][r];
+        if (i < nx && j < ny) table[(int64_t)j * pitch + i + 1] = (T)tile[threadIdx.x][r];
     }
 }
 
+// periodic x-halo: columns 0, nx + 1, nx + 2 repeat coef(nx - 1), coef(0),
+// coef(1 mod nx)
 template <typename T>
-__global__ void table_to_rowmajor(const T *table, double *dst, int nx, int ny) {
+__global__ void fill_x_halo(T *table, int nx, int ny, int pitch) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ny) return;
+    T *row = table + (int64_t)j * pitch;
+    row[0] = row[nx];
+    row[nx + 1] = row[1];
+    row[nx + 2] = row[(1 % nx) + 1];
+}
+
+template <typename T>
+__global__ void table_to_rowmajor(const T *table, double *dst, int nx, int ny, int pitch) {
     __shared__ double tile[32][33];
     const int j0 = blockIdx.y * 32, i0 = blockIdx.x * 32;
     for (int r = threadIdx.y; r < 32; r += blockDim.y) {
         const int j = j0 + r, i = i0 + threadIdx.x;
-        if (i < nx && j < ny) tile[r][threadIdx.x] = (double)table[(int64_t)j * nx + i];
+        if (i < nx && j < ny) tile[r][threadIdx.x] = (double)table[(int64_t)j * pitch + i + 1];
     }
     __syncthreads();
     for (int r = threadIdx.y; r < 32; r += blockDim.y) {
@@ -146,14 +159,21 @@ void make_table(qew_noise nz, const double *src, int sstride) {
     qew_ctx ctx = nz->ctx;
     const int nx = nz->desc.nx, ny = nz->desc.ny;
     const size_t es = nz->desc.dtype == 1 ? sizeof(float) : sizeof(double);
-    QEW_HIP(hipMalloc(&nz->table, es * (size_t)nx * ny));
+    const int pitch = ((nx + 3 + 3) / 4) * 4;
+    nz->desc.pitch = pitch;
+    QEW_HIP(hipMalloc(&nz->table, es * (size_t)pitch * ny));
     const dim3 g((ny + 31) / 32, (nx + 31) / 32), b(32, 8);
-    if (nz->desc.dtype == 1)
+    if (nz->desc.dtype == 1) {
         hipLaunchKernelGGL(transpose_to_table<float>, g, b, 0, ctx->stream, src, sstride,
-                           static_cast<float *>(nz->table), nx, ny);
-    else
+                           static_cast<float *>(nz->table), nx, ny, pitch);
+        hipLaunchKernelGGL(fill_x_halo<float>, dim3((ny + 255) / 256), dim3(256), 0, ctx->stream,
+                           static_cast<float *>(nz->table), nx, ny, pitch);
+    } else {
         hipLaunchKernelGGL(transpose_to_table<double>, g, b, 0, ctx->stream, src, sstride,
-                           static_cast<double *>(nz->table), nx, ny);
+                           static_cast<double *>(nz->table), nx, ny, pitch);
+        hipLaunchKernelGGL(fill_x_halo<double>, dim3((ny + 255) / 256), dim3(256), 0, ctx->stream,
+                           static_cast<double *>(nz->table), nx, ny, pitch);
+    }
     QEW_HIP(hipGetLastError());
     nz->desc.coef = nz->table;
 }
@@ -492,10 +512,12 @@ qew_status qew_noise_coefficients(qew_noise noise, double *coef_host) {
         const dim3 g((nx + 31) / 32, (ny + 31) / 32), b(32, 8);
         if (noise->desc.dtype == 1)
             hipLaunchKernelGGL(table_to_rowmajor<float>, g, b, 0, ctx->stream,
-                               static_cast<const float *>(noise->table), d, nx, ny);
+                               static_cast<const float *>(noise->table), d, nx, ny,
+                               noise->desc.pitch);
         else
             hipLaunchKernelGGL(table_to_rowmajor<double>, g, b, 0, ctx->stream,
-                               static_cast<const double *>(noise->table), d, nx, ny);
+                               static_cast<const double *>(noise->table), d, nx, ny,
+                               noise->desc.pitch);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess)
             e = hipMemcpyAsync(coef_host, d, sizeof(double) * (size_t)nx * ny,
