@@ -91,6 +91,80 @@ def cpu_baseline(coef_rowmajor, h0, budget_s=15.0):
                       f"OpenMP restatement of the reference two-pass gradient) in {dt:.1f} s"}
 
 
+def bench_lbfgs(log, nlines=512, plain_sample=8):
+    """Batched static minimisation: nlines independent realizations with line
+    lengths drawn from {1024, ..., 16384}, each on its own L x 2048 table with
+    xi from {2, 4, 8}, linear or arclength elasticity (seeded draw), flat start
+    at Ly/2, line tension A (lambda_p/xi)^{3/2} with lambda_p/xi = 4, L-BFGS
+    m = 10 with the FFT preconditioner and the drivers' per-line shift
+    0.5 dx A / xi^2. One persistent launch per precision; the plain FP64
+    host-driven single-line path is timed on a sample of the same lines."""
+    import torch
+    import qew_amd as q
+    rng = np.random.default_rng(2024)
+    lengths = rng.choice([1024, 2048, 4096, 8192, 16384], nlines).tolist()
+    xis = rng.choice([2.0, 4.0, 8.0], nlines).tolist()
+    models = rng.integers(0, 2, nlines).tolist()
+    ny = 2048
+    t0 = time.perf_counter()
+    specs = [(L_, ny, float(L_), float(ny), 1.0, xi, xi, 1 + k)
+             for k, (L_, xi) in enumerate(zip(lengths, xis))]
+    n64 = q.FilteredNoise.generate_batch(specs, dtype=q.F64)
+    n32 = [nz.convert(q.F32) for nz in n64]
+    torch.cuda.synchronize()
+    log(f"lbfgs: {nlines} tables generated in {time.perf_counter() - t0:.1f} s "
+        f"(sum L = {sum(lengths)})")
+    lt = 4.0 ** 1.5
+    params = [q.Params(float(L_), lt, 0.0, q.Model(m)) for L_, m in zip(lengths, models)]
+    shifts = [0.5 / xi ** 2 for xi in xis]
+    h0 = torch.full((sum(lengths),), ny / 2.0, dtype=torch.float64, device="cuda:0")
+    out = {}
+    finals = {}
+    # ftol 1e-5: above the Armijo roundoff floor of the longest lines (L=16384,
+    # |E| ~ 1e4), where the reference algorithm converges in ~100-300 iterations
+    for name, noises, dtype, ftol in (("f64", n64, q.F64, 1e-5), ("f32", n32, q.F32, 1e-5)):
+        b = q.Batch(lengths, noises, params)
+        b.set_precond_shifts(shifts)
+        lp = q.LbfgsParams(m=10, ftol=ftol, max_iter=10000, precondition=True)
+        h = h0.clone()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        res = b.lbfgs(h, lp, dtype=dtype)
+        dt = time.perf_counter() - t1
+        # energies of the minima, evaluated in FP64 on the FP64 tables
+        e, _ = q.Batch(lengths, n64, params).eval(h)
+        finals[name] = e.cpu().numpy()
+        its = [r.iterations for r in res]
+        out[name] = {"minima_per_s": nlines / dt, "seconds": dt, "ftol": ftol,
+                     "converged": int(sum(r.converged for r in res)),
+                     "iterations_mean": float(np.mean(its)), "iterations_max": int(max(its))}
+        log(f"lbfgs {name}: {json.dumps(out[name])}")
+        del b, h
+    rel_err = np.abs(finals["f32"] - finals["f64"]) / np.abs(finals["f64"])
+    out["f32_vs_f64_energy_rel_err"] = {"max": float(rel_err.max()),
+                                        "median": float(np.median(rel_err)),
+                                        "lines_within_1e-6": int((rel_err <= 1e-6).sum())}
+    # plain FP64 host-driven path, one line at a time, on a sample of lines
+    idx = list(range(0, nlines, max(1, nlines // plain_sample)))[:plain_sample]
+    t2 = time.perf_counter()
+    for k in idx:
+        h = torch.full((lengths[k],), ny / 2.0, dtype=torch.float64, device="cuda:0")
+        q.lbfgs_minimize(h, params[k], n64[k],
+                         q.LbfgsParams(m=10, ftol=1e-5, max_iter=10000, precondition=True,
+                                       precond_shift=shifts[k]))
+    tp = time.perf_counter() - t2
+    plain = len(idx) / tp
+    out["plain_f64_single_line_minima_per_s"] = plain
+    out["plain_sample"] = (f"lines {idx} (evenly spaced in the batch; lengths "
+                           f"{[lengths[k] for k in idx]}), {tp:.1f} s")
+    out["speedup_f32_batched_vs_plain_f64"] = out["f32"]["minima_per_s"] / plain
+    out["speedup_f64_batched_vs_plain_f64"] = out["f64"]["minima_per_s"] / plain
+    out["workload"] = (f"{nlines} lines, L from {{1024..16384}} (sum {sum(lengths)}), "
+                       f"tables L x {ny}, xi in {{2,4,8}}, m=10, FFT preconditioner")
+    log(f"lbfgs: {json.dumps(out)}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +172,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--evals", type=int, default=EVALS)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-lbfgs", action="store_true", help="skip the batched L-BFGS workload")
+    ap.add_argument("--lbfgs-lines", type=int, default=512)
     ap.add_argument("--only", choices=["both", "f32", "f64"], default="both")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (reference point for rocprofv3 "
@@ -234,6 +310,9 @@ def main():
         line["f32_bytes_per_eval"] = r32["bytes_per_eval"]
     if err:
         line["fp32_vs_fp64_error"] = err
+    if not args.no_lbfgs:
+        torch.cuda.empty_cache()
+        line["batched_lbfgs"] = bench_lbfgs(log, nlines=args.lbfgs_lines)
     print(json.dumps(line), flush=True)
 
 

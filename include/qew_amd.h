@@ -143,6 +143,17 @@ qew_status qew_copy_to_host(qew_ctx ctx, void *dst, const void *src, size_t byte
 qew_status qew_noise_generate(qew_ctx ctx, int nx, int ny, double Lx, double Ly,
                               double amplitude, double xi_x, double xi_y,
                               uint64_t seed, int dtype, qew_noise *out);
+/* `count` FilteredNoise tables in one call (arrays of count entries): the
+ * independent mt19937_64 phase streams are drawn on parallel host threads.
+ * Each out[k] equals qew_noise_generate with the k-th arguments. */
+qew_status qew_noise_generate_batch(qew_ctx ctx, int count, const int *nx, const int *ny,
+                                    const double *Lx, const double *Ly,
+                                    const double *amplitude, const double *xi_x,
+                                    const double *xi_y, const uint64_t *seed, int dtype,
+                                    qew_noise *out);
+/* A copy of a potential with its table stored as `dtype` (FP64 -> FP32
+ * rounds each coefficient once). No reference counterpart. */
+qew_status qew_noise_convert(qew_noise noise, int dtype, qew_noise *out);
 /* FilteredNoise(field, nx, ny, Lx, Ly) (src/filtered_noise.cpp:84-90): field
  * is host, row-major field[i*ny + j]. */
 qew_status qew_noise_from_field(qew_ctx ctx, const double *field, int nx, int ny,
@@ -195,6 +206,10 @@ qew_status qew_batch_create(qew_ctx ctx, int nlines, const int *lengths,
                             qew_batch *out);
 qew_status qew_batch_destroy(qew_batch batch);
 qew_status qew_batch_offsets(qew_batch batch, int64_t *offsets_host);
+/* Per-line L-BFGS preconditioner shifts for qew_batch_lbfgs (host array of
+ * nlines; a value > 0 replaces qew_lbfgs_params.precond_shift for its line,
+ * e.g. the driver's 0.5 dx A / xi^2 when xi differs between lines). */
+qew_status qew_batch_set_precond_shifts(qew_batch batch, const double *shifts);
 /* Changes the driving force of every line (host array of nlines doubles). */
 qew_status qew_batch_set_forces(qew_batch batch, const double *forces);
 

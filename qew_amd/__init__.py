@@ -241,6 +241,35 @@ class FilteredNoise(_Noise):
         self.handle = h
         return self
 
+    @classmethod
+    def generate_batch(cls, specs, dtype=F64, device=0):
+        """One table per spec (nx, ny, Lx, Ly, amplitude, xi_x, xi_y, seed); the
+        independent phase streams are drawn on parallel host threads."""
+        ctx = context(device)
+        n = len(specs)
+        col = lambda k, ct: (ct * n)(*[s[k] for s in specs])
+        out = (C.c_void_p * n)()
+        check(_lib.load().qew_noise_generate_batch(
+            ctx.handle, n, col(0, C.c_int), col(1, C.c_int), col(2, C.c_double),
+            col(3, C.c_double), col(4, C.c_double), col(5, C.c_double), col(6, C.c_double),
+            col(7, C.c_uint64), dtype, out))
+        res = []
+        for k in range(n):
+            self = cls.__new__(cls)
+            self.ctx = ctx
+            self.handle = C.c_void_p(out[k])
+            res.append(self)
+        return res
+
+    def convert(self, dtype):
+        """A copy of this potential with its table stored as `dtype`."""
+        other = type(self).__new__(type(self))
+        other.ctx = self.ctx
+        h = C.c_void_p()
+        check(_lib.load().qew_noise_convert(self.handle, dtype, C.byref(h)))
+        other.handle = h
+        return other
+
     def field(self):
         i = self.info()
         out = np.empty((i["nx"], i["ny"]))
@@ -415,6 +444,10 @@ class Batch:
                 self.handle = None
         except Exception:
             pass
+
+    def set_precond_shifts(self, shifts):
+        s = (C.c_double * self.nlines)(*map(float, shifts))
+        check(_lib.load().qew_batch_set_precond_shifts(self.handle, s))
 
     def set_forces(self, forces):
         f = (C.c_double * self.nlines)(*map(float, forces))

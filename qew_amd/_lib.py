@@ -60,6 +60,9 @@ SIGNATURES = [
     ("qew_copy_to_device", [VP, VP, VP, C.c_size_t]),
     ("qew_copy_to_host", [VP, VP, VP, C.c_size_t]),
     ("qew_noise_generate", [VP, I, I, D, D, D, D, D, U64, I, C.POINTER(VP)]),
+    ("qew_noise_generate_batch", [VP, I, C.POINTER(I), C.POINTER(I), DP, DP, DP, DP, DP,
+                                  C.POINTER(U64), I, C.POINTER(VP)]),
+    ("qew_noise_convert", [VP, I, C.POINTER(VP)]),
     ("qew_noise_from_field", [VP, DP, I, I, D, D, I, C.POINTER(VP)]),
     ("qew_noise_from_coefficients", [VP, DP, I, I, D, D, I, C.POINTER(VP)]),
     ("qew_noise_analytic", [VP, I, DP, C.POINTER(VP)]),
@@ -76,6 +79,7 @@ SIGNATURES = [
     ("qew_batch_destroy", [VP]),
     ("qew_batch_offsets", [VP, C.POINTER(C.c_int64)]),
     ("qew_batch_set_forces", [VP, DP]),
+    ("qew_batch_set_precond_shifts", [VP, DP]),
     ("qew_batch_eval", [VP, I, VP, VP, VP, VP]),
     ("qew_lbfgs_minimize", [VP, VP, C.POINTER(Params), C.POINTER(LbfgsParams), I, VP,
                             C.POINTER(LbfgsResult)]),

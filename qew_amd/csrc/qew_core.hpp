@@ -46,6 +46,7 @@ struct LineDesc {
     int tile0;  // first eval tile of this line
     int ntiles;
     int pad;
+    double shift;  // L-BFGS preconditioner shift for this line (> 0 overrides)
     double L;   // physical_size
     double lt;  // line_tension
     double f;   // driving_force

@@ -112,22 +112,25 @@ __device__ double wg_objective(const LineGeom &G, const T *x, const T *d, double
         }
         return block_sum(acc, r.d);
     } else {
+        // The sufficient-decrease test compares energies whose difference
+        // shrinks like |g|^2 while the energy itself is O(n): FP32 terms
+        // (even compensated) leave a rounding floor far above that, so the
+        // energy of the FP32 trial point (the same FP32 values the commit
+        // writes) is evaluated in FP64 on the FP32 table.
         const float al = (float)alpha;
-        const float lt = (float)G.lt, f = (float)G.f;
-        KSum acc{0.f, 0.f};
+        double acc = 0;
         for (int i = threadIdx.x; i < n; i += blockDim.x) {
             const int ip = (i + 1 == n) ? 0 : i + 1;
-            const float hi = d ? __builtin_fmaf(al, d[i], x[i]) : x[i];
-            const float hip = d ? __builtin_fmaf(al, d[ip], x[ip]) : x[ip];
-            float v, dummy;
-            edge32<true, false>(G, i, ip, hi, hip, v, dummy);
-            const float dh = (hip - hi) * G.inv_dx;
-            const float s2 = dh * dh;
-            const float le = (G.model == 0) ? 0.5f * lt * s2 : lt * (s2 / (sqrtf(1.0f + s2) + 1.0f));
-            acc = ksum_add(acc, KSum{G.dxf * (le + v - f * hi), 0.f});
+            const double oi = d ? (double)__builtin_fmaf(al, d[i], x[i]) : (double)x[i];
+            const double oip = d ? (double)__builtin_fmaf(al, d[ip], x[ip]) : (double)x[ip];
+            const double hi = G.anchor + oi, hip = G.anchor + oip;
+            const double dh = (oip - oi) / G.dx;
+            const double s2 = dh * dh;
+            const double le = (G.model == 0) ? G.lt * s2 * 0.5 : G.lt * (s2 / (sqrt(1.0 + s2) + 1.0));
+            const double xc = (double)(i + ip) * 0.5 * G.dx;
+            acc += G.dx * (le + sample_v64<float>(G.nd, xc, (hip + hi) * 0.5) - G.f * hi);
         }
-        acc = block_ksum(acc, r.k);
-        return ((double)acc.s + (double)acc.c) - G.f * G.anchor * G.L;
+        return block_sum(acc, r.d);
     }
 }
 
@@ -292,7 +295,7 @@ __device__ void solve_line(const BatchArgs &a, int line, Red r, double *rho, dou
     if (lp.precondition) {
         const double pi = 3.14159265358979323846;
         kdx = L.lt / G.dx;
-        shift = lp.precond_shift;
+        shift = L.shift > 0 ? L.shift : lp.precond_shift;
         if (shift <= 0) {
             const double s1 = sin(pi / (double)n);
             shift = kdx * 4 * s1 * s1;

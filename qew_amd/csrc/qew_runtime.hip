@@ -12,6 +12,7 @@
 #include <cstring>
 #include <random>
 #include <string>
+#include <thread>
 
 #include "qew_internal.hpp"
 
@@ -213,6 +214,96 @@ void check_table_args(int nx, int ny, int dtype) {
     QEW_REQUIRE(dtype == QEW_F64 || dtype == QEW_F32, "dtype must be QEW_F64 or QEW_F32");
 }
 
+template <typename S, typename D>
+__global__ void convert_table(const S *src, D *dst, int64_t total) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < total) dst[k] = (D)src[k];
+}
+
+// Host part of FilteredNoise (src/filtered_noise.cpp:35-51): the mt19937_64
+// stream, one phase per mode in (i, j) loop order, drawn before the mask
+// test; the in-band modes as (flat index, cos, sin).
+std::vector<Mode> draw_modes(int nx, int ny, double dx, double dy, double xi_x, double xi_y,
+                             uint64_t seed) {
+    const double two_pi = 2.0 * std::acos(-1.0);
+    std::vector<Mode> modes;
+    std::mt19937_64 rng(seed);
+    std::uniform_real_distribution<double> phase(0, two_pi);
+    auto fftfreq = [](int k, int n) {
+        const int ks = (k < (n + 1) / 2) ? k : k - n;
+        return (double)ks / (double)n;
+    };
+    for (int i = 0; i < nx; ++i) {
+        const double fx = fftfreq(i, nx) / dx;
+        for (int j = 0; j < ny; ++j) {
+            const double fy = fftfreq(j, ny) / dy;
+            const double ph = phase(rng);
+            if (!((fx * xi_x) * (fx * xi_x) + (fy * xi_y) * (fy * xi_y) > 1.0))
+                modes.push_back(Mode{(int64_t)i * ny + j, std::cos(ph), std::sin(ph)});
+        }
+    }
+    return modes;
+}
+
+// Device part: scatter, inverse 2-D FFT, normalisation, prefilter, table.
+qew_noise build_from_modes(qew_ctx ctx, int nx, int ny, double dx, double dy, double amplitude,
+                           int dtype, const std::vector<Mode> &modes) {
+    qew_noise nz = new_noise(ctx, TABLE, dtype, nx, ny, dx, dy);
+    const int64_t total = (int64_t)nx * ny;
+    double2 *A = nullptr, *tmp = nullptr;
+    double *f = nullptr, *part = nullptr;
+    Mode *dm = nullptr;
+    const int nparts = 1024;
+    try {
+        QEW_HIP(hipMalloc(&A, sizeof(double2) * total));
+        QEW_HIP(hipMalloc(&tmp, sizeof(double2) * total));
+        QEW_HIP(hipMalloc(&f, sizeof(double) * total));
+        QEW_HIP(hipMalloc(&part, sizeof(double) * (nparts + 2)));
+        QEW_HIP(hipMemsetAsync(A, 0, sizeof(double2) * total, ctx->stream));
+        if (!modes.empty()) {
+            QEW_HIP(hipMalloc(&dm, sizeof(Mode) * modes.size()));
+            QEW_HIP(hipMemcpyAsync(dm, modes.data(), sizeof(Mode) * modes.size(),
+                                   hipMemcpyHostToDevice, ctx->stream));
+            hipLaunchKernelGGL(scatter_modes, grid1((int64_t)modes.size()), dim3(256), 0,
+                               ctx->stream, A, dm, (int64_t)modes.size());
+            QEW_HIP(hipGetLastError());
+        }
+        // ifft2 with the 1/(nx ny) backward normalisation
+        fft2(ctx->stream, A, tmp, nx, ny, +1);
+        hipLaunchKernelGGL(scale_complex, grid1(total), dim3(256), 0, ctx->stream, A, total,
+                           1.0 / ((double)nx * (double)ny));
+        // mean, then population variance about it (src/filtered_noise.cpp:61-76)
+        double *mean = part + nparts, *var = part + nparts + 1;
+        hipLaunchKernelGGL(moment_partial, dim3(nparts), dim3(256), 0, ctx->stream, A, total, 0,
+                           mean, part);
+        hipLaunchKernelGGL(moment_final, dim3(1), dim3(256), 0, ctx->stream, part, nparts, mean,
+                           1.0 / (double)total);
+        hipLaunchKernelGGL(moment_partial, dim3(nparts), dim3(256), 0, ctx->stream, A, total, 1,
+                           mean, part);
+        hipLaunchKernelGGL(moment_final, dim3(1), dim3(256), 0, ctx->stream, part, nparts, var,
+                           1.0 / (double)total);
+        hipLaunchKernelGGL(field_from_spectrum, grid1(total), dim3(256), 0, ctx->stream, A, f,
+                           total, amplitude, var);
+        QEW_HIP(hipGetLastError());
+        prefilter_to_table(nz, A, tmp);
+        if (total <= kKeepFieldMax) {
+            nz->field.resize(total);
+            QEW_HIP(hipMemcpyAsync(nz->field.data(), f, sizeof(double) * total,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+        }
+        QEW_HIP(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+        for (void *p : {(void *)A, (void *)tmp, (void *)f, (void *)part, (void *)dm})
+            if (p) (void)hipFree(p);
+        if (nz->table) (void)hipFree(nz->table);
+        delete nz;
+        throw;
+    }
+    for (void *p : {(void *)A, (void *)tmp, (void *)f, (void *)part, (void *)dm})
+        if (p) (void)hipFree(p);
+    return nz;
+}
+
 }  // namespace
 
 extern "C" {
@@ -387,79 +478,106 @@ qew_status qew_noise_generate(qew_ctx ctx, int nx, int ny, double Lx, double Ly,
         check_table_args(nx, ny, dtype);
         QEW_REQUIRE(Lx > 0 && Ly > 0, "box lengths must be positive");
         use_device(ctx);
-        const double dx = Lx / nx, dy = Ly / ny;
-        const double two_pi = 2.0 * std::acos(-1.0);
-        // host: the mt19937_64 stream, one phase per mode in (i, j) loop
-        // order, drawn before the mask test (src/filtered_noise.cpp:35-51)
-        std::vector<Mode> modes;
-        std::mt19937_64 rng(seed);
-        std::uniform_real_distribution<double> phase(0, two_pi);
-        auto fftfreq = [](int k, int n) {
-            const int ks = (k < (n + 1) / 2) ? k : k - n;
-            return (double)ks / (double)n;
-        };
-        for (int i = 0; i < nx; ++i) {
-            const double fx = fftfreq(i, nx) / dx;
-            for (int j = 0; j < ny; ++j) {
-                const double fy = fftfreq(j, ny) / dy;
-                const double ph = phase(rng);
-                if (!((fx * xi_x) * (fx * xi_x) + (fy * xi_y) * (fy * xi_y) > 1.0))
-                    modes.push_back(Mode{(int64_t)i * ny + j, std::cos(ph), std::sin(ph)});
-            }
+        const std::vector<Mode> modes = draw_modes(nx, ny, Lx / nx, Ly / ny, xi_x, xi_y, seed);
+        *out = build_from_modes(ctx, nx, ny, Lx / nx, Ly / ny, amplitude, dtype, modes);
+    });
+}
+
+qew_status qew_noise_generate_batch(qew_ctx ctx, int count, const int *nx, const int *ny,
+                                    const double *Lx, const double *Ly, const double *amplitude,
+                                    const double *xi_x, const double *xi_y, const uint64_t *seed,
+                                    int dtype, qew_noise *out) {
+    return guard([&] {
+        ctx_check(ctx);
+        QEW_REQUIRE(count >= 0, "negative count");
+        QEW_REQUIRE(!count || (nx && ny && Lx && Ly && amplitude && xi_x && xi_y && seed && out),
+                    "null pointer");
+        for (int k = 0; k < count; ++k) {
+            check_table_args(nx[k], ny[k], dtype);
+            QEW_REQUIRE(Lx[k] > 0 && Ly[k] > 0, "box lengths must be positive");
+            out[k] = nullptr;
         }
-        qew_noise nz = new_noise(ctx, TABLE, dtype, nx, ny, dx, dy);
-        const int64_t total = (int64_t)nx * ny;
-        double2 *A = nullptr, *tmp = nullptr;
-        double *f = nullptr, *part = nullptr;
-        Mode *dm = nullptr;
-        const int nparts = 1024;
+        use_device(ctx);
+        // each table's phase stream is independent: draw up to `width` tables
+        // on host threads, then build them on the device in order
+        const int width = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
         try {
-            QEW_HIP(hipMalloc(&A, sizeof(double2) * total));
-            QEW_HIP(hipMalloc(&tmp, sizeof(double2) * total));
-            QEW_HIP(hipMalloc(&f, sizeof(double) * total));
-            QEW_HIP(hipMalloc(&part, sizeof(double) * (nparts + 2)));
-            QEW_HIP(hipMemsetAsync(A, 0, sizeof(double2) * total, ctx->stream));
-            if (!modes.empty()) {
-                QEW_HIP(hipMalloc(&dm, sizeof(Mode) * modes.size()));
-                QEW_HIP(hipMemcpyAsync(dm, modes.data(), sizeof(Mode) * modes.size(),
-                                       hipMemcpyHostToDevice, ctx->stream));
-                hipLaunchKernelGGL(scatter_modes, grid1((int64_t)modes.size()), dim3(256), 0,
-                                   ctx->stream, A, dm, (int64_t)modes.size());
-                QEW_HIP(hipGetLastError());
+            for (int k0 = 0; k0 < count; k0 += width) {
+                const int k1 = std::min(count, k0 + width);
+                std::vector<std::vector<Mode>> modes(k1 - k0);
+                std::vector<std::thread> pool;
+                for (int k = k0; k < k1; ++k)
+                    pool.emplace_back([&, k] {
+                        modes[k - k0] = draw_modes(nx[k], ny[k], Lx[k] / nx[k], Ly[k] / ny[k],
+                                                   xi_x[k], xi_y[k], seed[k]);
+                    });
+                for (auto &t : pool) t.join();
+                for (int k = k0; k < k1; ++k) {
+                    out[k] = build_from_modes(ctx, nx[k], ny[k], Lx[k] / nx[k], Ly[k] / ny[k],
+                                              amplitude[k], dtype, modes[k - k0]);
+                    std::vector<Mode>().swap(modes[k - k0]);
+                }
             }
-            // ifft2 with the 1/(nx ny) backward normalisation
-            fft2(ctx->stream, A, tmp, nx, ny, +1);
-            hipLaunchKernelGGL(scale_complex, grid1(total), dim3(256), 0, ctx->stream, A, total,
-                               1.0 / ((double)nx * (double)ny));
-            // mean, then population variance about it (src/filtered_noise.cpp:61-76)
-            double *mean = part + nparts, *var = part + nparts + 1;
-            hipLaunchKernelGGL(moment_partial, dim3(nparts), dim3(256), 0, ctx->stream, A, total, 0,
-                               mean, part);
-            hipLaunchKernelGGL(moment_final, dim3(1), dim3(256), 0, ctx->stream, part, nparts, mean,
-                               1.0 / (double)total);
-            hipLaunchKernelGGL(moment_partial, dim3(nparts), dim3(256), 0, ctx->stream, A, total, 1,
-                               mean, part);
-            hipLaunchKernelGGL(moment_final, dim3(1), dim3(256), 0, ctx->stream, part, nparts, var,
-                               1.0 / (double)total);
-            hipLaunchKernelGGL(field_from_spectrum, grid1(total), dim3(256), 0, ctx->stream, A, f,
-                               total, amplitude, var);
-            QEW_HIP(hipGetLastError());
-            prefilter_to_table(nz, A, tmp);
-            if (total <= kKeepFieldMax) {
-                nz->field.resize(total);
-                QEW_HIP(hipMemcpyAsync(nz->field.data(), f, sizeof(double) * total,
-                                       hipMemcpyDeviceToHost, ctx->stream));
-            }
-            QEW_HIP(hipStreamSynchronize(ctx->stream));
         } catch (...) {
-            for (void *p : {(void *)A, (void *)tmp, (void *)f, (void *)part, (void *)dm})
-                if (p) (void)hipFree(p);
-            if (nz->table) (void)hipFree(nz->table);
-            delete nz;
+            for (int k = 0; k < count; ++k)
+                if (out[k]) {
+                    qew_noise_destroy(out[k]);
+                    out[k] = nullptr;
+                }
             throw;
         }
-        for (void *p : {(void *)A, (void *)tmp, (void *)f, (void *)part, (void *)dm})
-            if (p) (void)hipFree(p);
+    });
+}
+
+qew_status qew_noise_convert(qew_noise noise, int dtype, qew_noise *out) {
+    return guard([&] {
+        QEW_REQUIRE(noise && out, "null pointer");
+        QEW_REQUIRE(dtype == QEW_F64 || dtype == QEW_F32, "dtype must be QEW_F64 or QEW_F32");
+        qew_ctx ctx = noise->ctx;
+        use_device(ctx);
+        qew_noise nz = new_noise(ctx, noise->desc.kind, dtype, noise->desc.nx, noise->desc.ny,
+                                 noise->desc.dx, noise->desc.dy);
+        for (int k = 0; k < 4; ++k) nz->desc.p[k] = noise->desc.p[k];
+        nz->field = noise->field;
+        if (noise->desc.kind == TABLE) {
+            const int pitch = noise->desc.pitch;
+            const int64_t total = (int64_t)pitch * noise->desc.ny;
+            const size_t es = dtype == QEW_F32 ? sizeof(float) : sizeof(double);
+            hipError_t e = hipMalloc(&nz->table, es * (size_t)total);
+            if (e != hipSuccess) {
+                delete nz;
+                QEW_HIP(e);
+            }
+            nz->desc.pitch = pitch;
+            nz->desc.coef = nz->table;
+            const bool from32 = noise->desc.dtype == 1;
+            if (dtype == QEW_F32) {
+                if (from32)
+                    hipLaunchKernelGGL((convert_table<float, float>), grid1(total), dim3(256), 0,
+                                       ctx->stream, static_cast<const float *>(noise->table),
+                                       static_cast<float *>(nz->table), total);
+                else
+                    hipLaunchKernelGGL((convert_table<double, float>), grid1(total), dim3(256), 0,
+                                       ctx->stream, static_cast<const double *>(noise->table),
+                                       static_cast<float *>(nz->table), total);
+            } else {
+                if (from32)
+                    hipLaunchKernelGGL((convert_table<float, double>), grid1(total), dim3(256), 0,
+                                       ctx->stream, static_cast<const float *>(noise->table),
+                                       static_cast<double *>(nz->table), total);
+                else
+                    hipLaunchKernelGGL((convert_table<double, double>), grid1(total), dim3(256), 0,
+                                       ctx->stream, static_cast<const double *>(noise->table),
+                                       static_cast<double *>(nz->table), total);
+            }
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) {
+                (void)hipFree(nz->table);
+                delete nz;
+                QEW_HIP(e);
+            }
+        }
         *out = nz;
     });
 }
@@ -612,6 +730,17 @@ qew_status qew_batch_offsets(qew_batch b, int64_t *offsets_host) {
     return guard([&] {
         QEW_REQUIRE(b && offsets_host, "null pointer");
         std::memcpy(offsets_host, b->offsets.data(), sizeof(int64_t) * (b->nlines + 1));
+    });
+}
+
+qew_status qew_batch_set_precond_shifts(qew_batch b, const double *shifts) {
+    return guard([&] {
+        QEW_REQUIRE(b && shifts, "null pointer");
+        for (int l = 0; l < b->nlines; ++l) b->lines[l].shift = shifts[l];
+        use_device(b->ctx);
+        QEW_HIP(hipMemcpyAsync(b->d_lines.ptr, b->lines.data(), sizeof(LineDesc) * b->nlines,
+                               hipMemcpyHostToDevice, b->ctx->stream));
+        QEW_HIP(hipStreamSynchronize(b->ctx->stream));
     });
 }
 

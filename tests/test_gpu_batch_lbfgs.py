@@ -109,3 +109,39 @@ def test_batch_lbfgs_rejects_bad_lengths_for_fft(q):
     b = q.Batch([100], [well], [p])
     with pytest.raises(ValueError):
         b.lbfgs(dev(np.zeros(100)), q.LbfgsParams(precondition=True))
+
+
+def test_batch_lbfgs_long_lines_match_oracle(q):
+    """Config-4 line lengths (2048 .. 16384, tables L x 2048): the persistent
+    kernel's multi-butterfly LDS FFT stages and long reductions. FP64 must
+    reach the oracle's minimum (energy 1e-10 relative) in a similar number of
+    iterations; FP32 within the stated 1e-6 relative energy."""
+    lengths = [2048, 8192, 16384]
+    xis = [2.0, 4.0, 8.0]
+    models = [0, 1, 0]
+    noises, oracle, params, h0 = make_problem(q, lengths, 2048, xis, models, [21, 22, 23], q.F64)
+    shifts = [0.5 / xi ** 2 for xi in xis]
+    refs = []
+    for k in range(3):
+        on, po = oracle[k]
+        h_ref, r_ref = O.lbfgs(h0[k], po, on, O.lbfgs_params(m=10, ftol=1e-5, max_iter=4000,
+                                                             precondition=True,
+                                                             precond_shift=shifts[k]), omp=True)
+        assert r_ref.converged
+        refs.append((h_ref, r_ref, O.objective(h_ref, po, on)))
+    for dtype, tol in ((q.F64, 1e-10), (q.F32, 1e-6)):
+        nz = noises if dtype == q.F64 else [n.convert(q.F32) for n in noises]
+        b = q.Batch(lengths, nz, params)
+        b.set_precond_shifts(shifts)
+        h = dev(np.concatenate(h0))
+        res = b.lbfgs(h, q.LbfgsParams(m=10, ftol=1e-5, max_iter=4000, precondition=True),
+                      dtype=dtype)
+        hh = host(h)
+        for k in range(3):
+            on, po = oracle[k]
+            h_ref, r_ref, e_ref = refs[k]
+            assert res[k].converged, (dtype, k, res[k])
+            if dtype == q.F64:
+                assert abs(res[k].iterations - r_ref.iterations) <= max(5, r_ref.iterations // 5)
+            e = O.objective(hh[b.offsets[k]:b.offsets[k + 1]], po, on)
+            assert rel(e, e_ref) < tol, (dtype, k, e, e_ref)
