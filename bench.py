@@ -122,7 +122,9 @@ def bench_lbfgs(log, nlines=512, plain_sample=8):
     finals = {}
     # ftol 1e-5: above the Armijo roundoff floor of the longest lines (L=16384,
     # |E| ~ 1e4), where the reference algorithm converges in ~100-300 iterations
-    for name, noises, dtype, ftol in (("f64", n64, q.F64, 1e-5), ("f32", n32, q.F32, 1e-5)):
+    # FP32: the line offsets' resolution floors the FP32 gradient near 1e-4, so
+    # FP32 stops there and is judged by its energy against the FP64 minimum
+    for name, noises, dtype, ftol in (("f64", n64, q.F64, 1e-5), ("f32", n32, q.F32, 1e-4)):
         b = q.Batch(lengths, noises, params)
         b.set_precond_shifts(shifts)
         lp = q.LbfgsParams(m=10, ftol=ftol, max_iter=10000, precondition=True)

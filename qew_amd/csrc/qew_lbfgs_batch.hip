@@ -387,14 +387,28 @@ __device__ void solve_line(const BatchArgs &a, int line, Red r, double *rho, dou
         }
         ls_fail_streak = 0;
         const T alT = (T)alpha;
-        for (int i = tid; i < n; i += blockDim.x) x[i] += alT * d[i];
+        // curvature pair into the spare buffers. FP64: s = alpha d as in the
+        // reference. FP32: s is the step the state actually took,
+        // x_new - x_old (exact: both are FP32 values within a factor 2 of
+        // each other, or the difference is exactly representable), because
+        // near convergence alpha d falls below the offsets' resolution and
+        // the intended step no longer matches the realised one.
+        T *st = Sb(sbuf[m]), *yt = Yb(sbuf[m]);
+        for (int i = tid; i < n; i += blockDim.x) {
+            if constexpr (sizeof(T) == 8) {
+                x[i] += alT * d[i];
+            } else {
+                const T xo = x[i];
+                const T xn = xo + alT * d[i];
+                x[i] = xn;
+                st[i] = xn - xo;
+            }
+        }
         __syncthreads();
         wg_gradient<T, C>(G, x, gn, edy, esl);
-        // curvature pair into the spare buffers, FP64 sums
-        T *st = Sb(sbuf[m]), *yt = Yb(sbuf[m]);
         double sy = 0, ss = 0, yy = 0;
         for (int i = tid; i < n; i += blockDim.x) {
-            const T s = alT * d[i];
+            const T s = (sizeof(T) == 8) ? alT * d[i] : st[i];
             const T y = gn[i] - g[i];
             st[i] = s;
             yt[i] = y;

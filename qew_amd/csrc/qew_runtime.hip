@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstring>
 #include <random>
+#include <future>
 #include <string>
 #include <thread>
 
@@ -245,27 +246,38 @@ std::vector<Mode> draw_modes(int nx, int ny, double dx, double dy, double xi_x, 
     return modes;
 }
 
+// Device scratch of the generator, reused across the tables of a batch.
+struct GenScratch {
+    qew_host::DevBuf A, tmp, f, part, modes;
+    void release() {
+        A.release();
+        tmp.release();
+        f.release();
+        part.release();
+        modes.release();
+    }
+};
+
 // Device part: scatter, inverse 2-D FFT, normalisation, prefilter, table.
 qew_noise build_from_modes(qew_ctx ctx, int nx, int ny, double dx, double dy, double amplitude,
-                           int dtype, const std::vector<Mode> &modes) {
+                           int dtype, const std::vector<Mode> &modes, GenScratch &g) {
     qew_noise nz = new_noise(ctx, TABLE, dtype, nx, ny, dx, dy);
     const int64_t total = (int64_t)nx * ny;
-    double2 *A = nullptr, *tmp = nullptr;
-    double *f = nullptr, *part = nullptr;
-    Mode *dm = nullptr;
     const int nparts = 1024;
     try {
-        QEW_HIP(hipMalloc(&A, sizeof(double2) * total));
-        QEW_HIP(hipMalloc(&tmp, sizeof(double2) * total));
-        QEW_HIP(hipMalloc(&f, sizeof(double) * total));
-        QEW_HIP(hipMalloc(&part, sizeof(double) * (nparts + 2)));
+        g.A.ensure(sizeof(double2) * total);
+        g.tmp.ensure(sizeof(double2) * total);
+        g.f.ensure(sizeof(double) * total);
+        g.part.ensure(sizeof(double) * (nparts + 2));
+        double2 *A = g.A.as<double2>(), *tmp = g.tmp.as<double2>();
+        double *f = g.f.as<double>(), *part = g.part.as<double>();
         QEW_HIP(hipMemsetAsync(A, 0, sizeof(double2) * total, ctx->stream));
         if (!modes.empty()) {
-            QEW_HIP(hipMalloc(&dm, sizeof(Mode) * modes.size()));
-            QEW_HIP(hipMemcpyAsync(dm, modes.data(), sizeof(Mode) * modes.size(),
+            g.modes.ensure(sizeof(Mode) * modes.size());
+            QEW_HIP(hipMemcpyAsync(g.modes.ptr, modes.data(), sizeof(Mode) * modes.size(),
                                    hipMemcpyHostToDevice, ctx->stream));
             hipLaunchKernelGGL(scatter_modes, grid1((int64_t)modes.size()), dim3(256), 0,
-                               ctx->stream, A, dm, (int64_t)modes.size());
+                               ctx->stream, A, g.modes.as<Mode>(), (int64_t)modes.size());
             QEW_HIP(hipGetLastError());
         }
         // ifft2 with the 1/(nx ny) backward normalisation
@@ -293,14 +305,10 @@ qew_noise build_from_modes(qew_ctx ctx, int nx, int ny, double dx, double dy, do
         }
         QEW_HIP(hipStreamSynchronize(ctx->stream));
     } catch (...) {
-        for (void *p : {(void *)A, (void *)tmp, (void *)f, (void *)part, (void *)dm})
-            if (p) (void)hipFree(p);
         if (nz->table) (void)hipFree(nz->table);
         delete nz;
         throw;
     }
-    for (void *p : {(void *)A, (void *)tmp, (void *)f, (void *)part, (void *)dm})
-        if (p) (void)hipFree(p);
     return nz;
 }
 
@@ -479,7 +487,14 @@ qew_status qew_noise_generate(qew_ctx ctx, int nx, int ny, double Lx, double Ly,
         QEW_REQUIRE(Lx > 0 && Ly > 0, "box lengths must be positive");
         use_device(ctx);
         const std::vector<Mode> modes = draw_modes(nx, ny, Lx / nx, Ly / ny, xi_x, xi_y, seed);
-        *out = build_from_modes(ctx, nx, ny, Lx / nx, Ly / ny, amplitude, dtype, modes);
+        GenScratch g;
+        try {
+            *out = build_from_modes(ctx, nx, ny, Lx / nx, Ly / ny, amplitude, dtype, modes, g);
+        } catch (...) {
+            g.release();
+            throw;
+        }
+        g.release();
     });
 }
 
@@ -498,27 +513,33 @@ qew_status qew_noise_generate_batch(qew_ctx ctx, int count, const int *nx, const
             out[k] = nullptr;
         }
         use_device(ctx);
-        // each table's phase stream is independent: draw up to `width` tables
-        // on host threads, then build them on the device in order
-        const int width = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+        // each table's phase stream is independent: host threads draw the
+        // next chunk of `width` tables while the device builds the current one
+        const int width = std::max(1, std::min(8, (int)std::thread::hardware_concurrency()));
+        GenScratch g;
+        using Chunk = std::vector<std::future<std::vector<Mode>>>;
+        auto launch_chunk = [&](int k0) {
+            Chunk c;
+            for (int k = k0; k < std::min(count, k0 + width); ++k)
+                c.push_back(std::async(std::launch::async, [=] {
+                    return draw_modes(nx[k], ny[k], Lx[k] / nx[k], Ly[k] / ny[k], xi_x[k], xi_y[k],
+                                      seed[k]);
+                }));
+            return c;
+        };
         try {
+            Chunk cur = launch_chunk(0);
             for (int k0 = 0; k0 < count; k0 += width) {
-                const int k1 = std::min(count, k0 + width);
-                std::vector<std::vector<Mode>> modes(k1 - k0);
-                std::vector<std::thread> pool;
-                for (int k = k0; k < k1; ++k)
-                    pool.emplace_back([&, k] {
-                        modes[k - k0] = draw_modes(nx[k], ny[k], Lx[k] / nx[k], Ly[k] / ny[k],
-                                                   xi_x[k], xi_y[k], seed[k]);
-                    });
-                for (auto &t : pool) t.join();
-                for (int k = k0; k < k1; ++k) {
+                Chunk next = (k0 + width < count) ? launch_chunk(k0 + width) : Chunk();
+                for (int k = k0; k < std::min(count, k0 + width); ++k) {
+                    const std::vector<Mode> modes = cur[k - k0].get();
                     out[k] = build_from_modes(ctx, nx[k], ny[k], Lx[k] / nx[k], Ly[k] / ny[k],
-                                              amplitude[k], dtype, modes[k - k0]);
-                    std::vector<Mode>().swap(modes[k - k0]);
+                                              amplitude[k], dtype, modes, g);
                 }
+                cur = std::move(next);
             }
         } catch (...) {
+            g.release();
             for (int k = 0; k < count; ++k)
                 if (out[k]) {
                     qew_noise_destroy(out[k]);
@@ -526,6 +547,7 @@ qew_status qew_noise_generate_batch(qew_ctx ctx, int count, const int *nx, const
                 }
             throw;
         }
+        g.release();
     });
 }
 

@@ -134,7 +134,11 @@ def test_batch_lbfgs_long_lines_match_oracle(q):
         b = q.Batch(lengths, nz, params)
         b.set_precond_shifts(shifts)
         h = dev(np.concatenate(h0))
-        res = b.lbfgs(h, q.LbfgsParams(m=10, ftol=1e-5, max_iter=4000, precondition=True),
+        # FP32 line offsets resolve ~ulp(|offset|) ~ 4e-6 cells at 50 cells from the
+        # anchor, which floors the FP32 gradient near 1e-4 x curvature scale:
+        # FP32 runs stop at ftol 1e-4 and are judged by the energy they reach
+        ftol = 1e-5 if dtype == q.F64 else 1e-4
+        res = b.lbfgs(h, q.LbfgsParams(m=10, ftol=ftol, max_iter=4000, precondition=True),
                       dtype=dtype)
         hh = host(h)
         for k in range(3):

@@ -180,3 +180,15 @@ def test_batch_eval_fp32_compensated_error(q):
     e_ref = O.objective(h, p_o, on)
     assert np.max(np.abs(host(g) - g_ref)) < 2e-4 * np.max(np.abs(g_ref))
     assert rel(host(e)[0], e_ref) < 1e-5
+
+
+def test_noise_generate_batch_equals_single_and_convert(q):
+    specs = [(64, 48, 3.2, 1.0, 1.5, 0.15, 0.15, 3), (48, 40, 2.5, 1.3, 1.0, 0.25, 0.25, 17),
+             (100, 30, 4.0, 1.0, 2.0, 0.2, 0.1, 42), (32, 32, 32.0, 32.0, 1.0, 4.0, 4.0, 9)]
+    batch = q.FilteredNoise.generate_batch(specs)
+    for spec, nb in zip(specs, batch):
+        single = q.FilteredNoise(*spec)
+        np.testing.assert_array_equal(nb.coefficients(), single.coefficients())
+        np.testing.assert_array_equal(nb.field(), single.field())
+        c32 = nb.convert(q.F32).coefficients()
+        np.testing.assert_array_equal(c32, nb.coefficients().astype(np.float32).astype(np.float64))

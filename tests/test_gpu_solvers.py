@@ -239,3 +239,47 @@ def test_batch_rk_matches_single_lines(q):
         r = q.rk_block(h, params[k], gn, rp)
         assert (res[k].blocked, res[k].runaway, res[k].sweeps) == (r.blocked, r.runaway, r.sweeps)
         np.testing.assert_array_equal(host(hb)[64 * k:64 * (k + 1)], host(h))
+
+
+def test_rk_fp32_table_bitwise_vs_oracle_on_rounded_table(q):
+    # FP32 storage, FP64 arithmetic: identical to the FP64 oracle run on the
+    # FP32-rounded coefficients
+    field, coef = O.filtered_noise(64, 64, 3.2, 1.0, 0.6, 0.2, 0.2, 29)
+    coef32 = coef.astype(np.float32).astype(np.float64)
+    on = O.OracleNoise.table(coef32, 3.2, 1.0)
+    gn = q.FilteredNoise.from_coefficients(coef, 0.05, 1.0 / 64, dtype=q.F32)
+    rpo = O.rk_params(dstep=0.5 / 64, max_advance=1.0, root_tol=1e-11, sweep_tol=1e-9, runaway=3.0)
+    h_ref = np.full(64, 0.5)
+    r_ref = O.rk_block(h_ref, O.params(3.2, 1.2, 0.1, 1), on, rpo)
+    h = dev(np.full(64, 0.5))
+    r = q.rk_block(h, q.Params(3.2, 1.2, 0.1, q.Model.Arclength), gn, q.RkParams(0.5 / 64, 1.0, 1e-11, 1e-9, 3.0))
+    assert (r.blocked, r.sweeps) == (bool(r_ref.blocked), r_ref.sweeps)
+    np.testing.assert_array_equal(host(h), h_ref)
+
+
+def test_batch_langevin_fp32_vs_fp64_statistics(q):
+    """FP32 line state (offsets from an anchor) + FP32 table against FP64 on
+    the same table values and the same Philox normals: mean height and
+    roughness W^2 after 2000 steps agree to 1e-5 relative."""
+    torch = torch_cuda()
+    n, ny = 4096, 1024
+    field, coef = O.filtered_noise(n, ny, float(n), float(ny), 1.0, 4.0, 4.0, 3)
+    g64 = q.FilteredNoise.from_coefficients(coef.astype(np.float32).astype(np.float64), 1.0, 1.0)
+    g32 = g64.convert(q.F32)
+    p = q.Params(float(n), 1.0, 0.05, q.Model.Linear)
+    steps = 2000
+    drift = np.full(steps, 0.01)
+    diff = np.full(steps, np.sqrt(2 * 0.01 * 0.05))
+    h0 = np.full(n, ny / 2.0)
+    b64 = q.Batch([n], [g64], [p])
+    b32 = q.Batch([n], [g32], [p])
+    h64 = dev(h0)
+    b64.langevin(h64, drift, diff, seed=11)
+    anchor = dev(np.array([ny / 2.0]))
+    o32 = dev(np.zeros(n, dtype=np.float32))
+    b32.langevin(o32, drift, diff, seed=11, dtype=q.F32, anchor=anchor)
+    a = host(h64)
+    b = ny / 2.0 + host(o32).astype(np.float64)
+    assert rel(b.mean(), a.mean()) < 1e-5
+    w2a, w2b = a.var(), b.var()
+    assert rel(w2b, w2a) < 1e-4, (w2a, w2b)
