@@ -139,6 +139,9 @@ def bench_lbfgs(log, nlines=512, plain_sample=8):
         its = [r.iterations for r in res]
         out[name] = {"minima_per_s": nlines / dt, "seconds": dt, "ftol": ftol,
                      "converged": int(sum(r.converged for r in res)),
+                     "curvature_skips": int(sum(r.n_curvature_skips for r in res)),
+                     "ls_restarts": int(sum(r.n_ls_restarts for r in res)),
+                     "slowest_line": int(np.argmax(its)),
                      "iterations_mean": float(np.mean(its)), "iterations_max": int(max(its))}
         log(f"lbfgs {name}: {json.dumps(out[name])}")
         del b, h

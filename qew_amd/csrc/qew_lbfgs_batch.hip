@@ -419,6 +419,17 @@ __device__ void solve_line(const BatchArgs &a, int line, Red r, double *rho, dou
         sy = wg_sum(sy, r);
         ss = wg_sum(ss, r);
         yy = wg_sum(yy, r);
+        if (sizeof(T) == 4 && ss == 0.0) {
+            // FP32 state: the accepted step is below the offsets' resolution
+            // everywhere, so the line can no longer move; stop (not
+            // converged) instead of iterating in place until max_iter
+            res.iterations = it + 1;
+            res.max_force = wg_maxabs(gn, n, r);
+            res.converged = res.max_force < lp.ftol ? 1 : 0;
+            g = gn;
+            finished = true;
+            break;
+        }
         if (sy > lp.curvature_eps * sqrt(ss * yy)) {
             int row;
             __syncthreads();
