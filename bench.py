@@ -32,6 +32,11 @@ XI = 8.0
 EVALS = 10000
 SHIFT = 8
 NSHIFT = NY // SHIFT
+# Rosso-Krauth estimate of the depinning force of the Langevin workload's
+# disorder (65536 x 4096 table, xi = 4, seed 1, linear, line tension 1, FP32
+# table, the depinning tool's knobs): blocked at 0.5078, not blocked at
+# 0.5156 (a 2*10^5-sweep timeout); ramp and bisection as in bench_rk.
+FC_EST_LANGEVIN = 0.512
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 
@@ -170,6 +175,142 @@ def bench_lbfgs(log, nlines=512, plain_sample=8):
     return out
 
 
+def bench_langevin(log, nsteps=100000):
+    """Finite-temperature anneal: one line of L = 65536 sites on a 65536 x 4096
+    FilteredNoise table (xi = 4, seed 1), linear elasticity, line tension 1,
+    flat start at Ny/2, f = 0.5 f_c(est). Euler-Maruyama with dt = 0.01 for
+    10^5 steps, temperature annealed linearly from 0.5 to 0.01 in the drivers'
+    parameterisation (drift = T dt, diffusion = T sqrt(2 dt) for A = tau = 1,
+    with the drivers' one-step lag, src/qew_runs.h:154-170), Philox normals
+    keyed by (seed 2, step, site). FP64 and FP32 (anchored offsets, FP32
+    table) run the same noise; the mean velocity and W^2 are compared."""
+    import torch
+    import qew_amd as q
+    Lc, ny = 65536, 4096
+    t0 = time.perf_counter()
+    n64 = q.FilteredNoise(Lc, ny, float(Lc), float(ny), 1.0, 4.0, 4.0, 1, dtype=q.F64)
+    n32 = n64.convert(q.F32)
+    torch.cuda.synchronize()
+    log(f"langevin: table generated in {time.perf_counter() - t0:.1f} s")
+    f = 0.5 * FC_EST_LANGEVIN
+    dt = 0.01
+    total = nsteps * dt
+    k = np.arange(nsteps)
+    temp = np.where(k == 0, 0.5, 0.5 + (0.01 - 0.5) * ((k - 1) * dt / total))
+    drift = temp * dt
+    diff = temp * np.sqrt(2 * dt)
+    p = q.Params(float(Lc), 1.0, f, q.Model.Linear)
+    out = {}
+    for name, nz, dtype in (("f64", n64, q.F64), ("f32", n32, q.F32)):
+        b = q.Batch([Lc], [nz], [p])
+        if dtype == q.F64:
+            h = torch.full((Lc,), ny / 2.0, dtype=torch.float64, device="cuda:0")
+            anchor = None
+        else:
+            h = torch.zeros(Lc, dtype=torch.float32, device="cuda:0")
+            anchor = torch.full((1,), ny / 2.0, dtype=torch.float64, device="cuda:0")
+        b.langevin(h.clone(), drift[:100], diff[:100], seed=2, step0=0, dtype=dtype, anchor=anchor)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        chunk = 10000
+        for s in range(0, nsteps, chunk):
+            b.langevin(h, drift[s:s + chunk], diff[s:s + chunk], seed=2, step0=s, dtype=dtype,
+                       anchor=anchor)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t1
+        hh = h.double().cpu().numpy() + (ny / 2.0 if dtype == q.F32 else 0.0)
+        out[name] = {"site_steps_per_s": nsteps * Lc / sec, "us_per_step": 1e6 * sec / nsteps,
+                     "seconds": sec, "mean_velocity": float((hh.mean() - ny / 2.0) / total),
+                     "W2": float(hh.var())}
+        log(f"langevin {name}: {json.dumps(out[name])}")
+    out["f32_vs_f64_rel_err"] = {
+        "mean_velocity": abs(out["f32"]["mean_velocity"] - out["f64"]["mean_velocity"])
+        / abs(out["f64"]["mean_velocity"]),
+        "W2": abs(out["f32"]["W2"] - out["f64"]["W2"]) / out["f64"]["W2"]}
+    out["workload"] = (f"L={Lc}, table {Lc}x{ny}, xi=4, linear, f=0.5*f_c(est)={f:g}, "
+                       f"dt={dt}, {nsteps} steps, T 0.5 -> 0.01")
+    log(f"langevin: {json.dumps(out)}")
+    return out
+
+
+def bench_rk(log, nlines=64, budget_s=20.0):
+    """Batched Rosso-Krauth depinning: nlines realizations of L = 16384 sites,
+    each on its own 16384 x 1024 table (period M = 1024 in h, xi = 4, seeds
+    1..nlines), linear elasticity, line tension 1, the depinning tool's knobs
+    (dstep 0.5, max_advance Ly, runaway 2 Ly, root_tol 1e-10, sweep_tol 1e-8,
+    max_sweeps 200000). Relax at f = 0, ramp f = 0.125 * 2^k per line until it
+    runs away, then bisect every line's bracket, each probe warm-started from
+    the line's highest blocked configuration (no-passing). Probes are issued
+    while `budget_s` lasts; near f_c a probe takes 10^4..2*10^5 sweeps, so the
+    bracket after the budget is reported, not f_c to 1e-10. FP32 tables (FP64
+    state, comparisons and bisection) and FP64 tables run the same probes."""
+    import torch
+    import qew_amd as q
+    Lr, ny = 16384, 1024
+    t0 = time.perf_counter()
+    specs = [(Lr, ny, float(Lr), float(ny), 1.0, 4.0, 4.0, 1 + k) for k in range(nlines)]
+    n64 = q.FilteredNoise.generate_batch(specs, dtype=q.F64)
+    n32 = [nz.convert(q.F32) for nz in n64]
+    torch.cuda.synchronize()
+    log(f"rk: {nlines} tables generated in {time.perf_counter() - t0:.1f} s")
+    params = [q.Params(float(Lr), 1.0, 0.0, q.Model.Linear)] * nlines
+    rp = q.RkParams(0.5, float(ny), 1e-10, 1e-8, 2.0 * ny, 200000)
+    out = {}
+    brackets = {}
+    for name, noises in (("f32", n32), ("f64", n64)):
+        b = q.Batch([Lr] * nlines, noises, params)
+        h_lo = torch.full((nlines * Lr,), ny / 2.0, dtype=torch.float64, device="cuda:0")
+        lo = np.zeros(nlines)
+        hi = np.full(nlines, np.inf)
+        evals = 0
+        sweeps = 0
+        probes = 0
+        f_ramp = 0.125
+        t1 = time.perf_counter()
+        forces = np.zeros(nlines)
+        while True:
+            b.set_forces(forces)
+            h = h_lo.clone()
+            res = b.rk_block(h, rp)
+            probes += 1
+            evals += sum(r.n_velocity_evals for r in res)
+            sweeps += max(r.sweeps for r in res)
+            blocked = np.array([r.blocked for r in res])
+            if probes == 1:
+                assert blocked.all(), "a line does not block at f = 0"
+                h_lo = h
+            else:
+                m = torch.tensor(np.repeat(blocked & todo, Lr), device="cuda:0")
+                h_lo = torch.where(m, h, h_lo)
+                lo = np.where(blocked & todo, forces, lo)
+                hi = np.where(~blocked & todo, forces, hi)
+            todo = np.isinf(hi) | (hi - lo > 1e-10 * np.maximum(np.abs(lo), np.abs(hi)))
+            if not todo.any() or time.perf_counter() - t1 > budget_s:
+                break
+            forces = np.where(np.isinf(hi), np.maximum(2 * lo, f_ramp), 0.5 * (lo + hi))
+            forces = np.where(todo, forces, lo)  # finished lines: one blocked sweep
+            f_ramp *= 2
+        sec = time.perf_counter() - t1
+        brackets[name] = (lo.copy(), hi.copy())
+        width = (hi - lo) / np.maximum(hi, 1e-300)
+        out[name] = {"velocity_evals_per_s": evals / sec, "sweeps_per_s": sweeps / sec,
+                     "seconds": sec, "probes": probes, "sweeps": sweeps,
+                     "velocity_evals": int(evals),
+                     "f_c_bracket_median": [float(np.median(lo)), float(np.median(hi))],
+                     "bracket_rel_width_max": float(width.max())}
+        log(f"rk {name}: {json.dumps(out[name])}")
+        del b, h, h_lo
+    same = (np.array_equal(brackets["f32"][0], brackets["f64"][0])
+            and np.array_equal(brackets["f32"][1], brackets["f64"][1]))
+    out["f32_f64_same_brackets"] = bool(same)
+    out["workload"] = (f"{nlines} lines x L={Lr}, tables {Lr}x{ny} (period M={ny}), xi=4, "
+                       f"linear, f_c ramp + bisection, {budget_s:g} s budget per precision")
+    out["velocity_evals_note"] = ("evaluations the reference algorithm makes (sites whose "
+                                  "inputs did not change are skipped on the GPU but counted)")
+    log(f"rk: {json.dumps(out)}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,6 +319,10 @@ def main():
     ap.add_argument("--evals", type=int, default=EVALS)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lbfgs", action="store_true", help="skip the batched L-BFGS workload")
+    ap.add_argument("--no-langevin", action="store_true", help="skip the Langevin anneal")
+    ap.add_argument("--no-rk", action="store_true", help="skip the batched Rosso-Krauth workload")
+    ap.add_argument("--rk-budget", type=float, default=20.0,
+                    help="seconds of f_c bisection per precision in the Rosso-Krauth workload")
     ap.add_argument("--lbfgs-lines", type=int, default=512)
     ap.add_argument("--only", choices=["both", "f32", "f64"], default="both")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -318,6 +463,12 @@ def main():
     if not args.no_lbfgs:
         torch.cuda.empty_cache()
         line["batched_lbfgs"] = bench_lbfgs(log, nlines=args.lbfgs_lines)
+    if not args.no_langevin:
+        torch.cuda.empty_cache()
+        line["langevin"] = bench_langevin(log)
+    if not args.no_rk:
+        torch.cuda.empty_cache()
+        line["rosso_krauth"] = bench_rk(log, budget_s=args.rk_budget)
     print(json.dumps(line), flush=True)
 
 

@@ -112,6 +112,7 @@ typedef struct qew_rk_result {
     int sweeps;
     double total_advance;
     double mean_h;
+    long n_velocity_evals; /* rk_site_velocity calls made (no reference field) */
 } qew_rk_result;
 
 /* ---- errors, context, memory ------------------------------------------- */

@@ -99,6 +99,7 @@ struct orc_rk_params {
 struct orc_rk_result {
     int blocked, runaway, sweeps;
     double total_advance, mean_h;
+    long n_velocity_evals;  // orc_rk_site_velocity calls (not a reference field)
 };
 
 // ---- B-spline basis: src/filtered_noise.h:33-62 ----
@@ -765,19 +766,22 @@ double orc_rk_site_velocity(int i, int n, double dx, double lt, double f, int mo
 
 double orc_rk_forward_zero(int i, int n, double dx, double lt, double f, int model,
                            double hi, double hl, double hr, const orc_noise *nz,
-                           const orc_rk_params *rp, int *found) {
+                           const orc_rk_params *rp, int *found, long *evals) {
     const double v0 = orc_rk_site_velocity(i, n, dx, lt, f, model, hi, hl, hr, nz);
+    *evals = 1;
     if (v0 <= 0) { *found = 1; return hi; }
     double a = hi;
     double advanced = 0;
     while (advanced < rp->max_advance) {
         const double b = a + rp->dstep;
         const double vb = orc_rk_site_velocity(i, n, dx, lt, f, model, b, hl, hr, nz);
+        ++*evals;
         if (vb <= 0) {
             double lo = a, hicur = b;
             for (int k = 0; k < 60 && (hicur - lo) > rp->root_tol; ++k) {
                 const double mid = 0.5 * (lo + hicur);
                 const double vm = orc_rk_site_velocity(i, n, dx, lt, f, model, mid, hl, hr, nz);
+                ++*evals;
                 if (vm > 0) lo = mid; else hicur = mid;
             }
             *found = 1;
@@ -798,6 +802,7 @@ int orc_rk_block(int n, double *h, const orc_params *p, const orc_noise *nz,
     const double lt = p->line_tension;
     const double f = p->driving_force;
     std::vector<double> adv(n);
+    std::vector<long> evals(n);
     double mean_h0 = 0;
     for (int i = 0; i < n; ++i) mean_h0 += h[i];
     mean_h0 /= (double)n;
@@ -812,13 +817,14 @@ int orc_rk_block(int n, double *h, const orc_params *p, const orc_noise *nz,
                 const int im = (i == 0) ? n - 1 : i - 1;
                 const int ip = (i + 1 == n) ? 0 : i + 1;
                 int found = 0;
-                const double hnew = orc_rk_forward_zero(i, n, dx, lt, f, p->model, h[i], h[im], h[ip], nz, rp, &found);
+                const double hnew = orc_rk_forward_zero(i, n, dx, lt, f, p->model, h[i], h[im], h[ip], nz, rp, &found, &evals[i]);
                 adv[i] = hnew - h[i];
                 h[i] = hnew;
             }
         }
         double total_adv = 0;
         for (int i = 0; i < n; ++i) total_adv += adv[i];
+        for (int i = 0; i < n; ++i) res.n_velocity_evals += evals[i];
         cum_adv += total_adv;
         const double mean_adv = cum_adv / (double)n;
         res.sweeps = sweep + 1;

@@ -114,10 +114,12 @@ class RkResult:
     sweeps: int = 0
     total_advance: float = 0.0
     mean_h: float = 0.0
+    n_velocity_evals: int = 0  # site-velocity evaluations (no reference field)
 
     @classmethod
     def from_c(cls, r):
-        return cls(bool(r.blocked), bool(r.runaway), r.sweeps, r.total_advance, r.mean_h)
+        return cls(bool(r.blocked), bool(r.runaway), r.sweeps, r.total_advance, r.mean_h,
+                   r.n_velocity_evals)
 
 
 def _torch():

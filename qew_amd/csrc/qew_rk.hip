@@ -1,13 +1,24 @@
 // Rosso-Krauth no-passing construction (src/rosso_krauth.h:52-194) for
-// batches of lines. Each colour pass is one launch over every still-active
-// line (one thread per site of that colour: its neighbours are the other
-// colour, so the sites are independent); a per-line reduction then updates
-// the line's sweep status on the device (runaway / blocked / running), so
-// finished lines freeze while the others continue and the host reads back
-// only an "any line still running" flag every few sweeps. The line state,
-// the velocity sum, the v <= 0 tests and the bisection are FP64; the tables
-// may be stored as FP64 or FP32.
+// batches of lines.
+//
+// A sweep updates colour 0 then colour 1 in place; within a colour the sites
+// are independent (their neighbours are the other colour). Two execution
+// paths give bit-identical results:
+//  * per-colour launches over every still-active line (any length), with a
+//    per-line status kernel that applies the runaway / blocked / timeout
+//    decision on the device; the host reads back only an "any line still
+//    running" flag every few sweeps;
+//  * one workgroup per line with the line state in LDS, running every sweep
+//    of the call in one launch (lines up to kRkLdsMaxN sites; used when the
+//    batch has enough lines to fill the device).
+// Both skip sites whose own height and both neighbours are unchanged since
+// the site was last found blocked (its velocity would be the same v <= 0),
+// and resolve the moving sites with lane groups that march and bisect
+// cooperatively (forward_zero_group). The line state, the advance sums, the
+// v <= 0 tests and the bisection are FP64; the tables may be FP64 or FP32.
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "qew_internal.hpp"
@@ -27,13 +38,18 @@ struct RkState {
     double total_adv;
     double mean_h0;
     double mean_h;
+    // site-velocity evaluations the reference algorithm makes in this call
+    // (skipped sites included), added to by every wave
+    unsigned long long vel_evals;
 };
 
 struct RkArgs {
     const LineDesc *lines;
     const NoiseDesc *noises;
     double *h;
-    double *adv;
+    unsigned char *moved;  // per site: moved in its last colour pass
+    double *partial;       // [line][tile][colour] advance sums of the current sweep
+    int tiles;             // workgroups per colour per line
     RkState *state;
     qew_rk_params rp;
 };
@@ -63,91 +79,319 @@ __device__ __forceinline__ double site_velocity(int i, int n, double dx, double 
     return -grad;
 }
 
-// rk_forward_zero (src/rosso_krauth.h:88-124): march in dstep steps
-// (accumulating a = b), then at most 60 bisections.
+// dV/dy on one edge of one site (fixed x) at successive trial heights, with
+// the 4x4 coefficient block of the last y cell kept in registers: a march or
+// bisection mostly stays in one cell, so most evaluations gather nothing.
+// Same locate, weights and summation order as sample_dy64, so the values
+// are bit-identical.
 template <typename C>
-__device__ double forward_zero(int i, int n, double dx, double lt, double f, int model, double hi,
-                               double hl, double hr, const NoiseDesc &nd, const qew_rk_params &rp) {
-    const double v0 = site_velocity<C>(i, n, dx, lt, f, model, hi, hl, hr, nd);
-    if (v0 <= 0) return hi;
-    double a = hi;
-    double advanced = 0;
-    while (advanced < rp.max_advance) {
-        const double b = a + rp.dstep;
-        const double vb = site_velocity<C>(i, n, dx, lt, f, model, b, hl, hr, nd);
-        if (vb <= 0) {
-            double lo = a, hic = b;
-            for (int k = 0; k < 60 && (hic - lo) > rp.root_tol; ++k) {
-                const double mid = 0.5 * (lo + hic);
-                const double vm = site_velocity<C>(i, n, dx, lt, f, model, mid, hl, hr, nd);
-                if (vm > 0) lo = mid;
-                else hic = mid;
+struct EdgeDy {
+    int iy = -1;  // cell of the held block (-1: none)
+    C c[4][4];    // c[a][b]: x-row a, y-level b
+
+    __device__ __forceinline__ double eval(const NoiseDesc &nd, double x, double y) {
+        if (nd.kind != TABLE) return sample_dy64<C>(nd, x, y);
+        int ixn[4], iyn[4];
+        double fx, fy;
+        locate64(nd, x, y, ixn, iyn, fx, fy);
+        double wx[4], dwy[4];
+        bspline_w(fx, wx);
+        bspline_dw(fy, dwy);
+        if (iyn[1] != iy) {
+            iy = iyn[1];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                double r[4];
+                row4_64<C>(nd, ixn[1], iyn[b], r);
+#pragma unroll
+                for (int a = 0; a < 4; ++a) c[a][b] = (C)r[a];
             }
-            return 0.5 * (lo + hic);
         }
-        a = b;
-        advanced += rp.dstep;
+        double d = 0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            double row_dy = 0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) row_dy += (double)c[a][b] * dwy[b];
+            d += wx[a] * row_dy;
+        }
+        return (nd.dy == 1.0) ? d : d / nd.dy;
     }
-    return a;
+};
+
+// site_velocity with the two edges' blocks held across calls for one site.
+template <typename C>
+__device__ __forceinline__ double site_velocity_held(int i, int n, double dx, double lt, double f,
+                                                     int model, double hi, double hl, double hr,
+                                                     const NoiseDesc &nd, EdgeDy<C> &el,
+                                                     EdgeDy<C> &er) {
+    const int im = (i == 0) ? n - 1 : i - 1;
+    const int ip = (i + 1 == n) ? 0 : i + 1;
+    double line_force;
+    if (model == 0) {
+        line_force = -lt * (hl + hr - 2.0 * hi) / (dx * dx);
+    } else {
+        const double dhr = (hr - hi) / dx;
+        const double dhl = (hi - hl) / dx;
+        line_force = lt * (dhl / sqrt(1.0 + dhl * dhl) - dhr / sqrt(1.0 + dhr * dhr)) / dx;
+    }
+    const double dvl = el.eval(nd, (double)(im + i) * 0.5 * dx, (hl + hi) * 0.5);
+    const double dvr = er.eval(nd, (double)(i + ip) * 0.5 * dx, (hi + hr) * 0.5);
+    const double grad = dx * (line_force + 0.5 * (dvl + dvr) - f);
+    return -grad;
 }
 
+// The lanes [base, base + g) of a wave, as the low g bits of a ballot.
+__device__ __forceinline__ unsigned long long group_bits(unsigned long long m, int base, int g) {
+    return (m >> base) & (g == kWave ? ~0ull : ((1ull << g) - 1ull));
+}
+
+// rk_forward_zero (src/rosso_krauth.h:88-124) for a site whose velocity at hi
+// is already known to be positive, computed by a group of g lanes (g a power
+// of two, 1..64, lanes base..base+g-1 of the wave; `li` = lane in the group).
+// The march evaluates g consecutive steps at once: lane j repeats the
+// sequential additions a += dstep, advanced += dstep j times, so its trial
+// point is bit-identical to step j of the loop, and the first step with
+// v <= 0 is the sequential one. The bisection evaluates the next d =
+// log2(g) levels at once: lane li owns node li + 1 of the heap-ordered
+// subtree below the current bracket (children 2x: v <= 0 -> hi = mid,
+// 2x + 1: v > 0 -> lo = mid), recomputes its bracket by the same halvings
+// and checks the same loop condition, so the path walked afterwards is the
+// sequential path. Every lane of the group must call this with the same
+// site; all return the same height. `evals` (every lane) grows by the number
+// of evaluations the sequential loop makes, march and bisection.
+template <typename C>
+__device__ double forward_zero_group(int i, int n, double dx, double lt, double f, int model,
+                                     double hi, double hl, double hr, const NoiseDesc &nd,
+                                     const qew_rk_params &rp, int g, int li, int base,
+                                     unsigned &evals) {
+    // one evaluation site for both phases keeps a single inlined copy of the
+    // velocity (two gathers) in the kernel
+    double a = hi, advanced = 0;  // march state
+    double lo = 0, hic = 0;       // bisection bracket
+    int k = 0;                    // bisections done
+    bool marching = true;
+    const int d = (g == 1) ? 1 : 31 - __clz(g);
+    const int node = li + 1;
+    const int depth = 31 - __clz(node);
+    EdgeDy<C> el, er;
+    for (;;) {
+        double trial, aa = 0, ad = 0;
+        bool want;
+        if (marching) {
+            if (!(advanced < rp.max_advance)) return a;
+            aa = a;
+            ad = advanced;
+            for (int j = 0; j < li; ++j) {
+                aa = aa + rp.dstep;
+                ad += rp.dstep;
+            }
+            // dstep > 0 (checked on the host), so the steps reached form a prefix
+            want = ad < rp.max_advance;
+            trial = aa + rp.dstep;
+        } else {
+            if (!(k < 60 && (hic - lo) > rp.root_tol)) return 0.5 * (lo + hic);
+            want = node < (1 << d);
+            double l = lo, h = hic;
+            for (int t = depth - 1; t >= 0 && want; --t) {
+                if (!(k + (depth - 1 - t) < 60 && (h - l) > rp.root_tol)) {
+                    want = false;
+                    break;
+                }
+                const double mid = 0.5 * (l + h);
+                if ((node >> t) & 1) l = mid;
+                else h = mid;
+            }
+            want = want && (k + depth < 60) && ((h - l) > rp.root_tol);
+            trial = 0.5 * (l + h);
+        }
+        double v = 0;
+        if (want) v = site_velocity_held<C>(i, n, dx, lt, f, model, trial, hl, hr, nd, el, er);
+        if (marching) {
+            const unsigned long long hm = group_bits(__ballot(want && v <= 0), base, g);
+            const unsigned long long rm = group_bits(__ballot(want), base, g);
+            if (hm) {
+                const int j = __builtin_ctzll(hm);
+                evals += j + 1;
+                lo = __shfl(aa, base + j);
+                hic = __shfl(trial, base + j);
+                marching = false;
+                continue;
+            }
+            const int steps = __popcll(rm);
+            evals += steps;
+            const int last = base + (steps > 0 ? steps - 1 : 0);
+            const double b_last = __shfl(trial, last);
+            const double ad_last = __shfl(ad + rp.dstep, last);
+            if (steps == 0) return a;
+            a = b_last;
+            advanced = ad_last;
+            if (steps < g) return a;
+        } else {
+            const unsigned long long pm = group_bits(__ballot(want && v > 0), base, g);
+            int x = 1;
+            for (int t = 0; t < d; ++t) {
+                if (!(k < 60 && (hic - lo) > rp.root_tol)) break;
+                const double mid = 0.5 * (lo + hic);
+                const bool p = (pm >> (x - 1)) & 1ull;
+                if (p) lo = mid;
+                else hic = mid;
+                ++k;
+                ++evals;
+                x = 2 * x + (p ? 1 : 0);
+            }
+        }
+    }
+}
+
+// Lanes per moving site for m moving sites shared by `threads` lanes.
+__device__ __forceinline__ int group_size(int m, int threads) {
+    int g = kWave;
+    while (g > 1 && g * m > threads) g >>= 1;
+    return g;
+}
+
+// The runaway / blocked / timeout decision after a sweep
+// (src/rosso_krauth.h:168-188). Returns whether the line keeps sweeping.
+__device__ bool sweep_status(RkState &st, double total, int n, const qew_rk_params &rp) {
+    st.cum_adv += total;
+    const double mean_adv = st.cum_adv / (double)n;
+    st.sweeps += 1;
+    st.total_adv = total;
+    st.mean_h = st.mean_h0 + mean_adv;
+    if (mean_adv > rp.runaway) {
+        st.runaway = 1;
+        st.blocked = 0;
+        return false;
+    }
+    if (total < rp.sweep_tol) {
+        st.blocked = 1;
+        st.runaway = 0;
+        return false;
+    }
+    return st.sweeps < rp.max_sweeps;  // else timeout
+}
+
+// One colour pass over 256 sites of one line per workgroup. A workgroup
+// whose sites and neighbours all kept their heights since their last pass
+// only records a zero advance. Otherwise every changed site's velocity is
+// evaluated, and the sites that move are listed in LDS and resolved by lane
+// groups, so a workgroup with few moving sites finishes in a few dependent
+// evaluation rounds instead of the ~40 of a sequential march and bisection.
+// The advance sum is formed per site owner in a fixed order.
 template <typename C>
 __global__ __launch_bounds__(kRkBlock) void rk_color_kernel(RkArgs a, int color) {
+    __shared__ int s_list[kRkBlock];
+    __shared__ double s_new[kRkBlock];
+    __shared__ double red[kRkBlock / kWave];
+    __shared__ int s_count;
     const int line = blockIdx.y;
     if (!a.state[line].active) return;
     const LineDesc L = a.lines[line];
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = 2 * j + color;
-    if (i >= L.n) return;
-    const NoiseDesc nd = a.noises[L.noise];
-    double *h = a.h + L.off;
     const int n = L.n;
-    const int im = (i == 0) ? n - 1 : i - 1;
-    const int ip = (i + 1 == n) ? 0 : i + 1;
+    const int tid = threadIdx.x;
+    const int half = (n - color + 1) / 2;  // sites of this colour: i = 2 j + color < n
+    const int j = blockIdx.x * kRkBlock + tid;
+    const int i = 2 * j + color;
+    const bool valid = j < half;
+    const unsigned char *mv = a.moved + L.off;
+    bool dirty = false;
+    int im = 0, ip = 0;
+    if (valid) {
+        im = (i == 0) ? n - 1 : i - 1;
+        ip = (i + 1 == n) ? 0 : i + 1;
+        dirty = (mv[i] | mv[im] | mv[ip]) != 0;
+    }
+    double *part = a.partial + ((size_t)line * a.tiles + blockIdx.x) * 2 + color;
+    if (!__syncthreads_or(dirty)) {
+        if (tid == 0) {
+            *part = 0.0;
+            const int cnt = std::min(kRkBlock, std::max(0, half - (int)blockIdx.x * kRkBlock));
+            if (cnt) atomicAdd(&a.state[line].vel_evals, (unsigned long long)cnt);
+        }
+        return;
+    }
+    const NoiseDesc nd = a.noises[L.noise];
     const double dx = L.L / (double)n;
-    const double hi = h[i];
-    const double hnew = forward_zero<C>(i, n, dx, L.lt, L.f, L.model, hi, h[im], h[ip], nd, a.rp);
-    a.adv[L.off + i] = hnew - hi;
-    h[i] = hnew;
+    double *h = a.h + L.off;
+    if (tid == 0) s_count = 0;
+    __syncthreads();
+    unsigned evals = valid ? 1u : 0u;
+    double hi = 0;
+    if (dirty) {
+        hi = h[i];
+        const double v0 = site_velocity<C>(i, n, dx, L.lt, L.f, L.model, hi, h[im], h[ip], nd);
+        if (v0 > 0) s_list[atomicAdd(&s_count, 1)] = tid;
+        else s_new[tid] = hi;
+    }
+    __syncthreads();
+    const int m = s_count;
+    const int g = group_size(m, kRkBlock);
+    const int li = tid & (g - 1);
+    const int base = (tid & (kWave - 1)) & ~(g - 1);
+    for (int idx = tid / g; idx < m; idx += kRkBlock / g) {
+        const int t = s_list[idx];
+        const int s = 2 * (blockIdx.x * kRkBlock + t) + color;
+        const int sm = (s == 0) ? n - 1 : s - 1;
+        const int sp = (s + 1 == n) ? 0 : s + 1;
+        unsigned e = 0;
+        const double hnew = forward_zero_group<C>(s, n, dx, L.lt, L.f, L.model, h[s], h[sm], h[sp],
+                                                  nd, a.rp, g, li, base, e);
+        if (li == 0) {
+            evals += e;
+            s_new[t] = hnew;
+        }
+    }
+    __syncthreads();
+    double adv = 0;
+    if (dirty) {
+        const double hn = s_new[tid];
+        adv = hn - hi;
+        const bool moved = hn != hi;
+        if (moved) h[i] = hn;
+        a.moved[L.off + i] = moved ? 1 : 0;
+    }
+    const double total = block_sum(adv, red);
+    if (tid == 0) *part = total;
+    // every lane of the wave reaches the reduction (no early return above)
+    const unsigned ev = wave_sum(evals);
+    if ((tid & (kWave - 1)) == 0 && ev) atomicAdd(&a.state[line].vel_evals, (unsigned long long)ev);
 }
 
-// One workgroup per line: sum of the sweep's advances, then the
-// runaway/blocked decision (src/rosso_krauth.h:168-188).
+// One workgroup per line: the sweep's advance (per-workgroup sums of both
+// colours, added in a fixed order) and the status decision.
 __global__ __launch_bounds__(kRkBlock) void rk_status_kernel(RkArgs a, unsigned *running) {
-    __shared__ double red[16];
+    __shared__ double red[kRkBlock / kWave];
     const int line = blockIdx.x;
     RkState st = a.state[line];
     if (!st.active) return;
     const LineDesc L = a.lines[line];
+    const double *part = a.partial + (size_t)line * a.tiles * 2;
     double acc = 0;
-    for (int i = threadIdx.x; i < L.n; i += blockDim.x) acc += a.adv[L.off + i];
+    for (int t = threadIdx.x; t < 2 * a.tiles; t += blockDim.x) acc += part[t];
     const double total = block_sum(acc, red);
     if (threadIdx.x != 0) return;
-    st.cum_adv += total;
-    const double mean_adv = st.cum_adv / (double)L.n;
-    st.sweeps += 1;
-    st.total_adv = total;
-    st.mean_h = st.mean_h0 + mean_adv;
-    if (mean_adv > a.rp.runaway) {
-        st.runaway = 1;
-        st.blocked = 0;
-        st.active = 0;
-    } else if (total < a.rp.sweep_tol) {
-        st.blocked = 1;
-        st.runaway = 0;
-        st.active = 0;
-    }
-    if (st.active && st.sweeps >= a.rp.max_sweeps) st.active = 0;  // timeout
-    a.state[line] = st;
-    if (st.active) atomicAdd(running, 1u);
+    const bool go = sweep_status(st, total, L.n, a.rp);
+    // vel_evals is added to by the colour kernels: leave it out of the store
+    a.state[line].active = go ? 1 : 0;
+    a.state[line].sweeps = st.sweeps;
+    a.state[line].blocked = st.blocked;
+    a.state[line].runaway = st.runaway;
+    a.state[line].cum_adv = st.cum_adv;
+    a.state[line].total_adv = st.total_adv;
+    a.state[line].mean_h = st.mean_h;
+    if (go) atomicAdd(running, 1u);
 }
 
 __global__ __launch_bounds__(kRkBlock) void rk_init_kernel(RkArgs a) {
-    __shared__ double red[16];
+    __shared__ double red[kRkBlock / kWave];
     const int line = blockIdx.x;
     const LineDesc L = a.lines[line];
     double acc = 0;
-    for (int i = threadIdx.x; i < L.n; i += blockDim.x) acc += a.h[L.off + i];
+    for (int i = threadIdx.x; i < L.n; i += blockDim.x) {
+        acc += a.h[L.off + i];
+        a.moved[L.off + i] = 1;  // every site is evaluated in the first sweep
+    }
     const double s = block_sum(acc, red);
     if (threadIdx.x == 0) {
         RkState st{};
@@ -155,6 +399,122 @@ __global__ __launch_bounds__(kRkBlock) void rk_init_kernel(RkArgs a) {
         st.mean_h0 = s / (double)L.n;
         st.mean_h = 0;
         a.state[line] = st;
+    }
+}
+
+// Whole rk_block calls for lines held in LDS: one workgroup per line runs
+// every sweep (both colours, the advance sum and the status decision) with
+// the line state and the moved flags in LDS, so a call is one launch however
+// many sweeps it takes. Sites of a colour are taken in chunks of
+// kRkLineBlock; a chunk with no changed site costs one barrier.
+// 512 threads: two FP64 gathers in flight take ~170 VGPRs, more than a
+// 1024-thread workgroup can give each wave.
+constexpr int kRkLineBlock = 512;
+constexpr int kRkLdsMaxN = 16384;
+
+template <typename C>
+__global__ __launch_bounds__(kRkLineBlock) void rk_line_kernel(RkArgs a, int lds_n) {
+    extern __shared__ double s_h[];
+    unsigned char *s_mv = reinterpret_cast<unsigned char *>(s_h + lds_n);
+    __shared__ int s_list[kRkLineBlock];
+    __shared__ double s_new[kRkLineBlock];
+    __shared__ double red[kRkLineBlock / kWave];
+    __shared__ int s_count;
+    __shared__ int s_go;
+    const int line = blockIdx.x;
+    const LineDesc L = a.lines[line];
+    const NoiseDesc nd = a.noises[L.noise];
+    const int n = L.n;
+    const double dx = L.L / (double)n;
+    const int tid = threadIdx.x;
+    double *hg = a.h + L.off;
+    double acc = 0;
+    for (int i = tid; i < n; i += kRkLineBlock) {
+        const double v = hg[i];
+        s_h[i] = v;
+        s_mv[i] = 1;
+        acc += v;
+    }
+    RkState st{};
+    st.mean_h0 = block_sum(acc, red) / (double)n;
+    unsigned long long evals = 0;
+    bool go = a.rp.max_sweeps > 0;
+    while (go) {
+        double my_adv = 0;
+        for (int color = 0; color < 2; ++color) {
+            const int half = (n - color + 1) / 2;
+            for (int c0 = 0; c0 < half; c0 += kRkLineBlock) {
+                const int i = 2 * (c0 + tid) + color;
+                const bool valid = c0 + tid < half;
+                evals += valid ? 1u : 0u;
+                bool dirty = false;
+                int im = 0, ip = 0;
+                if (valid) {
+                    im = (i == 0) ? n - 1 : i - 1;
+                    ip = (i + 1 == n) ? 0 : i + 1;
+                    dirty = (s_mv[i] | s_mv[im] | s_mv[ip]) != 0;
+                }
+                // own flags are written by their owner only; the neighbours'
+                // belong to the other colour, fixed during this pass
+                if (!__syncthreads_or(dirty)) continue;
+                if (tid == 0) s_count = 0;
+                __syncthreads();
+                double hi = 0;
+                if (dirty) {
+                    hi = s_h[i];
+                    const double v0 =
+                        site_velocity<C>(i, n, dx, L.lt, L.f, L.model, hi, s_h[im], s_h[ip], nd);
+                    if (v0 > 0) s_list[atomicAdd(&s_count, 1)] = tid;
+                    else s_new[tid] = hi;
+                }
+                __syncthreads();
+                const int m = s_count;
+                const int g = group_size(m, kRkLineBlock);
+                const int li = tid & (g - 1);
+                const int base = (tid & (kWave - 1)) & ~(g - 1);
+                for (int idx = tid / g; idx < m; idx += kRkLineBlock / g) {
+                    const int t = s_list[idx];
+                    const int s = 2 * (c0 + t) + color;
+                    const int sm = (s == 0) ? n - 1 : s - 1;
+                    const int sp = (s + 1 == n) ? 0 : s + 1;
+                    unsigned e = 0;
+                    const double hnew = forward_zero_group<C>(s, n, dx, L.lt, L.f, L.model, s_h[s],
+                                                              s_h[sm], s_h[sp], nd, a.rp, g, li,
+                                                              base, e);
+                    if (li == 0) {
+                        evals += e;
+                        s_new[t] = hnew;
+                    }
+                }
+                __syncthreads();
+                if (dirty) {
+                    const double hn = s_new[tid];
+                    my_adv += hn - hi;
+                    s_h[i] = hn;
+                    s_mv[i] = hn != hi ? 1 : 0;
+                }
+            }
+            __syncthreads();
+        }
+        // each thread's sites are fixed, so the sum is reproducible
+        const double total = block_sum(my_adv, red);
+        if (tid == 0) s_go = sweep_status(st, total, n, a.rp) ? 1 : 0;
+        __syncthreads();
+        go = s_go != 0;
+    }
+    for (int i = tid; i < n; i += kRkLineBlock) hg[i] = s_h[i];
+    const unsigned long long ev = wave_sum(evals);
+    if ((tid & (kWave - 1)) == 0 && ev) atomicAdd(&a.state[line].vel_evals, ev);
+    if (tid == 0) {
+        // the counter is added to by every wave: leave it out of the store
+        a.state[line].active = 0;
+        a.state[line].sweeps = st.sweeps;
+        a.state[line].blocked = st.blocked;
+        a.state[line].runaway = st.runaway;
+        a.state[line].cum_adv = st.cum_adv;
+        a.state[line].total_adv = st.total_adv;
+        a.state[line].mean_h0 = st.mean_h0;
+        a.state[line].mean_h = st.mean_h;
     }
 }
 
@@ -169,45 +529,111 @@ __global__ void rk_velocity_kernel(NoiseDesc nd, int n, double dx, double lt, do
         v[k] = site_velocity<double>(idx[k], n, dx, lt, f, model, hi[k], hl[k], hr[k], nd);
 }
 
+// Device scratch of run_rk for nlines lines of total length `total`.
+struct RkScratch {
+    RkState *state;
+    unsigned *running;
+    unsigned char *moved;
+    double *partial;
+};
+
+size_t rk_scratch_bytes(int nlines, int64_t total, int tiles) {
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    return up(sizeof(RkState) * nlines) + 256 + up((size_t)total) +
+           up(sizeof(double) * 2 * (size_t)tiles * nlines);
+}
+
+RkScratch rk_scratch(char *w, int nlines, int64_t total) {
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    RkScratch s;
+    s.state = reinterpret_cast<RkState *>(w);
+    w += up(sizeof(RkState) * nlines);
+    s.running = reinterpret_cast<unsigned *>(w);
+    w += 256;
+    s.moved = reinterpret_cast<unsigned char *>(w);
+    w += up((size_t)total);
+    s.partial = reinterpret_cast<double *>(w);
+    return s;
+}
+
+int rk_tiles(int max_n) { return std::max(1, ((max_n + 1) / 2 + kRkBlock - 1) / kRkBlock); }
+
+// Whether the LDS path serves this batch. QEW_RK_KERNEL=line|launch
+// overrides the choice (tuning and tests).
+bool use_line_kernel(qew_ctx ctx, int nlines, int max_n) {
+    int lds_max = 0, lds_optin = 0;
+    QEW_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+    QEW_HIP(hipDeviceGetAttribute(&lds_optin, hipDeviceAttributeSharedMemPerBlockOptin, ctx->device));
+    lds_max = std::max(lds_max, lds_optin);
+    const size_t need = (sizeof(double) + 1) * (size_t)max_n + 8192;
+    const bool fits = max_n <= kRkLdsMaxN && need <= (size_t)lds_max;
+    const char *env = std::getenv("QEW_RK_KERNEL");
+    if (env && std::strcmp(env, "launch") == 0) return false;
+    if (env && std::strcmp(env, "line") == 0) return fits;
+    // one workgroup per line leaves CUs idle for small batches, where the
+    // per-colour launches spread each line over many workgroups
+    return fits && nlines >= ctx->num_cus;
+}
+
 // Runs the sweeps for `nlines` lines whose descriptors are on the device.
 void run_rk(qew_ctx ctx, const LineDesc *d_lines, const NoiseDesc *d_noises, int nlines, int max_n,
-            int table_dtype, const qew_rk_params &rp, double *h, double *adv, RkState *state,
-            unsigned *running, unsigned *running_host, std::vector<RkState> &out) {
+            int table_dtype, const qew_rk_params &rp, double *h, const RkScratch &w,
+            unsigned *running_host, std::vector<RkState> &out) {
     // a march step of zero (or NaN) with a positive advance cap never ends;
     // the reference would spin forever, the device must not
     QEW_REQUIRE(rp.dstep > 0 || !(rp.max_advance > 0),
                 "rk_block: dstep must be positive when max_advance > 0");
     hipStream_t s = ctx->stream;
-    RkArgs a{d_lines, d_noises, h, adv, state, rp};
-    hipLaunchKernelGGL(rk_init_kernel, dim3(nlines), dim3(kRkBlock), 0, s, a);
-    QEW_HIP(hipGetLastError());
-    const int per_color = (max_n + 1) / 2;
-    const dim3 grid((per_color + kRkBlock - 1) / kRkBlock, nlines);
-    // check the running flag every `poll` sweeps; lines that finished in
-    // between are frozen, so the extra launches are no-ops for them
-    int poll = 4;
-    for (int sweep = 0; sweep < rp.max_sweeps;) {
-        QEW_HIP(hipMemsetAsync(running, 0, sizeof(unsigned), s));
-        const int todo = std::min(poll, rp.max_sweeps - sweep);
-        for (int k = 0; k < todo; ++k) {
-            for (int color = 0; color < 2; ++color) {
-                if (table_dtype == 1)
-                    hipLaunchKernelGGL(rk_color_kernel<float>, grid, dim3(kRkBlock), 0, s, a, color);
-                else
-                    hipLaunchKernelGGL(rk_color_kernel<double>, grid, dim3(kRkBlock), 0, s, a, color);
-            }
-            if (k == todo - 1) QEW_HIP(hipMemsetAsync(running, 0, sizeof(unsigned), s));
-            hipLaunchKernelGGL(rk_status_kernel, dim3(nlines), dim3(kRkBlock), 0, s, a, running);
+    const int tiles = rk_tiles(max_n);
+    RkArgs a{d_lines, d_noises, h, w.moved, w.partial, tiles, w.state, rp};
+    if (use_line_kernel(ctx, nlines, max_n)) {
+        const size_t lds = (sizeof(double) + 1) * (size_t)max_n;
+        QEW_HIP(hipMemsetAsync(w.state, 0, sizeof(RkState) * (size_t)nlines, s));
+        if (table_dtype == 1) {
+            QEW_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(rk_line_kernel<float>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL(rk_line_kernel<float>, dim3(nlines), dim3(kRkLineBlock), lds, s, a,
+                               max_n);
+        } else {
+            QEW_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(rk_line_kernel<double>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL(rk_line_kernel<double>, dim3(nlines), dim3(kRkLineBlock), lds, s, a,
+                               max_n);
         }
         QEW_HIP(hipGetLastError());
-        sweep += todo;
-        QEW_HIP(hipMemcpyAsync(running_host, running, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-        QEW_HIP(hipStreamSynchronize(s));
-        if (*running_host == 0) break;
-        poll = std::min(poll * 2, 256);
+    } else {
+        hipLaunchKernelGGL(rk_init_kernel, dim3(nlines), dim3(kRkBlock), 0, s, a);
+        QEW_HIP(hipGetLastError());
+        const dim3 grid(tiles, nlines);
+        // check the running flag every `poll` sweeps; lines that finished in
+        // between are frozen, so the extra launches are no-ops for them
+        int poll = 4;
+        for (int sweep = 0; sweep < rp.max_sweeps;) {
+            const int todo = std::min(poll, rp.max_sweeps - sweep);
+            for (int k = 0; k < todo; ++k) {
+                for (int color = 0; color < 2; ++color) {
+                    if (table_dtype == 1)
+                        hipLaunchKernelGGL(rk_color_kernel<float>, grid, dim3(kRkBlock), 0, s, a,
+                                           color);
+                    else
+                        hipLaunchKernelGGL(rk_color_kernel<double>, grid, dim3(kRkBlock), 0, s, a,
+                                           color);
+                }
+                if (k == todo - 1) QEW_HIP(hipMemsetAsync(w.running, 0, sizeof(unsigned), s));
+                hipLaunchKernelGGL(rk_status_kernel, dim3(nlines), dim3(kRkBlock), 0, s, a,
+                                   w.running);
+            }
+            QEW_HIP(hipGetLastError());
+            sweep += todo;
+            QEW_HIP(hipMemcpyAsync(running_host, w.running, sizeof(unsigned), hipMemcpyDeviceToHost,
+                                   s));
+            QEW_HIP(hipStreamSynchronize(s));
+            if (*running_host == 0) break;
+            poll = std::min(poll * 2, 256);
+        }
     }
     out.resize(nlines);
-    QEW_HIP(hipMemcpyAsync(out.data(), state, sizeof(RkState) * nlines, hipMemcpyDeviceToHost, s));
+    QEW_HIP(hipMemcpyAsync(out.data(), w.state, sizeof(RkState) * nlines, hipMemcpyDeviceToHost, s));
     QEW_HIP(hipStreamSynchronize(s));
 }
 
@@ -218,6 +644,7 @@ qew_rk_result to_result(const RkState &st) {
     r.sweeps = st.sweeps;
     r.total_advance = st.total_adv;
     r.mean_h = st.sweeps > 0 ? st.mean_h : 0.0;
+    r.n_velocity_evals = (long)st.vel_evals;
     return r;
 }
 
@@ -238,9 +665,9 @@ qew_status qew_rk_block(qew_ctx ctx, qew_noise noise, const qew_params *p,
                         "rk_block: line length must be even for the red-black sweep");
         use_device(ctx);
         LineDesc l = single_line(n, p);
-        // workspace: [LineDesc][NoiseDesc][RkState][running][adv n]
-        const size_t off_noise = 256, off_state = 512, off_run = 768, off_adv = 1024;
-        ctx->work.ensure(off_adv + sizeof(double) * (size_t)n);
+        // workspace: [LineDesc][NoiseDesc][run_rk scratch]
+        const size_t off_noise = 256, off_rk = 512;
+        ctx->work.ensure(off_rk + rk_scratch_bytes(1, n, rk_tiles(n)));
         char *w = ctx->work.as<char>();
         ctx->pinned.ensure(sizeof(double) * 64);
         QEW_HIP(hipMemcpyAsync(w, &l, sizeof(LineDesc), hipMemcpyHostToDevice, ctx->stream));
@@ -249,9 +676,7 @@ qew_status qew_rk_block(qew_ctx ctx, qew_noise noise, const qew_params *p,
         std::vector<RkState> out;
         const int dt = noise->desc.kind == TABLE ? noise->desc.dtype : 0;
         run_rk(ctx, reinterpret_cast<LineDesc *>(w), reinterpret_cast<NoiseDesc *>(w + off_noise), 1, n,
-               dt, *rp, h, reinterpret_cast<double *>(w + off_adv),
-               reinterpret_cast<RkState *>(w + off_state), reinterpret_cast<unsigned *>(w + off_run),
-               ctx->pinned.as<unsigned>(), out);
+               dt, *rp, h, rk_scratch(w + off_rk, 1, n), ctx->pinned.as<unsigned>(), out);
         *result = to_result(out[0]);
     });
 }
@@ -269,17 +694,12 @@ qew_status qew_batch_rk_block(qew_batch b, const qew_rk_params *rp, double *h,
                         "all tables of a batch must share one storage type");
         qew_ctx ctx = b->ctx;
         use_device(ctx);
-        const size_t off_state = 0;
-        const size_t off_run = ((sizeof(RkState) * b->nlines + 255) / 256) * 256;
-        const size_t off_adv = off_run + 256;
-        b->d_scratch.ensure(off_adv + sizeof(double) * (size_t)b->total);
+        b->d_scratch.ensure(rk_scratch_bytes(b->nlines, b->total, rk_tiles(b->max_n)));
         ctx->pinned.ensure(sizeof(double) * 64);
-        char *w = b->d_scratch.as<char>();
         std::vector<RkState> out;
         const int dt = b->noises[0].kind == TABLE ? b->noises[0].dtype : 0;
         run_rk(ctx, b->d_lines.as<LineDesc>(), b->d_noises.as<NoiseDesc>(), b->nlines, b->max_n, dt,
-               *rp, h, reinterpret_cast<double *>(w + off_adv),
-               reinterpret_cast<RkState *>(w + off_state), reinterpret_cast<unsigned *>(w + off_run),
+               *rp, h, rk_scratch(b->d_scratch.as<char>(), b->nlines, b->total),
                ctx->pinned.as<unsigned>(), out);
         for (int l = 0; l < b->nlines; ++l) results[l] = to_result(out[l]);
     });

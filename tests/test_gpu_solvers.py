@@ -185,6 +185,7 @@ def test_rk_block_matches_oracle_bitwise(q):
         assert (r.blocked, r.runaway, r.sweeps) == (bool(r_ref.blocked), bool(r_ref.runaway), r_ref.sweeps)
         np.testing.assert_array_equal(host(h), h_ref)
         assert abs(r.mean_h - r_ref.mean_h) < 1e-12
+        assert r.n_velocity_evals == r_ref.n_velocity_evals > 0
 
 
 def test_rk_washboard_critical_force(q):
@@ -206,6 +207,33 @@ def test_rk_washboard_critical_force(q):
     assert r.runaway and not r.blocked and r.mean_h > 3.0
 
 
+@pytest.mark.parametrize("path", ["line", "launch"])
+@pytest.mark.parametrize("n,lt,f,max_sweeps", [
+    (16384, 0.005, 0.1, 3000),   # largest line the LDS path holds; runs to blocked
+    (16384, 0.02, 0.2, 3000),
+    (32768, 0.02, 0.2, 300),     # beyond LDS (per-colour launches), stops at max_sweeps
+])
+def test_rk_long_lines_match_oracle_bitwise(q, monkeypatch, path, n, lt, f, max_sweeps):
+    monkeypatch.setenv("QEW_RK_KERNEL", path)
+    # 16 x 16384 and 64 x 32768 sites per column of a 256 x 64 table: many
+    # moving sites per workgroup in the first sweeps, few near the end, so
+    # the lane-group sizes of the cooperative march/bisection all occur
+    field, coef = O.filtered_noise(256, 64, 256.0, 1.0, 0.5, 4.0, 0.1, 3)
+    on = O.OracleNoise.table(coef, 256.0, 1.0)
+    gn = q.FilteredNoise.from_coefficients(coef, 1.0, 1.0 / 64)
+    rpo = O.rk_params(dstep=0.5 / 64, max_advance=1.0, root_tol=1e-10, sweep_tol=1e-9,
+                      runaway=3.0, max_sweeps=max_sweeps)
+    rp = q.RkParams(0.5 / 64, 1.0, 1e-10, 1e-9, 3.0, max_sweeps)
+    h_ref = np.full(n, 0.5)
+    r_ref = O.rk_block(h_ref, O.params(256.0, lt, f, 0), on, rpo, omp=True)
+    h = dev(np.full(n, 0.5))
+    r = q.rk_block(h, q.Params(256.0, lt, f, q.Model.Linear), gn, rp)
+    assert (r.blocked, r.runaway, r.sweeps) == (bool(r_ref.blocked), bool(r_ref.runaway), r_ref.sweeps)
+    assert r.n_velocity_evals == r_ref.n_velocity_evals
+    np.testing.assert_array_equal(host(h), h_ref)
+    assert abs(r.mean_h - r_ref.mean_h) < 1e-12
+
+
 def test_rk_odd_length_rejected(q):
     with pytest.raises(ValueError):
         q.rk_block(dev(np.zeros(17)), q.Params(17.0, 1.0, 0.0, q.Model.Linear), q.SinusoidNoise(0.1, 6.28),
@@ -225,20 +253,27 @@ def test_rk_site_velocity_is_minus_gradient(q):
         assert np.max(np.abs(host(v) + g)) < 1e-12
 
 
-def test_batch_rk_matches_single_lines(q):
+@pytest.mark.parametrize("path", ["line", "launch"])
+def test_batch_rk_matches_single_lines(q, monkeypatch, path):
+    # the batch runs on the chosen path, the single lines on the default one
     field, coef = O.filtered_noise(64, 64, 3.2, 1.0, 0.5, 0.2, 0.2, 7)
     gn = q.FilteredNoise.from_coefficients(coef, 0.05, 1.0 / 64)
     rp = q.RkParams(0.5 / 64, 1.0, 1e-11, 1e-9, 3.0)
-    forces = [0.0, 0.05, 0.1, 5.0]
+    forces = [0.0, 0.05, 0.1, 5.0, 0.07]
+    lengths = [64, 128, 32, 64, 1026]
     params = [q.Params(3.2, 1.0, f, q.Model.Linear) for f in forces]
-    b = q.Batch([64] * 4, [gn] * 4, params)
-    hb = dev(np.full(256, 0.5))
+    b = q.Batch(lengths, [gn] * len(lengths), params)
+    hb = dev(np.full(sum(lengths), 0.5))
+    monkeypatch.setenv("QEW_RK_KERNEL", path)
     res = b.rk_block(hb, rp)
+    monkeypatch.delenv("QEW_RK_KERNEL")
+    off = np.concatenate([[0], np.cumsum(lengths)])
     for k, f in enumerate(forces):
-        h = dev(np.full(64, 0.5))
+        h = dev(np.full(lengths[k], 0.5))
         r = q.rk_block(h, params[k], gn, rp)
         assert (res[k].blocked, res[k].runaway, res[k].sweeps) == (r.blocked, r.runaway, r.sweeps)
-        np.testing.assert_array_equal(host(hb)[64 * k:64 * (k + 1)], host(h))
+        assert res[k].n_velocity_evals == r.n_velocity_evals
+        np.testing.assert_array_equal(host(hb)[off[k]:off[k + 1]], host(h))
 
 
 def test_rk_fp32_table_bitwise_vs_oracle_on_rounded_table(q):

@@ -299,6 +299,25 @@ def test_rk_blocked_config_forward_and_arrested(model, amp, lt, f, seed):
     assert np.max(-O.gradient(h, p, nz)) < 1e-6
 
 
+def test_rk_velocity_evaluation_count():
+    # One sweep of a flat line in zero potential, linear, dx = lt = 1, f = 0.25:
+    # a site with fixed neighbours at height c stops at c + f/2. Colour 0
+    # (neighbours at 0): v0 > 0, marches to 0.1 (v > 0) and 0.2 (v < 0), then
+    # bisects [0.1, 0.2] down to 0.1/2^7 < 1e-3: 1 + 2 + 7 = 10 evaluations.
+    # Colour 1 (neighbours at ~0.125) stops near 0.25: marches 0.1, 0.2, 0.3,
+    # bisects [0.2, 0.3]: 1 + 3 + 7 = 11. Total 8*10 + 8*11.
+    rp = O.rk_params(dstep=0.1, max_advance=10.0, root_tol=1e-3, sweep_tol=1e-12,
+                     runaway=100.0, max_sweeps=1)
+    h = np.zeros(16)
+    r = O.rk_block(h, O.params(16, 1.0, 0.25, O.LINEAR), O.OracleNoise(O.ZERO), rp)
+    assert r.sweeps == 1 and not r.blocked and not r.runaway
+    assert r.n_velocity_evals == 8 * 10 + 8 * 11
+    assert np.all(np.abs(h[0::2] - 0.125) < 1e-3) and np.all(np.abs(h[1::2] - 0.25) < 2e-3)
+    # a line at rest: one evaluation per site, blocked after one sweep
+    r = O.rk_block(np.zeros(16), O.params(16, 1.0, 0.0, O.LINEAR), O.OracleNoise(O.ZERO), rp)
+    assert r.blocked and r.n_velocity_evals == 16
+
+
 def test_rk_runaway_flag_and_odd_n():
     r = O.rk_block(np.zeros(16), O.params(16, 1.0, 2 * 0.2 * math.pi, O.LINEAR), washboard(),
                    _rk_washboard_params())
