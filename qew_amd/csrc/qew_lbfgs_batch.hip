@@ -7,10 +7,20 @@
 // Workgroups pull lines from a queue ordered longest first; a finished line
 // retires and its workgroup takes the next, which balances ragged batches.
 //
+// A workgroup is one CU's worth of latency hiding for a whole line, so the
+// element loops issue the loads of kU elements per thread before using them
+// (stream()), and the passes over the line are fused where the algorithm
+// allows: each two-loop step applies its axpy and forms the next step's dot
+// product in the same pass, the last one also negates d and forms g.d.
+// Per-element arithmetic and each thread's summation order are those of the
+// plain loops, so the results do not depend on the fusion.
+//
 // The preconditioner d = irfft(rfft(q) / eig) runs in LDS: the real line of
 // length n is packed as n/2 complex values, transformed by an in-place
 // Stockham FFT (radix 4, one radix-2 stage for odd log2), unpacked to the
-// half spectrum, divided by eig, repacked and transformed back.
+// half spectrum, divided by eig, repacked and transformed back. Twiddles and
+// the eig sines come from one table of exp(2 pi i j / M), M = 4 * the
+// largest transform length, filled once per launch.
 //
 // Precision modes: FP64 (the reference's arithmetic; sums are workgroup tree
 // sums) and FP32 (line offsets from a per-line anchor, FP32 tables, FP32
@@ -27,9 +37,18 @@ using namespace qew_dev;
 
 namespace {
 
-constexpr int kLB = 512;      // threads per workgroup
-constexpr int kMaxM = 32;     // history length limit
+constexpr int kLB = 512;        // threads per workgroup
+constexpr int kU = 8;           // elements per thread with loads in flight (vector passes)
+// line-search trial steps evaluated per pass (each holds a 16-coefficient
+// gather in flight); the reference's backtracking here takes ~3.8 per iteration
+constexpr int kLS = 4;
+constexpr int kMaxM = 32;       // history length limit
 constexpr int kMaxFftN = 8192;  // complex LDS transform length (line length 16384)
+
+template <typename T>
+struct C2 {
+    T x, y;
+};
 
 struct BatchArgs {
     const LineDesc *lines;
@@ -40,12 +59,9 @@ struct BatchArgs {
     double *h;
     void *work;
     qew_lbfgs_result *results;
+    const void *tw;  // C2<T>[tw_m]: exp(2 pi i j / tw_m)
+    int tw_m;
     qew_lbfgs_params lp;
-};
-
-template <typename T>
-struct C2 {
-    T x, y;
 };
 
 template <typename T>
@@ -53,13 +69,46 @@ __device__ __forceinline__ C2<T> cm(C2<T> a, C2<T> b) {
     return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
 
-template <typename T>
-__device__ __forceinline__ C2<T> cis(double num, double den) {
-    // exp(i pi * 2 num / den)
-    double s, c;
-    sincospi(2.0 * num / den, &s, &c);
-    return {(T)c, (T)s};
+// for (i = tid; i < n; i += kLB) use(i, ld(i)), with the loads of kU
+// consecutive iterations issued before any of their uses. Full chunks run
+// without per-element guards, so the uses (and any loads inside them, such
+// as table gathers) share one basic block and can overlap.
+template <int U = kU, typename Ld, typename Use>
+__device__ __forceinline__ void stream(int n, Ld ld, Use use) {
+    for (int b = threadIdx.x; b < n; b += U * kLB) {
+        decltype(ld(0)) v[U];
+        if (b + (U - 1) * kLB < n) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = ld(b + u * kLB);
+#pragma unroll
+            for (int u = 0; u < U; ++u) use(b + u * kLB, v[u]);
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = b + u * kLB;
+                if (i < n) v[u] = ld(i);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = b + u * kLB;
+                if (i < n) use(i, v[u]);
+            }
+        }
+    }
 }
+
+template <typename T>
+struct P2 {
+    T a, b;
+};
+template <typename T>
+struct P3 {
+    T a, b, c;
+};
+template <typename T>
+struct P4 {
+    T a, b, c, d;
+};
 
 // ---- workgroup reductions (uniform results) ----
 struct Red {
@@ -67,50 +116,117 @@ struct Red {
     KSum *k;
 };
 
-__device__ __forceinline__ double wg_sum(double v, Red r) { return block_sum(v, r.d); }
+// dot-product accumulator: FP64 sum, or a Kahan pair for FP32 vectors
+template <typename T>
+struct Acc;
+template <>
+struct Acc<double> {
+    double s = 0;
+    __device__ void add(double p) { s += p; }
+    __device__ double reduce(Red r) { return block_sum(s, r.d); }
+};
+template <>
+struct Acc<float> {
+    KSum s{0.f, 0.f};
+    __device__ void add(float p) { s = ksum_add(s, KSum{p, 0.f}); }
+    __device__ double reduce(Red r) {
+        const KSum k = block_ksum(s, r.k);
+        return (double)k.s + (double)k.c;
+    }
+};
+
+// three FP64 sums with one pair of barriers
+__device__ __forceinline__ void block_sum3(double &x, double &y, double &z, double *red) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    x = wave_sum(x);
+    y = wave_sum(y);
+    z = wave_sum(z);
+    __syncthreads();
+    if (lane == 0) {
+        red[wid] = x;
+        red[nw + wid] = y;
+        red[2 * nw + wid] = z;
+    }
+    __syncthreads();
+    double a = 0, b = 0, c = 0;
+    for (int w = 0; w < nw; ++w) {
+        a += red[w];
+        b += red[nw + w];
+        c += red[2 * nw + w];
+    }
+    x = a;
+    y = b;
+    z = c;
+}
+
+// K FP64 sums with one pair of barriers; each equals block_sum of its value
+template <int K>
+__device__ __forceinline__ void block_sumk(double *v, double *red) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+#pragma unroll
+    for (int j = 0; j < K; ++j) v[j] = wave_sum(v[j]);
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int j = 0; j < K; ++j) red[j * nw + wid] = v[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        double a = 0;
+        for (int w = 0; w < nw; ++w) a += red[j * nw + w];
+        v[j] = a;
+    }
+}
 
 template <typename T>
 __device__ double wg_dot(const T *a, const T *b, int n, Red r) {
-    if constexpr (sizeof(T) == 8) {
-        double acc = 0;
-        for (int i = threadIdx.x; i < n; i += blockDim.x) acc += a[i] * b[i];
-        return block_sum(acc, r.d);
-    } else {
-        KSum acc{0.f, 0.f};
-        for (int i = threadIdx.x; i < n; i += blockDim.x) acc = ksum_add(acc, KSum{a[i] * b[i], 0.f});
-        acc = block_ksum(acc, r.k);
-        return (double)acc.s + (double)acc.c;
-    }
+    Acc<T> acc;
+    stream(n, [&](int i) { return P2<T>{a[i], b[i]}; },
+           [&](int, P2<T> v) { acc.add(v.a * v.b); });
+    return acc.reduce(r);
 }
 
 template <typename T>
 __device__ double wg_maxabs(const T *a, int n, Red r) {
     double m = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const double v = fabs((double)a[i]);
-        m = v > m ? v : m;
-    }
+    stream(n, [&](int i) { return a[i]; }, [&](int, T v) {
+        const double f = fabs((double)v);
+        m = f > m ? f : m;
+    });
     return block_max(m, r.d);
 }
 
 // ---- objective and gradient of one line ----
+// table gathers in flight per thread: 16 coefficients each, held as FP64 for
+// FP64 state
+template <typename T>
+constexpr int kUG = sizeof(T) == 8 ? 2 : 4;
+
 template <typename T, typename C>
 __device__ double wg_objective(const LineGeom &G, const T *x, const T *d, double alpha, Red r) {
     const int n = G.n;
+    double acc = 0;
     if constexpr (sizeof(T) == 8) {
         // src/qew_model.h:53-99 (TrialAt: x0(i) + alpha * d(i))
-        double acc = 0;
-        for (int i = threadIdx.x; i < n; i += blockDim.x) {
-            const int ip = (i + 1 == n) ? 0 : i + 1;
-            const double hi = d ? x[i] + alpha * d[i] : x[i];
-            const double hip = d ? x[ip] + alpha * d[ip] : x[ip];
-            const double dh = (hip - hi) / G.dx;
-            const double le = (G.model == 0) ? G.lt * dh * dh * 0.5 : G.lt * (sqrt(1.0 + dh * dh) - 1.0);
-            const double xc = (double)(i + ip) * 0.5 * G.dx;
-            const double hc = (hip + hi) * 0.5;
-            acc += G.dx * (le + sample_v64<C>(G.nd, xc, hc) - G.f * hi);
-        }
-        return block_sum(acc, r.d);
+        stream<kUG<T>>(n, [&](int i) {
+                   const int ip = (i + 1 == n) ? 0 : i + 1;
+                   return P2<double>{d ? x[i] + alpha * d[i] : x[i],
+                                     d ? x[ip] + alpha * d[ip] : x[ip]};
+               },
+               [&](int i, P2<double> v) {
+                   const int ip = (i + 1 == n) ? 0 : i + 1;
+                   const double hi = v.a, hip = v.b;
+                   const double dh = (hip - hi) / G.dx;
+                   const double le =
+                       (G.model == 0) ? G.lt * dh * dh * 0.5 : G.lt * (sqrt(1.0 + dh * dh) - 1.0);
+                   const double xc = (double)(i + ip) * 0.5 * G.dx;
+                   const double hc = (hip + hi) * 0.5;
+                   acc += G.dx * (le + sample_v64<C>(G.nd, xc, hc) - G.f * hi);
+               });
     } else {
         // The sufficient-decrease test compares energies whose difference
         // shrinks like |g|^2 while the energy itself is O(n): FP32 terms
@@ -118,114 +234,215 @@ __device__ double wg_objective(const LineGeom &G, const T *x, const T *d, double
         // energy of the FP32 trial point (the same FP32 values the commit
         // writes) is evaluated in FP64 on the FP32 table.
         const float al = (float)alpha;
-        double acc = 0;
-        for (int i = threadIdx.x; i < n; i += blockDim.x) {
-            const int ip = (i + 1 == n) ? 0 : i + 1;
-            const double oi = d ? (double)__builtin_fmaf(al, d[i], x[i]) : (double)x[i];
-            const double oip = d ? (double)__builtin_fmaf(al, d[ip], x[ip]) : (double)x[ip];
-            const double hi = G.anchor + oi, hip = G.anchor + oip;
-            const double dh = (oip - oi) / G.dx;
-            const double s2 = dh * dh;
-            const double le = (G.model == 0) ? G.lt * s2 * 0.5 : G.lt * (s2 / (sqrt(1.0 + s2) + 1.0));
-            const double xc = (double)(i + ip) * 0.5 * G.dx;
-            acc += G.dx * (le + sample_v64<float>(G.nd, xc, (hip + hi) * 0.5) - G.f * hi);
-        }
-        return block_sum(acc, r.d);
+        stream<kUG<T>>(n, [&](int i) {
+                   const int ip = (i + 1 == n) ? 0 : i + 1;
+                   return P2<double>{d ? (double)__builtin_fmaf(al, d[i], x[i]) : (double)x[i],
+                                     d ? (double)__builtin_fmaf(al, d[ip], x[ip]) : (double)x[ip]};
+               },
+               [&](int i, P2<double> v) {
+                   const int ip = (i + 1 == n) ? 0 : i + 1;
+                   const double oi = v.a, oip = v.b;
+                   const double hi = G.anchor + oi, hip = G.anchor + oip;
+                   const double dh = (oip - oi) / G.dx;
+                   const double s2 = dh * dh;
+                   const double le =
+                       (G.model == 0) ? G.lt * s2 * 0.5 : G.lt * (s2 / (sqrt(1.0 + s2) + 1.0));
+                   const double xc = (double)(i + ip) * 0.5 * G.dx;
+                   acc += G.dx * (le + sample_v64<float>(G.nd, xc, (hip + hi) * 0.5) - G.f * hi);
+               });
     }
+    return block_sum(acc, r.d);
+}
+
+// Energies at the kLS trial points x + alpha[j] d in one pass over the line.
+// Each site term, each thread's summation order and the workgroup sum are
+// those of wg_objective, so phi[j] equals wg_objective(G, x, d, alpha[j]).
+template <typename T, typename C>
+__device__ __forceinline__ void wg_objective_trials(const LineGeom &G, const T *x, const T *d,
+                                    const double *alpha, double *phi, Red r) {
+    const int n = G.n;
+    constexpr int K = kLS;
+    double acc[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) acc[j] = 0;
+    stream<1>(
+        n, [&](int i) {
+            const int ip = (i + 1 == n) ? 0 : i + 1;
+            return P4<T>{x[i], d[i], x[ip], d[ip]};
+        },
+        [&](int i, P4<T> v) {
+            const int ip = (i + 1 == n) ? 0 : i + 1;
+            const double xc = (double)(i + ip) * 0.5 * G.dx;
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                if constexpr (sizeof(T) == 8) {
+                    // src/qew_model.h:53-99 (TrialAt: x0(i) + alpha * d(i))
+                    const double hi = v.a + alpha[j] * v.b, hip = v.c + alpha[j] * v.d;
+                    const double dh = (hip - hi) / G.dx;
+                    const double le = (G.model == 0) ? G.lt * dh * dh * 0.5
+                                                     : G.lt * (sqrt(1.0 + dh * dh) - 1.0);
+                    const double hc = (hip + hi) * 0.5;
+                    acc[j] += G.dx * (le + sample_v64<C>(G.nd, xc, hc) - G.f * hi);
+                } else {
+                    const float al = (float)alpha[j];
+                    const double oi = (double)__builtin_fmaf(al, v.b, v.a);
+                    const double oip = (double)__builtin_fmaf(al, v.d, v.c);
+                    const double hi = G.anchor + oi, hip = G.anchor + oip;
+                    const double dh = (oip - oi) / G.dx;
+                    const double s2 = dh * dh;
+                    const double le = (G.model == 0) ? G.lt * s2 * 0.5
+                                                     : G.lt * (s2 / (sqrt(1.0 + s2) + 1.0));
+                    acc[j] += G.dx * (le + sample_v64<float>(G.nd, xc, (hip + hi) * 0.5) - G.f * hi);
+                }
+            }
+        });
+    block_sumk<K>(acc, r.d);
+#pragma unroll
+    for (int j = 0; j < K; ++j) phi[j] = acc[j];
 }
 
 template <typename T, typename C>
 __device__ void wg_gradient(const LineGeom &G, const T *x, T *g, T *edy, T *esl) {
     const int n = G.n;
     const bool arc = G.model == 1;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const int ip = (i + 1 == n) ? 0 : i + 1;
-        if constexpr (sizeof(T) == 8) {
-            const double dh = (x[ip] - x[i]) / G.dx;
-            edy[i] = sample_dy64<C>(G.nd, (double)(i + ip) * 0.5 * G.dx, (x[ip] + x[i]) * 0.5);
-            if (arc) esl[i] = dh / sqrt(1.0 + dh * dh);
-        } else {
-            float v, dy;
-            edge32<false, true>(G, i, ip, x[i], x[ip], v, dy);
-            edy[i] = dy;
-            const float dh = (x[ip] - x[i]) * G.inv_dx;
-            if (arc) esl[i] = dh * rsqrtf(1.0f + dh * dh);
-        }
-    }
+    stream<kUG<T>>(n, [&](int i) {
+               const int ip = (i + 1 == n) ? 0 : i + 1;
+               return P2<T>{x[i], x[ip]};
+           },
+           [&](int i, P2<T> v) {
+               const int ip = (i + 1 == n) ? 0 : i + 1;
+               if constexpr (sizeof(T) == 8) {
+                   const double dh = (v.b - v.a) / G.dx;
+                   edy[i] = sample_dy64<C>(G.nd, (double)(i + ip) * 0.5 * G.dx, (v.b + v.a) * 0.5);
+                   if (arc) esl[i] = dh / sqrt(1.0 + dh * dh);
+               } else {
+                   float vv, dy;
+                   edge32<false, true>(G, i, ip, v.a, v.b, vv, dy);
+                   edy[i] = dy;
+                   const float dh = (v.b - v.a) * G.inv_dx;
+                   if (arc) esl[i] = dh * rsqrtf(1.0f + dh * dh);
+               }
+           });
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const int ip = (i + 1 == n) ? 0 : i + 1;
-        const int im = (i == 0) ? n - 1 : i - 1;
-        if constexpr (sizeof(T) == 8) {
-            double lf;
-            if (!arc) lf = -G.lt * (x[im] + x[ip] - 2.0 * x[i]) / (G.dx * G.dx);
-            else lf = G.lt * (esl[im] - esl[i]) / G.dx;
-            g[i] = G.dx * (lf + 0.5 * (edy[im] + edy[i]) - G.f);
-        } else {
-            float lf;
-            const float lt = (float)G.lt;
-            if (!arc) lf = -lt * ((x[im] + x[ip]) - 2.0f * x[i]) * (G.inv_dx * G.inv_dx);
-            else lf = lt * (esl[im] - esl[i]) * G.inv_dx;
-            g[i] = G.dxf * (lf + 0.5f * (edy[im] + edy[i]) - (float)G.f);
-        }
-    }
+    struct Nb {
+        T xm, xi, xp, em, ei, sm, si;
+    };
+    stream<4>(n, [&](int i) {
+               const int ip = (i + 1 == n) ? 0 : i + 1;
+               const int im = (i == 0) ? n - 1 : i - 1;
+               Nb b;
+               b.em = edy[im];
+               b.ei = edy[i];
+               if (!arc) {
+                   b.xm = x[im];
+                   b.xi = x[i];
+                   b.xp = x[ip];
+                   b.sm = b.si = 0;
+               } else {
+                   b.sm = esl[im];
+                   b.si = esl[i];
+                   b.xm = b.xi = b.xp = 0;
+               }
+               return b;
+           },
+           [&](int i, Nb b) {
+               if constexpr (sizeof(T) == 8) {
+                   double lf;
+                   if (!arc) lf = -G.lt * (b.xm + b.xp - 2.0 * b.xi) / (G.dx * G.dx);
+                   else lf = G.lt * (b.sm - b.si) / G.dx;
+                   g[i] = G.dx * (lf + 0.5 * (b.em + b.ei) - G.f);
+               } else {
+                   float lf;
+                   const float lt = (float)G.lt;
+                   if (!arc) lf = -lt * ((b.xm + b.xp) - 2.0f * b.xi) * (G.inv_dx * G.inv_dx);
+                   else lf = lt * (b.sm - b.si) * G.inv_dx;
+                   g[i] = G.dxf * (lf + 0.5f * (b.em + b.ei) - (float)G.f);
+               }
+           });
     __syncthreads();
 }
 
 // ---- in-place LDS Stockham FFT of N = 2^k complex values ----
-template <typename T>
-__device__ void lds_fft(C2<T> *buf, int N, int sign) {
-    int Ns = 1;
-    while (Ns < N) {
-        const int R = ((N / Ns) % 4 == 0) ? 4 : 2;
-        const int m = N / R;
-        // N <= 8192 and 512 threads: at most 4 radix-4 or 8 radix-2
-        // butterflies per thread, 16 values either way
-        C2<T> v[16];
-        int c = 0;
-        for (int j = threadIdx.x; j < m; j += blockDim.x, ++c)
-            for (int q = 0; q < R; ++q) v[c * R + q] = buf[j + q * m];
-        __syncthreads();
-        c = 0;
-        for (int j = threadIdx.x; j < m; j += blockDim.x, ++c) {
-            const int k = j % Ns;
-            C2<T> u[4];
-            for (int q = 0; q < R; ++q) {
-                u[q] = v[c * R + q];
-                if (q > 0 && k > 0) u[q] = cm(u[q], cis<T>((double)sign * q * k, (double)Ns * R));
-            }
-            const int idxD = (j / Ns) * Ns * R + k;
-            if (R == 2) {
-                buf[idxD] = {u[0].x + u[1].x, u[0].y + u[1].y};
-                buf[idxD + Ns] = {u[0].x - u[1].x, u[0].y - u[1].y};
-            } else {
-                const C2<T> a = {u[0].x + u[2].x, u[0].y + u[2].y};
-                const C2<T> b = {u[0].x - u[2].x, u[0].y - u[2].y};
-                const C2<T> cc = {u[1].x + u[3].x, u[1].y + u[3].y};
-                const C2<T> e = {u[1].x - u[3].x, u[1].y - u[3].y};
-                const C2<T> je = (sign < 0) ? C2<T>{e.y, -e.x} : C2<T>{-e.y, e.x};
-                buf[idxD] = {a.x + cc.x, a.y + cc.y};
-                buf[idxD + Ns] = {b.x + je.x, b.y + je.y};
-                buf[idxD + 2 * Ns] = {a.x - cc.x, a.y - cc.y};
-                buf[idxD + 3 * Ns] = {b.x - je.x, b.y - je.y};
+// tw[j] = exp(2 pi i j / M); the stage twiddle exp(sign 2 pi i q k / (Ns R))
+// is tw[q k M / (Ns R)], conjugated for sign < 0.
+// One radix-R stage. N <= kMaxFftN and kLB threads: at most kMaxFftN / (R kLB)
+// butterflies per thread; the loops have compile-time bounds so the values
+// stay in registers.
+template <typename T, int R>
+__device__ __forceinline__ void fft_stage(C2<T> *buf, int N, int Ns, int sign, const C2<T> *tw,
+                                          int M) {
+    constexpr int CB = kMaxFftN / (R * kLB);
+    const int m = N / R;
+    const int step = M / (Ns * R);
+    C2<T> v[CB][R];
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+        const int j = threadIdx.x + c * kLB;
+        if (j < m)
+#pragma unroll
+            for (int q = 0; q < R; ++q) v[c][q] = buf[j + q * m];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+        const int j = threadIdx.x + c * kLB;
+        if (j >= m) continue;
+        const int k = j % Ns;
+        C2<T> u[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            u[q] = v[c][q];
+            if (q > 0 && k > 0) {
+                C2<T> w = tw[q * k * step];
+                if (sign < 0) w.y = -w.y;
+                u[q] = cm(u[q], w);
             }
         }
-        __syncthreads();
-        Ns *= R;
+        const int idxD = (j / Ns) * Ns * R + k;
+        if constexpr (R == 2) {
+            buf[idxD] = {u[0].x + u[1].x, u[0].y + u[1].y};
+            buf[idxD + Ns] = {u[0].x - u[1].x, u[0].y - u[1].y};
+        } else {
+            const C2<T> a = {u[0].x + u[2].x, u[0].y + u[2].y};
+            const C2<T> b = {u[0].x - u[2].x, u[0].y - u[2].y};
+            const C2<T> cc = {u[1].x + u[3].x, u[1].y + u[3].y};
+            const C2<T> e = {u[1].x - u[3].x, u[1].y - u[3].y};
+            const C2<T> je = (sign < 0) ? C2<T>{e.y, -e.x} : C2<T>{-e.y, e.x};
+            buf[idxD] = {a.x + cc.x, a.y + cc.y};
+            buf[idxD + Ns] = {b.x + je.x, b.y + je.y};
+            buf[idxD + 2 * Ns] = {a.x - cc.x, a.y - cc.y};
+            buf[idxD + 3 * Ns] = {b.x - je.x, b.y - je.y};
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ void lds_fft(C2<T> *buf, int N, int sign, const C2<T> *tw, int M) {
+    int Ns = 1;
+    while (Ns < N) {
+        if ((N / Ns) % 4 == 0) {
+            fft_stage<T, 4>(buf, N, Ns, sign, tw, M);
+            Ns *= 4;
+        } else {
+            fft_stage<T, 2>(buf, N, Ns, sign, tw, M);
+            Ns *= 2;
+        }
     }
 }
 
 // d = irfft(rfft(q) / eig), eig[k] = (kappa/dx) 4 sin^2(pi k / n) + shift
-// (src/lbfgs.h:116-146, 173-179); n = 2N.
+// (src/lbfgs.h:116-146, 173-179); n = 2N. `buf` holds the packed q on entry.
 template <typename T>
-__device__ void wg_precondition(const T *q, T *d, int n, double kdx, double shift, C2<T> *buf) {
+__device__ void wg_precondition(T *d, int n, double kdx, double shift, C2<T> *buf,
+                                const C2<T> *tw, int M) {
     const int N = n / 2;
-    for (int j = threadIdx.x; j < N; j += blockDim.x) buf[j] = {q[2 * j], q[2 * j + 1]};
-    __syncthreads();
-    lds_fft(buf, N, -1);
-    const double pi = 3.14159265358979323846;
+    lds_fft(buf, N, -1, tw, M);
+    const int e_w = M / n;        // W_n^k = conj(tw[k e_w])
+    const int e_s = M / (2 * n);  // sin(pi k / n) = tw[k e_s].y
     auto eig = [&](int k) {
-        const double s = sin(pi * (double)k / (double)n);
+        // sin(pi k / n) for k in [0, N]; k e_s <= M / 4
+        const double s = (double)tw[k * e_s].y;
         return kdx * 4 * s * s + shift;
     };
     for (int k = threadIdx.x; k <= N / 2; k += blockDim.x) {
@@ -241,7 +458,8 @@ __device__ void wg_precondition(const T *q, T *d, int n, double kdx, double shif
         // E_k = (Z_k + conj Z_{N-k}) / 2, O_k = (Z_k - conj Z_{N-k}) / (2i)
         const C2<T> E = {(Zk.x + Zn.x) * (T)0.5, (Zk.y - Zn.y) * (T)0.5};
         const C2<T> O = {(Zk.y + Zn.y) * (T)0.5, -(Zk.x - Zn.x) * (T)0.5};
-        const C2<T> w = cis<T>(-(double)k, (double)n);  // W_n^k
+        const C2<T> t = tw[k * e_w];
+        const C2<T> w = {t.x, -t.y};  // W_n^k
         const C2<T> wO = cm(w, O);
         const T ek = (T)eig(k), enk = (T)eig(N - k);
         const C2<T> Yk = {(E.x + wO.x) / ek, (E.y + wO.y) / ek};
@@ -259,11 +477,12 @@ __device__ void wg_precondition(const T *q, T *d, int n, double kdx, double shif
         }
     }
     __syncthreads();
-    lds_fft(buf, N, +1);
+    lds_fft(buf, N, +1, tw, M);
     const T inv = (T)(1.0 / (double)N);
     for (int j = threadIdx.x; j < N; j += blockDim.x) {
-        d[2 * j] = buf[j].x * inv;
-        d[2 * j + 1] = buf[j].y * inv;
+        const C2<T> z = buf[j];
+        d[2 * j] = z.x * inv;
+        d[2 * j + 1] = z.y * inv;
     }
     __syncthreads();
 }
@@ -277,6 +496,7 @@ __device__ void solve_line(const BatchArgs &a, int line, Red r, double *rho, dou
     const int n = L.n;
     const int m = lp.m;
     const int tid = threadIdx.x;
+    const C2<T> *tw = static_cast<const C2<T> *>(a.tw);
     double *h = a.h + L.off;
     const double anchor = (sizeof(T) == 8) ? 0.0 : h[0];
     const LineGeom G = make_geom(L, nd, anchor);
@@ -310,7 +530,22 @@ __device__ void solve_line(const BatchArgs &a, int line, Red r, double *rho, dou
     double phi = 0;
     bool finished = false;
     for (int it = 0; it < lp.max_iter; ++it) {
-        const double gmax = wg_maxabs(g, n, r);
+        const int hs = nslots;
+        // one pass: max|g|, q = g and the first two-loop dot S_newest . q
+        double gmax = 0, al_dot = 0;
+        {
+            const T *S0 = hs > 0 ? Sb(sbuf[slot[hs - 1]]) : nullptr;
+            Acc<T> acc;
+            stream(n, [&](int i) { return P2<T>{g[i], S0 ? S0[i] : (T)0}; },
+                   [&](int i, P2<T> v) {
+                       const double f = fabs((double)v.a);
+                       gmax = f > gmax ? f : gmax;
+                       q[i] = v.a;
+                       if (S0) acc.add(v.b * v.a);
+                   });
+            gmax = block_max(gmax, r.d);
+            if (S0) al_dot = acc.reduce(r);
+        }
         if (gmax < lp.ftol) {
             res.iterations = it;
             res.max_force = gmax;
@@ -318,42 +553,88 @@ __device__ void solve_line(const BatchArgs &a, int line, Red r, double *rho, dou
             finished = true;
             break;
         }
-        // two-loop recursion; q starts as g
-        for (int i = tid; i < n; i += blockDim.x) q[i] = g[i];
-        __syncthreads();
-        const int hs = nslots;
+        // two-loop recursion, first loop: q -= al Y for the newest pair first;
+        // each pass also forms the next (older) pair's S . q. The last pass
+        // writes q packed into the FFT buffer when the preconditioner follows.
+        const bool pre = lp.precondition != 0;
         for (int idx = hs - 1; idx >= 0; --idx) {
             const int row = slot[idx];
-            const T *S = Sb(sbuf[row]), *Y = Yb(sbuf[row]);
-            const double al = rho[row] * wg_dot(S, q, n, r);
+            const double al = rho[row] * al_dot;
             if (tid == 0) la[idx] = al;
             const T c = (T)(-al);
-            for (int i = tid; i < n; i += blockDim.x) q[i] += c * Y[i];
-            __syncthreads();
+            const T *Y = Yb(sbuf[row]);
+            const T *Sn = idx > 0 ? Sb(sbuf[slot[idx - 1]]) : nullptr;
+            Acc<T> acc;
+            stream(n, [&](int i) { return P3<T>{q[i], Y[i], Sn ? Sn[i] : (T)0}; },
+                   [&](int i, P3<T> v) {
+                       const T qn = v.a + c * v.b;
+                       if (idx == 0 && pre) {
+                           T *fb = reinterpret_cast<T *>(fftbuf);
+                           fb[i] = qn;
+                       } else {
+                           q[i] = qn;
+                       }
+                       if (Sn) acc.add(v.c * qn);
+                   });
+            if (Sn) al_dot = acc.reduce(r);
+            else __syncthreads();
         }
-        if (lp.precondition) {
-            wg_precondition<T>(q, d, n, kdx, shift, fftbuf);
+        if (pre) {
+            if (hs == 0) {
+                T *fb = reinterpret_cast<T *>(fftbuf);
+                for (int i = tid; i < n; i += blockDim.x) fb[i] = q[i];
+                __syncthreads();
+            }
+            wg_precondition<T>(d, n, kdx, shift, fftbuf, tw, a.tw_m);
         } else {
             const T g0 = (T)(nslots == 0 ? 1.0 : gamma);
-            for (int i = tid; i < n; i += blockDim.x) d[i] = g0 * q[i];
+            stream(n, [&](int i) { return q[i]; }, [&](int i, T v) { d[i] = g0 * v; });
             __syncthreads();
         }
-        for (int idx = 0; idx < hs; ++idx) {
-            const int row = slot[idx];
-            const T *S = Sb(sbuf[row]), *Y = Yb(sbuf[row]);
-            const double b = rho[row] * wg_dot(Y, d, n, r);
-            const T c = (T)(la[idx] - b);
-            for (int i = tid; i < n; i += blockDim.x) d[i] += c * S[i];
-            __syncthreads();
+        // second loop, oldest pair first: d += (la - b) S with b = rho Y . d;
+        // the last pass also negates d and forms g . d
+        double dphi0 = 0;
+        {
+            double b_dot = 0;
+            if (hs > 0) b_dot = wg_dot(Yb(sbuf[slot[0]]), d, n, r);
+            for (int idx = 0; idx < hs; ++idx) {
+                const int row = slot[idx];
+                const double bb = rho[row] * b_dot;
+                const T c = (T)(la[idx] - bb);
+                const T *S = Sb(sbuf[row]);
+                const bool last = idx == hs - 1;
+                const T *Yn = last ? g : Yb(sbuf[slot[idx + 1]]);
+                Acc<T> acc;
+                stream(n, [&](int i) { return P3<T>{d[i], S[i], Yn[i]}; },
+                       [&](int i, P3<T> v) {
+                           T dn = v.a + c * v.b;
+                           if (last) dn = -dn;
+                           d[i] = dn;
+                           acc.add(v.c * dn);
+                       });
+                const double s = acc.reduce(r);
+                if (last) dphi0 = s;
+                else b_dot = s;
+            }
+            if (hs == 0) {
+                Acc<T> acc;
+                stream(n, [&](int i) { return P2<T>{d[i], g[i]}; },
+                       [&](int i, P2<T> v) {
+                           const T dn = -v.a;
+                           d[i] = dn;
+                           acc.add(v.b * dn);
+                       });
+                dphi0 = acc.reduce(r);
+            }
         }
-        for (int i = tid; i < n; i += blockDim.x) d[i] = -d[i];
-        __syncthreads();
-        double dphi0 = wg_dot(g, d, n, r);
         if (dphi0 >= 0) {
             nslots = 0;
-            for (int i = tid; i < n; i += blockDim.x) d[i] = -g[i];
-            __syncthreads();
-            dphi0 = -wg_dot(g, g, n, r);
+            Acc<T> acc;
+            stream(n, [&](int i) { return g[i]; }, [&](int i, T v) {
+                d[i] = -v;
+                acc.add(v * v);
+            });
+            dphi0 = -acc.reduce(r);
         }
         if (it == 0) {
             phi = wg_objective<T, C>(G, x, nullptr, 0.0, r);
@@ -362,16 +643,30 @@ __device__ void solve_line(const BatchArgs &a, int line, Red r, double *rho, dou
         const double phi0 = phi;
         double alpha = 1.0;
         if (!lp.precondition && nslots == 0) alpha = 1.0 / fmax(1.0, sqrt(wg_dot(g, g, n, r)));
+        // Armijo backtracking alpha, alpha s, alpha s^2, ... (the sequence and
+        // the first accepted step of the reference loop); the energies of K
+        // consecutive trials come from one pass, the evaluation count is the
+        // sequential one
+        constexpr int K = kLS;
         bool ok = false;
-        for (int ls = 0; ls < lp.max_ls; ++ls) {
-            const double pt = wg_objective<T, C>(G, x, d, alpha, r);
-            ++res.n_func_evals;
-            if (pt <= phi0 + lp.c1 * alpha * dphi0) {
-                phi = pt;
-                ok = true;
-                break;
+        for (int ls = 0; ls < lp.max_ls && !ok;) {
+            double al[K], ph[K];
+            al[0] = alpha;
+#pragma unroll
+            for (int j = 1; j < K; ++j) al[j] = al[j - 1] * lp.ls_shrink;
+            wg_objective_trials<T, C>(G, x, d, al, ph, r);
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                if (ok || ls >= lp.max_ls) continue;
+                ++ls;
+                ++res.n_func_evals;
+                if (ph[j] <= phi0 + lp.c1 * al[j] * dphi0) {
+                    phi = ph[j];
+                    alpha = al[j];
+                    ok = true;
+                }
             }
-            alpha *= lp.ls_shrink;
+            if (!ok) alpha = al[K - 1] * lp.ls_shrink;
         }
         if (!ok) {
             if (nslots == 0 || ++ls_fail_streak > 2) {
@@ -394,31 +689,22 @@ __device__ void solve_line(const BatchArgs &a, int line, Red r, double *rho, dou
         // near convergence alpha d falls below the offsets' resolution and
         // the intended step no longer matches the realised one.
         T *st = Sb(sbuf[m]), *yt = Yb(sbuf[m]);
-        for (int i = tid; i < n; i += blockDim.x) {
-            if constexpr (sizeof(T) == 8) {
-                x[i] += alT * d[i];
-            } else {
-                const T xo = x[i];
-                const T xn = xo + alT * d[i];
-                x[i] = xn;
-                st[i] = xn - xo;
-            }
-        }
+        stream(n, [&](int i) { return P2<T>{x[i], d[i]}; }, [&](int i, P2<T> v) {
+            const T xn = v.a + alT * v.b;
+            x[i] = xn;
+            st[i] = (sizeof(T) == 8) ? alT * v.b : xn - v.a;
+        });
         __syncthreads();
         wg_gradient<T, C>(G, x, gn, edy, esl);
         double sy = 0, ss = 0, yy = 0;
-        for (int i = tid; i < n; i += blockDim.x) {
-            const T s = (sizeof(T) == 8) ? alT * d[i] : st[i];
-            const T y = gn[i] - g[i];
-            st[i] = s;
+        stream(n, [&](int i) { return P3<T>{st[i], gn[i], g[i]}; }, [&](int i, P3<T> v) {
+            const T y = v.b - v.c;
             yt[i] = y;
-            sy += (double)s * (double)y;
-            ss += (double)s * (double)s;
+            sy += (double)v.a * (double)y;
+            ss += (double)v.a * (double)v.a;
             yy += (double)y * (double)y;
-        }
-        sy = wg_sum(sy, r);
-        ss = wg_sum(ss, r);
-        yy = wg_sum(yy, r);
+        });
+        block_sum3(sy, ss, yy, r.d);
         if (sizeof(T) == 4 && ss == 0.0) {
             // FP32 state: the accepted step is below the offsets' resolution
             // everywhere, so the line can no longer move; stop (not
@@ -472,8 +758,8 @@ __device__ void solve_line(const BatchArgs &a, int line, Red r, double *rho, dou
 template <typename T, typename C>
 __global__ __launch_bounds__(kLB) void lbfgs_batch_kernel(BatchArgs a) {
     extern __shared__ unsigned char lds[];
-    __shared__ double s_red[16];
-    __shared__ KSum s_kred[16];
+    __shared__ double s_red[kLS * (kLB / kWave)];
+    __shared__ KSum s_kred[kLB / kWave];
     __shared__ double rho[kMaxM], la[kMaxM];
     __shared__ int slot[kMaxM], sbuf[kMaxM + 1];
     __shared__ int s_line;
@@ -486,6 +772,16 @@ __global__ __launch_bounds__(kLB) void lbfgs_batch_kernel(BatchArgs a) {
         if (qi >= a.nlines) return;
         solve_line<T, C>(a, a.order[qi], r, rho, la, slot, sbuf, reinterpret_cast<C2<T> *>(lds));
     }
+}
+
+// tw[j] = exp(2 pi i j / M), FP64 sincospi rounded to T
+template <typename T>
+__global__ void twiddle_kernel(C2<T> *tw, int M) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= M) return;
+    double s, c;
+    sincospi(2.0 * (double)j / (double)M, &s, &c);
+    tw[j] = {(T)c, (T)s};
 }
 
 __global__ void zero_u32(unsigned *p) { *p = 0u; }
@@ -517,10 +813,18 @@ extern "C" qew_status qew_batch_lbfgs(qew_batch b, int dtype, const qew_lbfgs_pa
         use_device(ctx);
         const size_t es = dtype == QEW_F32 ? sizeof(float) : sizeof(double);
         const size_t nvec = 7 + 2 * ((size_t)lp->m + 1);
-        // layout: [order nlines ints][counter][results][work]
-        const size_t off_next = ((sizeof(int) * b->nlines + 255) / 256) * 256;
+        int maxN = 1;
+        if (lp->precondition)
+            for (const auto &l : b->lines) maxN = std::max(maxN, l.n / 2);
+        // twiddle table length: 4 * the largest transform, so that the stage
+        // twiddles, W_n^k (n = 2N) and sin(pi k / n) are all entries
+        const int tw_m = 4 * maxN;
+        // layout: [order nlines ints][counter][results][twiddles][work]
+        auto up = [](size_t v) { return (v + 255) / 256 * 256; };
+        const size_t off_next = up(sizeof(int) * b->nlines);
         const size_t off_res = off_next + 256;
-        const size_t off_work = off_res + ((sizeof(qew_lbfgs_result) * b->nlines + 255) / 256) * 256;
+        const size_t off_tw = off_res + up(sizeof(qew_lbfgs_result) * b->nlines);
+        const size_t off_work = off_tw + up(2 * es * (size_t)tw_m);
         b->d_scratch.ensure(off_work + es * nvec * (size_t)b->total);
         char *w = b->d_scratch.as<char>();
         std::vector<int> order(b->nlines);
@@ -531,6 +835,13 @@ extern "C" qew_status qew_batch_lbfgs(qew_batch b, int dtype, const qew_lbfgs_pa
                                ctx->stream));
         hipLaunchKernelGGL(zero_u32, dim3(1), dim3(1), 0, ctx->stream,
                            reinterpret_cast<unsigned *>(w + off_next));
+        if (dtype == QEW_F32)
+            hipLaunchKernelGGL(twiddle_kernel<float>, dim3((tw_m + 255) / 256), dim3(256), 0,
+                               ctx->stream, reinterpret_cast<C2<float> *>(w + off_tw), tw_m);
+        else
+            hipLaunchKernelGGL(twiddle_kernel<double>, dim3((tw_m + 255) / 256), dim3(256), 0,
+                               ctx->stream, reinterpret_cast<C2<double> *>(w + off_tw), tw_m);
+        QEW_HIP(hipGetLastError());
         BatchArgs a{};
         a.lines = b->d_lines.as<LineDesc>();
         a.noises = b->d_noises.as<NoiseDesc>();
@@ -540,10 +851,10 @@ extern "C" qew_status qew_batch_lbfgs(qew_batch b, int dtype, const qew_lbfgs_pa
         a.h = h;
         a.work = w + off_work;
         a.results = reinterpret_cast<qew_lbfgs_result *>(w + off_res);
+        a.tw = w + off_tw;
+        a.tw_m = tw_m;
         a.lp = *lp;
-        int maxN = 1;
-        if (lp->precondition)
-            for (const auto &l : b->lines) maxN = std::max(maxN, l.n / 2);
+        // the FFT buffer doubles as the packed-q buffer: n reals = N complex
         const size_t lds = (size_t)maxN * 2 * es;
         auto launch = [&](auto kernel) {
             if (lds > 64 * 1024)
