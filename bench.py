@@ -320,6 +320,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lbfgs", action="store_true", help="skip the batched L-BFGS workload")
     ap.add_argument("--no-langevin", action="store_true", help="skip the Langevin anneal")
+    ap.add_argument("--langevin-steps", type=int, default=100000,
+                    help="anneal length (the workload is 10^5 steps; fewer only for profiling)")
     ap.add_argument("--no-rk", action="store_true", help="skip the batched Rosso-Krauth workload")
     ap.add_argument("--rk-budget", type=float, default=20.0,
                     help="seconds of f_c bisection per precision in the Rosso-Krauth workload")
@@ -465,7 +467,7 @@ def main():
         line["batched_lbfgs"] = bench_lbfgs(log, nlines=args.lbfgs_lines)
     if not args.no_langevin:
         torch.cuda.empty_cache()
-        line["langevin"] = bench_langevin(log)
+        line["langevin"] = bench_langevin(log, nsteps=args.langevin_steps)
     if not args.no_rk:
         torch.cuda.empty_cache()
         line["rosso_krauth"] = bench_rk(log, budget_s=args.rk_budget)
