@@ -129,6 +129,10 @@ struct qew_batch_s {
     std::vector<int64_t> offsets;
     qew_host::DevBuf d_lines, d_noises, d_tile_line, d_partial;
     qew_host::DevBuf d_scratch;  // solver workspaces (grown on demand)
+    // multi-step Langevin tiling: (line, first output site) per tile, built
+    // on first use
+    int ltiles = 0;
+    qew_host::DevBuf d_ltile;
 };
 
 namespace qew_host {

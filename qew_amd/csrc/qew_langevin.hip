@@ -151,6 +151,141 @@ __global__ __launch_bounds__(kTile) void langevin32_kernel(StepArgs a) {
     static_cast<float *>(a.h_out)[L.off + i] = hn;
 }
 
+// Several steps per launch. A workgroup holds an extended tile of kLvExt
+// consecutive sites (mod n) in LDS and advances all of them; after step s
+// the outer s + 1 sites at each end depend on sites outside the tile, so
+// after kLvSteps steps the central kLvOut sites are exact and are the tile's
+// output. Each site's update is the single-step kernel's (same operands,
+// operation order and Philox key (seed + line, step, site)), so a launch
+// equals kLvSteps single steps bit for bit; the halo work is redundant
+// (kLvExt / kLvOut = 1.14) and saves kLvSteps - 1 launches and round trips
+// through memory. Short lines (n < kLvExt) alias sites within the tile: the
+// copies compute identical values and each site is written once.
+constexpr int kLvSteps = 16;
+constexpr int kLvExt = 256;
+constexpr int kLvOut = kLvExt - 2 * kLvSteps;
+
+struct MultiArgs {
+    const LineDesc *lines;
+    const NoiseDesc *noises;
+    const int2 *tiles;  // (line, first output site) per workgroup
+    const void *h_in;
+    void *h_out;
+    const double *anchor;
+    double drift[kLvSteps], diff[kLvSteps];
+    int nsteps;     // steps of this launch, 1..kLvSteps
+    uint64_t seed;  // line l uses seed + l
+    uint64_t step0;
+};
+
+// extended position t of a tile whose first output site is `out0`
+__device__ __forceinline__ int ext_site(int out0, int t, int n) {
+    int gi = (out0 - kLvSteps + t) % n;
+    return gi < 0 ? gi + n : gi;
+}
+
+template <typename C>
+__global__ __launch_bounds__(kLvExt) void langevin64_multi(MultiArgs a) {
+    __shared__ double s_h[kLvExt];
+    __shared__ double s_dvdy[kLvExt];   // [t + 1]: edge between positions t, t + 1
+    __shared__ double s_slope[kLvExt];
+    const int t = threadIdx.x;
+    const int2 tl = a.tiles[blockIdx.x];
+    const int line = tl.x;
+    const LineDesc L = a.lines[line];
+    const NoiseDesc N = a.noises[L.noise];
+    const int n = L.n;
+    const double dx = L.L / (double)n;
+    const double lt = L.lt, f = L.f;
+    const bool arc = (L.model == 1);
+    const int gi = ext_site(tl.y, t, n);
+    const int gp = (gi + 1 == n) ? 0 : gi + 1;
+    const double xc = (double)(gi + gp) * 0.5 * dx;
+    s_h[t] = static_cast<const double *>(a.h_in)[L.off + gi];
+    __syncthreads();
+    for (int s = 0; s < a.nsteps; ++s) {
+        const double hi = s_h[t];
+        if (t + 1 < kLvExt) {
+            const double hp = s_h[t + 1];
+            s_dvdy[t + 1] = sample_dy64<C>(N, xc, (hp + hi) * 0.5);
+            if (arc) {
+                const double dh = (hp - hi) / dx;
+                s_slope[t + 1] = dh / sqrt(1.0 + dh * dh);
+            }
+        }
+        __syncthreads();
+        double hn = hi;
+        if (t >= 1 && t + 1 < kLvExt) {
+            double line_force;
+            if (!arc) line_force = -lt * (s_h[t - 1] + s_h[t + 1] - 2.0 * hi) / (dx * dx);
+            else line_force = lt * (s_slope[t] - s_slope[t + 1]) / dx;
+            const double g = dx * (line_force + 0.5 * (s_dvdy[t] + s_dvdy[t + 1]) - f);
+            const double dr = a.drift[s], df = a.diff[s];
+            if (df > 0)
+                hn = hi + (-dr * g + df * philox_normal(a.seed + (uint64_t)line, a.step0 + (uint64_t)s,
+                                                        (uint64_t)gi));
+            else
+                hn = hi + -dr * g;
+        }
+        __syncthreads();
+        s_h[t] = hn;
+        __syncthreads();
+    }
+    const int o = t - kLvSteps;
+    if (o >= 0 && o < kLvOut && tl.y + o < n)
+        static_cast<double *>(a.h_out)[L.off + tl.y + o] = s_h[t];
+}
+
+__global__ __launch_bounds__(kLvExt) void langevin32_multi(MultiArgs a) {
+    __shared__ float s_h[kLvExt];
+    __shared__ float s_dvdy[kLvExt];
+    __shared__ float s_slope[kLvExt];
+    const int t = threadIdx.x;
+    const int2 tl = a.tiles[blockIdx.x];
+    const int line = tl.x;
+    const LineDesc L = a.lines[line];
+    const NoiseDesc N = a.noises[L.noise];
+    const int n = L.n;
+    const double anchor = a.anchor ? a.anchor[line] : 0.0;
+    const LineGeom G = make_geom(L, N, anchor);
+    const float dxf = G.dxf, inv_dx = G.inv_dx;
+    const float lt = (float)L.lt, f = (float)L.f;
+    const bool arc = (L.model == 1);
+    const int gi = ext_site(tl.y, t, n);
+    const int gp = (gi + 1 == n) ? 0 : gi + 1;
+    s_h[t] = static_cast<const float *>(a.h_in)[L.off + gi];
+    __syncthreads();
+    for (int s = 0; s < a.nsteps; ++s) {
+        const float hi = s_h[t];
+        if (t + 1 < kLvExt) {
+            const float hp = s_h[t + 1];
+            float v, dy;
+            edge32<false, true>(G, gi, gp, hi, hp, v, dy);
+            s_dvdy[t + 1] = dy;
+            const float dh = (hp - hi) * inv_dx;
+            s_slope[t + 1] = dh * rsqrtf(1.0f + dh * dh);
+        }
+        __syncthreads();
+        float hn = hi;
+        if (t >= 1 && t + 1 < kLvExt) {
+            float lf;
+            if (!arc) lf = -lt * ((s_h[t - 1] + s_h[t + 1]) - 2.0f * hi) * (inv_dx * inv_dx);
+            else lf = lt * (s_slope[t] - s_slope[t + 1]) * inv_dx;
+            const float g = dxf * (lf + 0.5f * (s_dvdy[t] + s_dvdy[t + 1]) - f);
+            hn = hi - (float)a.drift[s] * g;
+            if (a.diff[s] > 0)
+                hn += (float)(a.diff[s] * philox_normal(a.seed + (uint64_t)line,
+                                                        a.step0 + (uint64_t)s, (uint64_t)gi));
+        }
+        __syncthreads();
+        s_h[t] = hn;
+        __syncthreads();
+    }
+    const int o = t - kLvSteps;
+    if (o >= 0 && o < kLvOut && tl.y + o < n)
+        static_cast<float *>(a.h_out)[L.off + tl.y + o] = s_h[t];
+}
+
 }  // namespace
 
 using namespace qew_host;
@@ -210,6 +345,49 @@ qew_status qew_batch_langevin(qew_batch b, int dtype, void *h, const double *anc
         }
         b->d_scratch.ensure(es * (size_t)b->total);
         void *bufs[2] = {h, b->d_scratch.ptr};
+        const int table_dtype0 = b->noises[0].dtype;
+        if (nsteps >= 2) {
+            // kLvSteps steps per launch on extended tiles
+            if (b->ltiles == 0) {
+                std::vector<int2> tiles;
+                for (int l = 0; l < b->nlines; ++l)
+                    for (int o = 0; o < b->lines[l].n; o += kLvOut) tiles.push_back(make_int2(l, o));
+                b->d_ltile.ensure(sizeof(int2) * tiles.size());
+                QEW_HIP(hipMemcpy(b->d_ltile.ptr, tiles.data(), sizeof(int2) * tiles.size(),
+                                  hipMemcpyHostToDevice));
+                b->ltiles = (int)tiles.size();
+            }
+            MultiArgs m{};
+            m.lines = b->d_lines.as<LineDesc>();
+            m.noises = b->d_noises.as<NoiseDesc>();
+            m.tiles = b->d_ltile.as<int2>();
+            m.anchor = anchor;
+            m.seed = seed;
+            int launches = 0;
+            for (int k0 = 0; k0 < nsteps; k0 += kLvSteps, ++launches) {
+                m.nsteps = std::min(kLvSteps, nsteps - k0);
+                for (int s = 0; s < m.nsteps; ++s) {
+                    m.drift[s] = drift[k0 + s];
+                    m.diff[s] = diff[k0 + s];
+                }
+                m.step0 = step0 + (uint64_t)k0;
+                m.h_in = bufs[launches & 1];
+                m.h_out = bufs[(launches + 1) & 1];
+                if (dtype == QEW_F32)
+                    hipLaunchKernelGGL(langevin32_multi, dim3(b->ltiles), dim3(kLvExt), 0, ctx->stream, m);
+                else if (table_dtype0 == 1)
+                    hipLaunchKernelGGL(langevin64_multi<float>, dim3(b->ltiles), dim3(kLvExt), 0,
+                                       ctx->stream, m);
+                else
+                    hipLaunchKernelGGL(langevin64_multi<double>, dim3(b->ltiles), dim3(kLvExt), 0,
+                                       ctx->stream, m);
+                QEW_HIP(hipGetLastError());
+            }
+            if (launches & 1)
+                QEW_HIP(hipMemcpyAsync(h, b->d_scratch.ptr, es * (size_t)b->total,
+                                       hipMemcpyDeviceToDevice, ctx->stream));
+            return;
+        }
         StepArgs a{};
         a.lines = b->d_lines.as<LineDesc>();
         a.noises = b->d_noises.as<NoiseDesc>();

@@ -744,6 +744,7 @@ qew_status qew_batch_destroy(qew_batch b) {
         b->d_tile_line.release();
         b->d_partial.release();
         b->d_scratch.release();
+        b->d_ltile.release();
         delete b;
     });
 }

@@ -169,6 +169,55 @@ def test_batch_langevin_matches_single(q):
     assert np.max(np.abs(host(hb)[n:] - host(hs))) > 0  # line 1 uses seed + 1
 
 
+@pytest.mark.parametrize("table", ["f64", "f32"])
+def test_batch_langevin_multistep_tiles_bitwise(q, table):
+    # several steps per launch on tiles with a halo (lines shorter and longer
+    # than a tile, both models, a partial last launch, steps without noise):
+    # every line equals single steps of the same stream bit for bit
+    field, coef = O.filtered_noise(256, 64, 2.56, 1.0, 0.5, 0.2, 0.2, 6)
+    gn = q.FilteredNoise.from_coefficients(coef, 0.01, 1.0 / 64)
+    if table == "f32":
+        gn = gn.convert(q.F32)
+    lengths = [5, 100, 224, 1000, 2]
+    params = [q.Params(2.56, 0.8, 0.05, q.Model(k % 2)) for k in range(len(lengths))]
+    b = q.Batch(lengths, [gn] * len(lengths), params)
+    rng = np.random.default_rng(12)
+    h0 = [rng.uniform(0.3, 0.7, n) for n in lengths]
+    steps = 37
+    drift = np.full(steps, 0.002)
+    diff = np.where(np.arange(steps) % 5 == 3, 0.0, 0.01)
+    hb = dev(np.concatenate(h0))
+    b.langevin(hb, drift, diff, seed=21, step0=1000)
+    off = np.concatenate([[0], np.cumsum(lengths)])
+    for k, n in enumerate(lengths):
+        hs = dev(h0[k])
+        dyn = q.LangevinDynamics(n, 21 + k)
+        dyn.steps = 1000
+        for s in range(steps):
+            dyn.step(hs, params[k], gn, drift[s], diff[s])
+        np.testing.assert_array_equal(host(hb)[off[k]:off[k + 1]], host(hs), err_msg=f"line {k} n={n}")
+
+
+def test_batch_langevin_fp32_multistep_matches_single_steps(q):
+    # FP32 line state: a multi-step call equals the same steps issued one per call
+    n, ny = 700, 256
+    field, coef = O.filtered_noise(n, ny, float(n), float(ny), 1.0, 4.0, 4.0, 5)
+    g32 = q.FilteredNoise.from_coefficients(coef, 1.0, 1.0, dtype=q.F32)
+    p = [q.Params(float(n), 1.0, 0.05, q.Model.Linear), q.Params(float(n), 1.0, 0.05, q.Model.Arclength)]
+    b = q.Batch([n, n], [g32, g32], p)
+    anchor = dev(np.array([ny / 2.0, ny / 2.0 + 0.25]))
+    o0 = np.random.default_rng(2).normal(0, 1, 2 * n).astype(np.float32)
+    steps = 40
+    drift = np.full(steps, 0.01)
+    diff = np.full(steps, np.sqrt(2 * 0.01 * 0.05))
+    om = dev(o0)
+    b.langevin(om, drift, diff, seed=3, dtype=q.F32, anchor=anchor)
+    os_ = dev(o0)
+    for s in range(steps):
+        b.langevin(os_, drift[s:s + 1], diff[s:s + 1], seed=3, step0=s, dtype=q.F32, anchor=anchor)
+    np.testing.assert_array_equal(host(om), host(os_))
+
+
 # ---- Rosso-Krauth --------------------------------------------------------------
 
 def test_rk_block_matches_oracle_bitwise(q):
