@@ -232,6 +232,9 @@ private:
     DeviceNoise dn_;
 };
 
+// a FilteredNoise passed where a Noise is expected samples its device table
+inline DeviceNoise as_device(const FilteredNoise &n) { return n.device_noise(); }
+
 // ---- model (src/qew_model.h) ----------------------------------------------
 enum class Model { Linear = QEW_LINEAR, Arclength = QEW_ARCLENGTH };
 
@@ -336,6 +339,24 @@ public:
         check(qew_langevin_step(context(), as_device(noise).handle(), &c, n_, h.data(), drift_coeff,
                                 diff_coeff, seed_, step_));
         ++step_;
+    }
+    // `count` steps with per-step coefficients drift[k], diff[k]; the same
+    // stream and result as `count` calls of step(), several steps per launch.
+    // No reference counterpart.
+    template <class Noise>
+    void steps(const View1D &h, const Params &p, const Noise &noise, const real_t *drift,
+               const real_t *diff, int count) {
+        if ((int)h.extent(0) != n_) throw std::invalid_argument("line length differs");
+        if (count <= 0) return;
+        const qew_params c = p.c();
+        const DeviceNoise dn = as_device(noise);
+        qew_noise nh = dn.handle();
+        const int len = n_;
+        qew_batch b = nullptr;
+        check(qew_batch_create(context(), 1, &len, &nh, &c, &b));
+        std::unique_ptr<qew_batch_s, qew_status (*)(qew_batch)> owner(b, qew_batch_destroy);
+        check(qew_batch_langevin(b, QEW_F64, h.data(), nullptr, count, drift, diff, seed_, step_));
+        step_ += (std::uint64_t)count;
     }
     std::uint64_t steps_taken() const { return step_; }
 

@@ -137,17 +137,28 @@ DynamicResult dynamic_anneal(const DynamicConfig &c, const Noise &noise) {
     real_t t = 0;
     real_t next_save = 10 * dt;
     real_t tl = c.tl_start;
+    // the schedule is run on the host exactly as the reference loop runs it;
+    // the steps between snapshots go to the device together
+    std::vector<real_t> drift, diff;
+    auto flush = [&] {
+        dyn.steps(h, p, dn, drift.data(), diff.data(), (int)drift.size());
+        drift.clear();
+        diff.clear();
+    };
     while (t < c.total_time) {
-        const real_t drift = tl * dt / (tau * c.amplitude);
-        const real_t diff = tl * std::sqrt(2 * dt / tau);
-        dyn.step(h, p, dn, drift, diff);
+        drift.push_back(tl * dt / (tau * c.amplitude));
+        diff.push_back(tl * std::sqrt(2 * dt / tau));
         tl = c.tl_start + (c.tl_end - c.tl_start) * (t / c.total_time);
         t += dt;
         if (t > next_save - dt / 2) {
+            flush();
             res.snapshots.push_back(DynamicSnapshot{t, to_host_vec(h)});
             next_save *= 10;
+        } else if (drift.size() >= (1u << 16)) {
+            flush();
         }
     }
+    flush();
     return res;
 }
 

@@ -87,6 +87,53 @@ static void dynamic_anneal_timestep_and_snapshots() {
     CHECK(res.snapshots.back().t <= res.total_time);
 }
 
+// dynamic_anneal sends the steps between snapshots to the device together;
+// the snapshots must equal the reference loop run one step per call
+static void dynamic_anneal_matches_step_loop() {
+    DynamicConfig cfg;
+    cfg.model = Model::Arclength;
+    cfg.amplitude = 1.0;
+    cfg.xi = 0.1;
+    cfg.pinning_length_over_xi = 1.0;
+    cfg.Lx = 3.0;
+    cfg.Ly = 1.0;
+    cfg.nx = 300;
+    cfg.ny = 64;
+    cfg.seed = 5;
+    cfg.tl_start = 0.01;
+    cfg.tl_end = 0.001;
+    cfg.total_time = 50.0;
+    const FilteredNoise noise(cfg.nx, cfg.ny, cfg.Lx, cfg.Ly, cfg.amplitude, cfg.xi, cfg.xi, 7);
+    const DynamicResult res = dynamic_anneal(cfg, noise);
+
+    const double dt = dynamic_timestep(cfg);
+    Params p;
+    p.physical_size = cfg.Lx;
+    p.line_tension = larkin_line_tension(cfg.pinning_length_over_xi, cfg.amplitude);
+    p.driving_force = cfg.driving_force;
+    p.model = cfg.model;
+    View1D h("h", cfg.nx);
+    deep_copy(h, 0.5 * cfg.Ly);
+    LangevinDynamics dyn(cfg.nx, cfg.seed + 1);
+    double t = 0, next_save = 10 * dt, tl = cfg.tl_start;
+    size_t k = 0;
+    while (t < cfg.total_time) {
+        dyn.step(h, p, noise, tl * dt / (cfg.tau * cfg.amplitude), tl * std::sqrt(2 * dt / cfg.tau));
+        tl = cfg.tl_start + (cfg.tl_end - cfg.tl_start) * (t / cfg.total_time);
+        t += dt;
+        if (t > next_save - dt / 2) {
+            CHECK(k < res.snapshots.size());
+            if (k < res.snapshots.size()) {
+                CHECK(res.snapshots[k].t == t);
+                CHECK(res.snapshots[k].h == to_host_vec(h));
+            }
+            ++k;
+            next_save *= 10;
+        }
+    }
+    CHECK(k == res.snapshots.size() && k >= 2);
+}
+
 static void depinning_ramp_brackets_critical_force() {
     const SinusoidNoise noise;
     const double fc_exact = noise.A * noise.k;
@@ -184,6 +231,7 @@ int main() {
     fmt_specs();
     static_sweep_shape_and_mapping();
     dynamic_anneal_timestep_and_snapshots();
+    dynamic_anneal_matches_step_loop();
     depinning_ramp_brackets_critical_force();
     facade_semantics();
     finalize();
