@@ -74,6 +74,18 @@ __device__ __forceinline__ void bspline_dw(double t, double dw[4]) {
     dw[3] = 3.0 * t2 / 6.0;
 }
 
+// a / b, skipping the division when b is exactly 1 (a / 1.0 == a bit for bit)
+__device__ __forceinline__ double div1(double a, double b) { return b == 1.0 ? a : a / b; }
+
+// g - floor(g / n) * n. For 0 <= g <= fl(n (1 - 2^-52)) the correctly rounded
+// g / n is at most 1 - 2^-53, so the floor is 0 and g comes back unchanged:
+// the division is skipped there with the same result bit for bit.
+__device__ __forceinline__ double wrap_cells(double g, int n) {
+    const double nd = (double)n;
+    if (g >= 0.0 && g <= nd * (1.0 - 0x1p-52)) return g;
+    return g - floor(g / nd) * nd;
+}
+
 // DeviceNoise::locate (src/filtered_noise.h:77-101). Division by a spacing of
 // exactly 1 is skipped: x / 1.0 == x bit for bit.
 __device__ __forceinline__ void locate64(const NoiseDesc &nd, double x, double y, int ixn[4],
@@ -81,8 +93,8 @@ __device__ __forceinline__ void locate64(const NoiseDesc &nd, double x, double y
     const int nx = nd.nx, ny = nd.ny;
     double gx = (nd.dx == 1.0) ? x : x / nd.dx;
     double gy = (nd.dy == 1.0) ? y : y / nd.dy;
-    gx -= floor(gx / (double)nx) * (double)nx;
-    gy -= floor(gy / (double)ny) * (double)ny;
+    gx = wrap_cells(gx, nx);
+    gy = wrap_cells(gy, ny);
     int ix = (int)floor(gx);
     int iy = (int)floor(gy);
     if (ix >= nx) { ix -= nx; gx -= (double)nx; }

@@ -220,7 +220,7 @@ __device__ double wg_objective(const LineGeom &G, const T *x, const T *d, double
                [&](int i, P2<double> v) {
                    const int ip = (i + 1 == n) ? 0 : i + 1;
                    const double hi = v.a, hip = v.b;
-                   const double dh = (hip - hi) / G.dx;
+                   const double dh = div1(hip - hi, G.dx);
                    const double le =
                        (G.model == 0) ? G.lt * dh * dh * 0.5 : G.lt * (sqrt(1.0 + dh * dh) - 1.0);
                    const double xc = (double)(i + ip) * 0.5 * G.dx;
@@ -243,7 +243,7 @@ __device__ double wg_objective(const LineGeom &G, const T *x, const T *d, double
                    const int ip = (i + 1 == n) ? 0 : i + 1;
                    const double oi = v.a, oip = v.b;
                    const double hi = G.anchor + oi, hip = G.anchor + oip;
-                   const double dh = (oip - oi) / G.dx;
+                   const double dh = div1(oip - oi, G.dx);
                    const double s2 = dh * dh;
                    const double le =
                        (G.model == 0) ? G.lt * s2 * 0.5 : G.lt * (s2 / (sqrt(1.0 + s2) + 1.0));
@@ -278,7 +278,7 @@ __device__ __forceinline__ void wg_objective_trials(const LineGeom &G, const T *
                 if constexpr (sizeof(T) == 8) {
                     // src/qew_model.h:53-99 (TrialAt: x0(i) + alpha * d(i))
                     const double hi = v.a + alpha[j] * v.b, hip = v.c + alpha[j] * v.d;
-                    const double dh = (hip - hi) / G.dx;
+                    const double dh = div1(hip - hi, G.dx);
                     const double le = (G.model == 0) ? G.lt * dh * dh * 0.5
                                                      : G.lt * (sqrt(1.0 + dh * dh) - 1.0);
                     const double hc = (hip + hi) * 0.5;
@@ -288,7 +288,7 @@ __device__ __forceinline__ void wg_objective_trials(const LineGeom &G, const T *
                     const double oi = (double)__builtin_fmaf(al, v.b, v.a);
                     const double oip = (double)__builtin_fmaf(al, v.d, v.c);
                     const double hi = G.anchor + oi, hip = G.anchor + oip;
-                    const double dh = (oip - oi) / G.dx;
+                    const double dh = div1(oip - oi, G.dx);
                     const double s2 = dh * dh;
                     const double le = (G.model == 0) ? G.lt * s2 * 0.5
                                                      : G.lt * (s2 / (sqrt(1.0 + s2) + 1.0));
@@ -312,7 +312,7 @@ __device__ void wg_gradient(const LineGeom &G, const T *x, T *g, T *edy, T *esl)
            [&](int i, P2<T> v) {
                const int ip = (i + 1 == n) ? 0 : i + 1;
                if constexpr (sizeof(T) == 8) {
-                   const double dh = (v.b - v.a) / G.dx;
+                   const double dh = div1(v.b - v.a, G.dx);
                    edy[i] = sample_dy64<C>(G.nd, (double)(i + ip) * 0.5 * G.dx, (v.b + v.a) * 0.5);
                    if (arc) esl[i] = dh / sqrt(1.0 + dh * dh);
                } else {
@@ -348,8 +348,8 @@ __device__ void wg_gradient(const LineGeom &G, const T *x, T *g, T *edy, T *esl)
            [&](int i, Nb b) {
                if constexpr (sizeof(T) == 8) {
                    double lf;
-                   if (!arc) lf = -G.lt * (b.xm + b.xp - 2.0 * b.xi) / (G.dx * G.dx);
-                   else lf = G.lt * (b.sm - b.si) / G.dx;
+                   if (!arc) lf = div1(-G.lt * (b.xm + b.xp - 2.0 * b.xi), G.dx * G.dx);
+                   else lf = div1(G.lt * (b.sm - b.si), G.dx);
                    g[i] = G.dx * (lf + 0.5 * (b.em + b.ei) - G.f);
                } else {
                    float lf;

@@ -63,11 +63,11 @@ __device__ __forceinline__ double site_velocity(int i, int n, double dx, double 
     const int ip = (i + 1 == n) ? 0 : i + 1;
     double line_force;
     if (model == 0) {
-        line_force = -lt * (hl + hr - 2.0 * hi) / (dx * dx);
+        line_force = div1(-lt * (hl + hr - 2.0 * hi), dx * dx);
     } else {
-        const double dhr = (hr - hi) / dx;
-        const double dhl = (hi - hl) / dx;
-        line_force = lt * (dhl / sqrt(1.0 + dhl * dhl) - dhr / sqrt(1.0 + dhr * dhr)) / dx;
+        const double dhr = div1(hr - hi, dx);
+        const double dhl = div1(hi - hl, dx);
+        line_force = div1(lt * (dhl / sqrt(1.0 + dhl * dhl) - dhr / sqrt(1.0 + dhr * dhr)), dx);
     }
     const double xcl = (double)(im + i) * 0.5 * dx;
     const double hcl = (hl + hi) * 0.5;
@@ -129,11 +129,11 @@ __device__ __forceinline__ double site_velocity_held(int i, int n, double dx, do
     const int ip = (i + 1 == n) ? 0 : i + 1;
     double line_force;
     if (model == 0) {
-        line_force = -lt * (hl + hr - 2.0 * hi) / (dx * dx);
+        line_force = div1(-lt * (hl + hr - 2.0 * hi), dx * dx);
     } else {
-        const double dhr = (hr - hi) / dx;
-        const double dhl = (hi - hl) / dx;
-        line_force = lt * (dhl / sqrt(1.0 + dhl * dhl) - dhr / sqrt(1.0 + dhr * dhr)) / dx;
+        const double dhr = div1(hr - hi, dx);
+        const double dhl = div1(hi - hl, dx);
+        line_force = div1(lt * (dhl / sqrt(1.0 + dhl * dhl) - dhr / sqrt(1.0 + dhr * dhr)), dx);
     }
     const double dvl = el.eval(nd, (double)(im + i) * 0.5 * dx, (hl + hi) * 0.5);
     const double dvr = er.eval(nd, (double)(i + ip) * 0.5 * dx, (hi + hr) * 0.5);
