@@ -74,6 +74,17 @@ __device__ __forceinline__ void bspline_dw(double t, double dw[4]) {
     dw[3] = 3.0 * t2 / 6.0;
 }
 
+// Workgroups are dispatched round-robin over the part's 8 XCDs (block b on
+// XCD b % 8), each with its own L2. This bijection of [0, nb) gives every XCD
+// a contiguous range of tiles, so neighbouring tiles of a line -- which share
+// the 128-B lines at their edges -- meet in the same L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    const int x = b % kXcds, s = b / kXcds;
+    const int q = nb / kXcds, r = nb % kXcds;
+    return x * q + (x < r ? x : r) + s;
+}
+
 // a / b, skipping the division when b is exactly 1 (a / 1.0 == a bit for bit)
 __device__ __forceinline__ double div1(double a, double b) { return b == 1.0 ? a : a / b; }
 

@@ -51,7 +51,7 @@ __global__ __launch_bounds__(kTile) void eval64_kernel(qew_host::EvalArgs a, Lin
     __shared__ double s_h[kTile + 1];
     __shared__ double red[16];
     const int tid = threadIdx.x;
-    const int tile = blockIdx.x;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
     int line;
     LineDesc L;
     NoiseDesc N;
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(kTile) void eval32_kernel(qew_host::EvalArgs a) {
     __shared__ float s_h[kTile + 1];
     __shared__ KSum red[16];
     const int tid = threadIdx.x;
-    const int tile = blockIdx.x;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int line = a.tile_line[tile];
     const LineDesc L = a.lines[line];
     const NoiseDesc N = a.noises[L.noise];
