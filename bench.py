@@ -327,6 +327,7 @@ def main():
                     help="seconds of f_c bisection per precision in the Rosso-Krauth workload")
     ap.add_argument("--lbfgs-lines", type=int, default=512)
     ap.add_argument("--only", choices=["both", "f32", "f64"], default="both")
+    ap.add_argument("--no-bf16", action="store_true", help="skip the BF16-table evaluation leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (reference point for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE scaling) and exit")
@@ -359,11 +360,15 @@ def main():
     results = {}
     params = q.Params(float(L), 1.0, 0.0, q.Model.Arclength)
 
-    def run(dtype):
+    def run(table):
+        # table storage "f64" (FP64 line state), "f32" or "bf16" (FP32 line state)
         t_gen = time.perf_counter()
+        dtype = q.F64 if table == "f64" else q.F32
         noise = q.FilteredNoise(L, NY, float(L), float(NY), 1.0, XI, XI, 1, dtype=dtype)
+        if table == "bf16":
+            noise = noise.convert(q.BF16)
         torch.cuda.synchronize()
-        log(f"table {'f32' if dtype else 'f64'} generated in {time.perf_counter() - t_gen:.1f} s")
+        log(f"table {table} generated in {time.perf_counter() - t_gen:.1f} s")
         batch = q.Batch([L] * B, [noise] * B, [params] * B)
         h0d = torch.tensor(h0, dtype=torch.float64, device=dev)
         if dtype == q.F64:
@@ -389,8 +394,9 @@ def main():
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         kms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
-        s_c = 4 if dtype == q.F32 else 8
-        bytes_eval = s_c * u_sum + L * 2 * s_c
+        s_c = {"f64": 8, "f32": 4, "bf16": 2}[table]  # coefficient bytes
+        s_h = 8 if dtype == q.F64 else 4               # line state and gradient bytes
+        bytes_eval = s_c * u_sum + L * 2 * s_h
         res = {
             "evals_per_s": B * args.steps / wall,
             "ms_per_step": 1e3 * wall / args.steps,
@@ -399,28 +405,32 @@ def main():
             "achieved_GBs": bytes_eval * B / (kms * 1e-3) / 1e9,
         }
         first = (h[:L].clone(), grad[:L].clone(), float(energy[0]))
-        if dtype == q.F64:
+        if table == "f64":
             results["table64"] = noise
         del batch, h, grad, energy
-        if dtype == q.F32:
+        if table != "f64":
             del noise
         torch.cuda.empty_cache()
         return res, first
 
-    r32 = r64 = None
+    r32 = r64 = r16 = None
     if args.only in ("both", "f64"):
-        r64, first64 = run(q.F64)
+        r64, first64 = run("f64")
         log(f"f64: {json.dumps(r64)}")
     if args.only in ("both", "f32"):
-        r32, first32 = run(q.F32)
+        r32, first32 = run("f32")
         log(f"f32: {json.dumps(r32)}")
+    if args.only == "both" and not args.no_bf16:
+        r16, first16 = run("bf16")
+        log(f"bf16: {json.dumps(r16)}")
 
-    err = None
-    if r32 and r64:
+    def error_vs_f64(first):
         g64 = first64[1].cpu().numpy()
-        g32 = first32[1].cpu().numpy().astype(np.float64)
-        err = {"grad_max_abs_err_over_max_grad": float(np.max(np.abs(g32 - g64)) / np.max(np.abs(g64))),
-               "energy_rel_err": abs(first32[2] - first64[2]) / abs(first64[2])}
+        g = first[1].cpu().numpy().astype(np.float64)
+        return {"grad_max_abs_err_over_max_grad": float(np.max(np.abs(g - g64)) / np.max(np.abs(g64))),
+                "energy_rel_err": abs(first[2] - first64[2]) / abs(first64[2])}
+
+    err = error_vs_f64(first32) if (r32 and r64) else None
 
     cpu = None
     if not args.no_cpu and "table64" in results:
@@ -462,6 +472,13 @@ def main():
         line["f32_bytes_per_eval"] = r32["bytes_per_eval"]
     if err:
         line["fp32_vs_fp64_error"] = err
+    if r16:
+        # BF16 coefficient storage, FP32 line state and arithmetic (optional mode)
+        line["bf16_table"] = {"value": r16["evals_per_s"], "ms_per_step": r16["ms_per_step"],
+                              "bytes_per_eval": r16["bytes_per_eval"],
+                              "roofline_achieved_GBs": r16["achieved_GBs"],
+                              "roofline_frac": r16["achieved_GBs"] / HBM_PEAK_GBS,
+                              "error_vs_f64": error_vs_f64(first16) if r64 else None}
     if not args.no_lbfgs:
         torch.cuda.empty_cache()
         line["batched_lbfgs"] = bench_lbfgs(log, nlines=args.lbfgs_lines)

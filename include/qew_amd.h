@@ -34,8 +34,12 @@ typedef enum qew_status {
     QEW_ERR_UNSUPPORTED = 4
 } qew_status;
 
-/* Arithmetic / storage type of line state and coefficient tables. */
-typedef enum qew_dtype { QEW_F64 = 0, QEW_F32 = 1 } qew_dtype;
+/* Arithmetic / storage type of line state and coefficient tables. QEW_BF16
+ * is a table storage type only (bfloat16 coefficients, rounded to nearest
+ * even from FP64 via FP32): such tables are read by qew_batch_eval and
+ * qew_batch_langevin with QEW_F32 line state; every other entry point
+ * rejects them with QEW_ERR_INVALID_ARGUMENT. */
+typedef enum qew_dtype { QEW_F64 = 0, QEW_F32 = 1, QEW_BF16 = 2 } qew_dtype;
 
 /* qew::Model (src/qew_model.h:18). */
 typedef enum qew_model { QEW_LINEAR = 0, QEW_ARCLENGTH = 1 } qew_model;

@@ -10,7 +10,7 @@ _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libqew_amd.so")
 
 QEW_OK, QEW_ERR_INVALID_ARGUMENT, QEW_ERR_HIP, QEW_ERR_OUT_OF_MEMORY, QEW_ERR_UNSUPPORTED = range(5)
-F64, F32 = 0, 1
+F64, F32, BF16 = 0, 1, 2  # BF16: table storage only (read with F32 line state)
 LINEAR, ARCLENGTH = 0, 1
 TABLE, SINCOS, CONST, ZERO, HARMONIC, WASHBOARD = range(6)
 

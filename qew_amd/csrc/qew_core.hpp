@@ -138,6 +138,25 @@ __device__ __forceinline__ void row4(const float *t, int pitch, int ix, int iy, 
     out[3] = r.w;
 }
 
+// The same 4 coefficients from a BF16 table (dtype 2): one 4-byte-aligned
+// 16-byte load from the even element at or below the window (BF16 rows are
+// padded to roundup(nx + 7, 8) elements, so the load stays in the row), then
+// each bfloat16 widened to float by a 16-bit shift (little-endian: the even
+// element is the low half of a dword).
+typedef unsigned u4u __attribute__((ext_vector_type(4), aligned(4)));
+__device__ __forceinline__ void row4_bf16(const unsigned short *t, int pitch, int ix, int iy,
+                                          float out[4]) {
+    const u4u r = *(const QEW_GLOBAL u4u *)(t + (size_t)iy * pitch + (ix & ~1));
+    const bool odd = (ix & 1) != 0;
+    const unsigned lo0 = r.x << 16, hi0 = r.x & 0xffff0000u;
+    const unsigned lo1 = r.y << 16, hi1 = r.y & 0xffff0000u;
+    const unsigned lo2 = r.z << 16;
+    out[0] = __uint_as_float(odd ? hi0 : lo0);
+    out[1] = __uint_as_float(odd ? lo1 : hi0);
+    out[2] = __uint_as_float(odd ? hi1 : lo1);
+    out[3] = __uint_as_float(odd ? lo2 : hi1);
+}
+
 template <typename C>
 __device__ __forceinline__ void row4_64(const NoiseDesc &nd, int ix, int iy, double out[4]) {
     if constexpr (sizeof(C) == 8) {
@@ -534,37 +553,37 @@ __device__ __forceinline__ LineGeom make_geom(const LineDesc &L, const NoiseDesc
     return g;
 }
 
-// FP32 edge (e, ep) of a line whose heights are FP32 offsets from the
-// anchor: value and/or y-derivative of the potential at the edge midpoint.
-template <bool WANT_V, bool WANT_DY>
-__device__ __forceinline__ void edge32(const LineGeom &G, int e, int ep, float he, float hep,
-                                       float &v, float &dvdy) {
+// Cell and fractions of the FP32 edge (e, ep) of a line whose heights are
+// FP32 offsets from the anchor (iy already reduced into [0, ny)).
+struct Edge32Loc {
+    int ix, iy;
+    float fx, fy;
+};
+__device__ __forceinline__ Edge32Loc edge32_locate(const LineGeom &G, int e, int ep, float he,
+                                                   float hep) {
     const int nx = G.nd.nx, ny = G.nd.ny;
     double gx = (double)(e + ep) * G.cx;
     if (gx >= (double)nx) gx -= floor(gx / (double)nx) * (double)nx;
-    int ix = (int)gx;
-    const float fx = (float)(gx - (double)ix);
-    if (ix >= nx) ix -= nx;
+    Edge32Loc l;
+    l.ix = (int)gx;
+    l.fx = (float)(gx - (double)l.ix);
+    if (l.ix >= nx) l.ix -= nx;
     const float t = __builtin_fmaf((he + hep) * 0.5f, G.inv_dy, G.ya.frac);
     const float kf = floorf(t);
-    const float fy = t - kf;
-    int iy = G.ya.cell + (int)kf;
-    if (iy < 0 || iy >= ny) iy = ((iy % ny) + ny) % ny;
+    l.fy = t - kf;
+    l.iy = G.ya.cell + (int)kf;
+    if (l.iy < 0 || l.iy >= ny) l.iy = ((l.iy % ny) + ny) % ny;
+    return l;
+}
+
+// V and/or dV/dy from the window's coefficients cf[q][b] (x-row q, y level b)
+template <bool WANT_V, bool WANT_DY>
+__device__ __forceinline__ void edge32_combine(const LineGeom &G, const Edge32Loc &l,
+                                               const float (&cf)[4][4], float &v, float &dvdy) {
     float wx[4], wy[4], dwy[4];
-    bspline_wf(fx, wx);
-    if (WANT_V) bspline_wf(fy, wy);
-    if (WANT_DY) bspline_dwf(fy, dwy);
-    const int y2 = (iy + 1 == ny) ? 0 : iy + 1;
-    const int ys[4] = {(iy == 0) ? ny - 1 : iy - 1, iy, y2, (y2 + 1 == ny) ? 0 : y2 + 1};
-    const float *coef = static_cast<const float *>(G.nd.coef);
-    float cf[4][4];
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        float r[4];
-        row4(coef, G.nd.pitch, ix, ys[b], r);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) cf[q][b] = r[q];
-    }
+    bspline_wf(l.fx, wx);
+    if (WANT_V) bspline_wf(l.fy, wy);
+    if (WANT_DY) bspline_dwf(l.fy, dwy);
     float vv = 0.f, dd = 0.f;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -580,6 +599,39 @@ __device__ __forceinline__ void edge32(const LineGeom &G, int e, int ep, float h
     }
     v = vv;
     dvdy = dd * G.inv_dy;
+}
+
+// FP32 edge (e, ep) of a line whose heights are FP32 offsets from the
+// anchor: value and/or y-derivative of the potential at the edge midpoint.
+template <bool WANT_V, bool WANT_DY>
+__device__ __forceinline__ void edge32(const LineGeom &G, int e, int ep, float he, float hep,
+                                       float &v, float &dvdy) {
+    const Edge32Loc l = edge32_locate(G, e, ep, he, hep);
+    const int ny = G.nd.ny;
+    const int y2 = (l.iy + 1 == ny) ? 0 : l.iy + 1;
+    const int ys[4] = {(l.iy == 0) ? ny - 1 : l.iy - 1, l.iy, y2, (y2 + 1 == ny) ? 0 : y2 + 1};
+    float cf[4][4];
+    // one storage-type branch around the four row loads
+    if (G.nd.dtype == 2) {
+        const unsigned short *coef = static_cast<const unsigned short *>(G.nd.coef);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            float r[4];
+            row4_bf16(coef, G.nd.pitch, l.ix, ys[b], r);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) cf[q][b] = r[q];
+        }
+    } else {
+        const float *coef = static_cast<const float *>(G.nd.coef);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            float r[4];
+            row4(coef, G.nd.pitch, l.ix, ys[b], r);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) cf[q][b] = r[q];
+        }
+    }
+    edge32_combine<WANT_V, WANT_DY>(G, l, cf, v, dvdy);
 }
 
 }  // namespace qew_dev

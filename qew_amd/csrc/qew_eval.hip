@@ -306,6 +306,7 @@ void check_line(qew_ctx ctx, qew_noise noise, const qew_params *p, int n) {
     QEW_REQUIRE(p != nullptr, "null params");
     QEW_REQUIRE(n >= 1, "line length must be >= 1");
     QEW_REQUIRE(p->model == 0 || p->model == 1, "model must be QEW_LINEAR or QEW_ARCLENGTH");
+    require_wide_table(noise->desc);
 }
 
 double single_eval(qew_ctx ctx, qew_noise noise, const qew_params *p, int n, const double *h,
@@ -392,11 +393,12 @@ qew_status qew_batch_eval(qew_batch b, int dtype, const void *h, const double *a
         a.partial = b->d_partial.ptr;
         if (dtype == QEW_F32) {
             for (const auto &nd : b->noises)
-                QEW_REQUIRE(nd.kind == TABLE && nd.dtype == 1,
-                            "QEW_F32 evaluation needs FP32 coefficient tables");
+                QEW_REQUIRE(nd.kind == TABLE && (nd.dtype == QEW_F32 || nd.dtype == QEW_BF16),
+                            "QEW_F32 evaluation needs FP32 or BF16 coefficient tables");
             launch_eval_f32(ctx->stream, a, b->nlines);
         } else {
             QEW_REQUIRE(!anchor, "anchors apply to QEW_F32 line state only");
+            require_wide_tables(b->noises);
             NoiseDesc dtype_probe{};
             dtype_probe.dtype = b->noises.empty() ? 0 : b->noises[0].dtype;
             for (const auto &nd : b->noises)
@@ -428,6 +430,7 @@ qew_status qew_noise_sample(qew_ctx ctx, qew_noise noise, int m, const double *x
         QEW_REQUIRE(m >= 0, "negative point count");
         if (m == 0) return;
         QEW_REQUIRE(x && y && v && dvdx && dvdy, "null pointer");
+        require_wide_table(noise->desc);
         use_device(ctx);
         hipLaunchKernelGGL(sample_kernel, dim3((m + 255) / 256), dim3(256), 0, ctx->stream,
                            noise->desc, m, x, y, v, dvdx, dvdy);

@@ -141,6 +141,18 @@ inline void ctx_check(qew_ctx ctx) { QEW_REQUIRE(ctx != nullptr, "null context")
 
 inline void use_device(qew_ctx ctx) { QEW_HIP(hipSetDevice(ctx->device)); }
 
+// 2-byte (BF16) tables are read only by the FP32 batch evaluation and
+// dynamics; every entry point that reads tables as FP64 or FP32 values
+// rejects them before launching anything.
+inline void require_wide_table(const qew_dev::NoiseDesc &nd) {
+    QEW_REQUIRE(nd.kind != qew_dev::TABLE || nd.dtype != QEW_BF16,
+                "BF16 coefficient tables are read only by qew_batch_eval and qew_batch_langevin "
+                "with QEW_F32 line state");
+}
+inline void require_wide_tables(const std::vector<qew_dev::NoiseDesc> &v) {
+    for (const auto &nd : v) require_wide_table(nd);
+}
+
 // Makes sure the context's scratch covers the energy partials of `ntiles`
 // tiles and the results of `nlines` lines.
 void ctx_reserve_eval(qew_ctx ctx, int ntiles, int nlines);

@@ -299,6 +299,7 @@ qew_status qew_langevin_step(qew_ctx ctx, qew_noise noise, const qew_params *p, 
         ctx_check(ctx);
         QEW_REQUIRE(noise && p && h, "null pointer");
         QEW_REQUIRE(n >= 1, "line length must be >= 1");
+        require_wide_table(noise->desc);
         use_device(ctx);
         const LineDesc l = single_line(n, p);
         ctx->work.ensure(sizeof(double) * (size_t)n);
@@ -335,10 +336,11 @@ qew_status qew_batch_langevin(qew_batch b, int dtype, void *h, const double *anc
         const size_t es = dtype == QEW_F32 ? sizeof(float) : sizeof(double);
         if (dtype == QEW_F32) {
             for (const auto &nd : b->noises)
-                QEW_REQUIRE(nd.kind == TABLE && nd.dtype == 1,
-                            "QEW_F32 dynamics needs FP32 coefficient tables");
+                QEW_REQUIRE(nd.kind == TABLE && (nd.dtype == QEW_F32 || nd.dtype == QEW_BF16),
+                            "QEW_F32 dynamics needs FP32 or BF16 coefficient tables");
         } else {
             QEW_REQUIRE(!anchor, "anchors apply to QEW_F32 line state only");
+            require_wide_tables(b->noises);
             for (const auto &nd : b->noises)
                 QEW_REQUIRE(nd.kind != TABLE || nd.dtype == b->noises[0].dtype,
                             "all tables of a batch must share one storage type");

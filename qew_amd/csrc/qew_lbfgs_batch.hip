@@ -796,6 +796,7 @@ extern "C" qew_status qew_batch_lbfgs(qew_batch b, int dtype, const qew_lbfgs_pa
         QEW_REQUIRE(b && lp && h && results, "null pointer");
         QEW_REQUIRE(dtype == QEW_F64 || dtype == QEW_F32, "dtype must be QEW_F64 or QEW_F32");
         QEW_REQUIRE(lp->m >= 1 && lp->m <= kMaxM, "history length m must be in [1, 32]");
+        require_wide_tables(b->noises);
         const int tdt = b->noises[0].kind == TABLE ? b->noises[0].dtype : 0;
         for (const auto &nd : b->noises) {
             QEW_REQUIRE(nd.kind != TABLE || nd.dtype == tdt,

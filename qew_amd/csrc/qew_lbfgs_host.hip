@@ -191,6 +191,7 @@ qew_status qew_lbfgs_minimize(qew_ctx ctx, qew_noise noise, const qew_params *p,
         QEW_REQUIRE(noise && p && lp && h && result, "null pointer");
         QEW_REQUIRE(n >= 1, "line length must be >= 1");
         QEW_REQUIRE(lp->m >= 1, "history length m must be >= 1");
+        require_wide_table(noise->desc);
         use_device(ctx);
         const int m = lp->m;
         const LineDesc line = single_line(n, p);

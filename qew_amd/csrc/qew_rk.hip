@@ -663,6 +663,7 @@ qew_status qew_rk_block(qew_ctx ctx, qew_noise noise, const qew_params *p,
         if (n % 2 != 0)
             throw Error(QEW_ERR_INVALID_ARGUMENT,
                         "rk_block: line length must be even for the red-black sweep");
+        require_wide_table(noise->desc);
         use_device(ctx);
         LineDesc l = single_line(n, p);
         // workspace: [LineDesc][NoiseDesc][run_rk scratch]
@@ -692,6 +693,7 @@ qew_status qew_batch_rk_block(qew_batch b, const qew_rk_params *rp, double *h,
         for (const auto &nd : b->noises)
             QEW_REQUIRE(nd.kind != TABLE || nd.dtype == b->noises[0].dtype,
                         "all tables of a batch must share one storage type");
+        require_wide_tables(b->noises);
         qew_ctx ctx = b->ctx;
         use_device(ctx);
         b->d_scratch.ensure(rk_scratch_bytes(b->nlines, b->total, rk_tiles(b->max_n)));
@@ -714,6 +716,7 @@ qew_status qew_rk_site_velocity(qew_ctx ctx, qew_noise noise, const qew_params *
         QEW_REQUIRE(n >= 1 && m >= 0, "bad sizes");
         if (m == 0) return;
         QEW_REQUIRE(i && hi && hl && hr && v, "null pointer");
+        require_wide_table(noise->desc);
         use_device(ctx);
         hipLaunchKernelGGL(rk_velocity_kernel, dim3((m + 255) / 256), dim3(256), 0, ctx->stream,
                            noise->desc, n, p->physical_size / (double)n, p->line_tension,

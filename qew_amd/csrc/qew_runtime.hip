@@ -221,6 +221,35 @@ __global__ void convert_table(const S *src, D *dst, int64_t total) {
     if (k < total) dst[k] = (D)src[k];
 }
 
+// float -> bfloat16 bits, round to nearest even (NaN stays a quiet NaN)
+__device__ __forceinline__ unsigned short bf16_bits(float f) {
+    const unsigned u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x0040u);
+    return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+// Padded BF16 rows (pitch >= nx + 7): element j of row iy holds
+// coef((j - 1) mod nx, iy), rounded FP64 -> FP32 -> BF16. The source rows
+// have the standard halo (j = 0 .. nx + 2).
+template <typename S>
+__global__ void bf16_table(const S *src, int s_pitch, unsigned short *dst, int d_pitch, int nx,
+                           int ny) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (int64_t)d_pitch * ny) return;
+    const int j = (int)(k % d_pitch);
+    const int iy = (int)(k / d_pitch);
+    const int cell = ((j - 1) % nx + nx) % nx;
+    dst[(int64_t)iy * d_pitch + j] = bf16_bits((float)src[(int64_t)iy * s_pitch + cell + 1]);
+}
+
+__global__ void bf16_table_to_rowmajor(const unsigned short *table, double *dst, int nx, int ny,
+                                       int pitch) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (int64_t)nx * ny) return;
+    const int i = (int)(k / ny), j = (int)(k % ny);
+    dst[k] = (double)__uint_as_float((unsigned)table[(int64_t)j * pitch + i + 1] << 16);
+}
+
 // Host part of FilteredNoise (src/filtered_noise.cpp:35-51): the mt19937_64
 // stream, one phase per mode in (i, j) loop order, drawn before the mask
 // test; the in-band modes as (flat index, cos, sin).
@@ -554,14 +583,43 @@ qew_status qew_noise_generate_batch(qew_ctx ctx, int count, const int *nx, const
 qew_status qew_noise_convert(qew_noise noise, int dtype, qew_noise *out) {
     return guard([&] {
         QEW_REQUIRE(noise && out, "null pointer");
-        QEW_REQUIRE(dtype == QEW_F64 || dtype == QEW_F32, "dtype must be QEW_F64 or QEW_F32");
+        QEW_REQUIRE(dtype == QEW_F64 || dtype == QEW_F32 || dtype == QEW_BF16,
+                    "dtype must be QEW_F64, QEW_F32 or QEW_BF16");
+        QEW_REQUIRE(noise->desc.kind != TABLE || noise->desc.dtype != QEW_BF16,
+                    "BF16 tables are not converted further: convert the FP64 or FP32 table");
         qew_ctx ctx = noise->ctx;
         use_device(ctx);
         qew_noise nz = new_noise(ctx, noise->desc.kind, dtype, noise->desc.nx, noise->desc.ny,
                                  noise->desc.dx, noise->desc.dy);
         for (int k = 0; k < 4; ++k) nz->desc.p[k] = noise->desc.p[k];
         nz->field = noise->field;
-        if (noise->desc.kind == TABLE) {
+        if (noise->desc.kind == TABLE && dtype == QEW_BF16) {
+            const int nx = noise->desc.nx, ny = noise->desc.ny;
+            const int pitch = ((nx + 7 + 7) / 8) * 8;
+            hipError_t e = hipMalloc(&nz->table, sizeof(unsigned short) * (size_t)pitch * ny);
+            if (e != hipSuccess) {
+                delete nz;
+                QEW_HIP(e);
+            }
+            nz->desc.pitch = pitch;
+            nz->desc.coef = nz->table;
+            const dim3 g = grid1((int64_t)pitch * ny);
+            if (noise->desc.dtype == 1)
+                hipLaunchKernelGGL(bf16_table<float>, g, dim3(256), 0, ctx->stream,
+                                   static_cast<const float *>(noise->table), noise->desc.pitch,
+                                   static_cast<unsigned short *>(nz->table), pitch, nx, ny);
+            else
+                hipLaunchKernelGGL(bf16_table<double>, g, dim3(256), 0, ctx->stream,
+                                   static_cast<const double *>(noise->table), noise->desc.pitch,
+                                   static_cast<unsigned short *>(nz->table), pitch, nx, ny);
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) {
+                (void)hipFree(nz->table);
+                delete nz;
+                QEW_HIP(e);
+            }
+        } else if (noise->desc.kind == TABLE) {
             const int pitch = noise->desc.pitch;
             const int64_t total = (int64_t)pitch * noise->desc.ny;
             const size_t es = dtype == QEW_F32 ? sizeof(float) : sizeof(double);
@@ -650,7 +708,11 @@ qew_status qew_noise_coefficients(qew_noise noise, double *coef_host) {
         double *d = nullptr;
         QEW_HIP(hipMalloc(&d, sizeof(double) * (size_t)nx * ny));
         const dim3 g((nx + 31) / 32, (ny + 31) / 32), b(32, 8);
-        if (noise->desc.dtype == 1)
+        if (noise->desc.dtype == 2)
+            hipLaunchKernelGGL(bf16_table_to_rowmajor, grid1((int64_t)nx * ny), dim3(256), 0,
+                               ctx->stream, static_cast<const unsigned short *>(noise->table), d,
+                               nx, ny, noise->desc.pitch);
+        else if (noise->desc.dtype == 1)
             hipLaunchKernelGGL(table_to_rowmajor<float>, g, b, 0, ctx->stream,
                                static_cast<const float *>(noise->table), d, nx, ny,
                                noise->desc.pitch);
