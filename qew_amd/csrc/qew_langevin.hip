@@ -6,6 +6,7 @@
 // Philox4x32-10 computed in registers: no RNG state in memory, no pool locks,
 // and the same stream on any grid shape. diff == 0 skips the draw entirely.
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "qew_internal.hpp"
@@ -154,16 +155,18 @@ __global__ __launch_bounds__(kTile) void langevin32_kernel(StepArgs a) {
 // Several steps per launch. A workgroup holds an extended tile of kLvExt
 // consecutive sites (mod n) in LDS and advances all of them; after step s
 // the outer s + 1 sites at each end depend on sites outside the tile, so
-// after kLvSteps steps the central kLvOut sites are exact and are the tile's
+// after K steps the central kLvExt - 2K sites are exact and are the tile's
 // output. Each site's update is the single-step kernel's (same operands,
 // operation order and Philox key (seed + line, step, site)), so a launch
-// equals kLvSteps single steps bit for bit; the halo work is redundant
-// (kLvExt / kLvOut = 1.14) and saves kLvSteps - 1 launches and round trips
-// through memory. Short lines (n < kLvExt) alias sites within the tile: the
-// copies compute identical values and each site is written once.
-constexpr int kLvSteps = 16;
+// equals K single steps bit for bit, whatever K; the halo work is redundant
+// (kLvExt / (kLvExt - 2K)) and saves K - 1 launches and round trips through
+// memory. K (8, 16 or 32) is picked per batch by timing (lv_steps_per_launch).
+// Short lines (n < kLvExt) alias sites within the tile: the copies compute
+// identical values and each site is written once.
 constexpr int kLvExt = 256;
-constexpr int kLvOut = kLvExt - 2 * kLvSteps;
+constexpr int kLvMaxSteps = 32;
+template <int K>
+constexpr int lv_out() { return kLvExt - 2 * K; }
 
 struct MultiArgs {
     const LineDesc *lines;
@@ -172,19 +175,20 @@ struct MultiArgs {
     const void *h_in;
     void *h_out;
     const double *anchor;
-    double drift[kLvSteps], diff[kLvSteps];
-    int nsteps;     // steps of this launch, 1..kLvSteps
+    double drift[kLvMaxSteps], diff[kLvMaxSteps];
+    int nsteps;     // steps of this launch, 1..K
     uint64_t seed;  // line l uses seed + l
     uint64_t step0;
 };
 
-// extended position t of a tile whose first output site is `out0`
+// extended position t of a tile (halo K) whose first output site is `out0`
+template <int K>
 __device__ __forceinline__ int ext_site(int out0, int t, int n) {
-    int gi = (out0 - kLvSteps + t) % n;
+    int gi = (out0 - K + t) % n;
     return gi < 0 ? gi + n : gi;
 }
 
-template <typename C>
+template <typename C, int K>
 __global__ __launch_bounds__(kLvExt) void langevin64_multi(MultiArgs a) {
     __shared__ double s_h[kLvExt];
     __shared__ double s_dvdy[kLvExt];   // [t + 1]: edge between positions t, t + 1
@@ -198,7 +202,7 @@ __global__ __launch_bounds__(kLvExt) void langevin64_multi(MultiArgs a) {
     const double dx = L.L / (double)n;
     const double lt = L.lt, f = L.f;
     const bool arc = (L.model == 1);
-    const int gi = ext_site(tl.y, t, n);
+    const int gi = ext_site<K>(tl.y, t, n);
     const int gp = (gi + 1 == n) ? 0 : gi + 1;
     const double xc = (double)(gi + gp) * 0.5 * dx;
     s_h[t] = static_cast<const double *>(a.h_in)[L.off + gi];
@@ -231,11 +235,12 @@ __global__ __launch_bounds__(kLvExt) void langevin64_multi(MultiArgs a) {
         s_h[t] = hn;
         __syncthreads();
     }
-    const int o = t - kLvSteps;
-    if (o >= 0 && o < kLvOut && tl.y + o < n)
+    const int o = t - K;
+    if (o >= 0 && o < lv_out<K>() && tl.y + o < n)
         static_cast<double *>(a.h_out)[L.off + tl.y + o] = s_h[t];
 }
 
+template <int K>
 __global__ __launch_bounds__(kLvExt) void langevin32_multi(MultiArgs a) {
     __shared__ float s_h[kLvExt];
     __shared__ float s_dvdy[kLvExt];
@@ -251,7 +256,7 @@ __global__ __launch_bounds__(kLvExt) void langevin32_multi(MultiArgs a) {
     const float dxf = G.dxf, inv_dx = G.inv_dx;
     const float lt = (float)L.lt, f = (float)L.f;
     const bool arc = (L.model == 1);
-    const int gi = ext_site(tl.y, t, n);
+    const int gi = ext_site<K>(tl.y, t, n);
     const int gp = (gi + 1 == n) ? 0 : gi + 1;
     s_h[t] = static_cast<const float *>(a.h_in)[L.off + gi];
     __syncthreads();
@@ -281,9 +286,107 @@ __global__ __launch_bounds__(kLvExt) void langevin32_multi(MultiArgs a) {
         s_h[t] = hn;
         __syncthreads();
     }
-    const int o = t - kLvSteps;
-    if (o >= 0 && o < kLvOut && tl.y + o < n)
+    const int o = t - K;
+    if (o >= 0 && o < lv_out<K>() && tl.y + o < n)
         static_cast<float *>(a.h_out)[L.off + tl.y + o] = s_h[t];
+}
+
+// (line, first output site) per tile for K steps per launch
+void ensure_tiles(qew_batch b, int K) {
+    if (b->ltile_k == K) return;
+    const int out = kLvExt - 2 * K;
+    std::vector<int2> tiles;
+    for (int l = 0; l < b->nlines; ++l)
+        for (int o = 0; o < b->lines[l].n; o += out) tiles.push_back(make_int2(l, o));
+    b->d_ltile.ensure(sizeof(int2) * tiles.size());
+    QEW_HIP(hipMemcpyAsync(b->d_ltile.ptr, tiles.data(), sizeof(int2) * tiles.size(),
+                           hipMemcpyHostToDevice, b->ctx->stream));
+    QEW_HIP(hipStreamSynchronize(b->ctx->stream));
+    b->ltiles = (int)tiles.size();
+    b->ltile_k = K;
+}
+
+template <int K>
+void launch_multi(qew_batch b, int dtype, const MultiArgs &m) {
+    const dim3 grid(b->ltiles), block(kLvExt);
+    hipStream_t s = b->ctx->stream;
+    if (dtype == QEW_F32) hipLaunchKernelGGL(langevin32_multi<K>, grid, block, 0, s, m);
+    else if (b->noises[0].dtype == 1) hipLaunchKernelGGL((langevin64_multi<float, K>), grid, block, 0, s, m);
+    else hipLaunchKernelGGL((langevin64_multi<double, K>), grid, block, 0, s, m);
+    QEW_HIP(hipGetLastError());
+}
+
+// nsteps steps, K per launch, ping-ponging between bufs[0] (input) and
+// bufs[1]; returns the buffer that holds the result
+void *multi_steps(qew_batch b, int K, int dtype, void *bufs[2], const double *anchor, int nsteps,
+                  const double *drift, const double *diff, uint64_t seed, uint64_t step0) {
+    ensure_tiles(b, K);
+    MultiArgs m{};
+    m.lines = b->d_lines.as<LineDesc>();
+    m.noises = b->d_noises.as<NoiseDesc>();
+    m.tiles = b->d_ltile.as<int2>();
+    m.anchor = anchor;
+    m.seed = seed;
+    int launches = 0;
+    for (int k0 = 0; k0 < nsteps; k0 += K, ++launches) {
+        m.nsteps = std::min(K, nsteps - k0);
+        for (int s = 0; s < m.nsteps; ++s) {
+            m.drift[s] = drift[k0 + s];
+            m.diff[s] = diff[k0 + s];
+        }
+        m.step0 = step0 + (uint64_t)k0;
+        m.h_in = bufs[launches & 1];
+        m.h_out = bufs[(launches + 1) & 1];
+        if (K == 8) launch_multi<8>(b, dtype, m);
+        else if (K == 16) launch_multi<16>(b, dtype, m);
+        else launch_multi<kLvMaxSteps>(b, dtype, m);
+    }
+    return bufs[launches & 1];
+}
+
+// Steps per launch for this batch: QEW_LANGEVIN_STEPS_PER_LAUNCH (8, 16 or
+// 32) if set; else the value timed on the batch's first call of at least 96
+// steps (32 steps of each candidate on a copy of the line state, so the
+// caller's state is untouched; all candidates give identical results); 16
+// before that.
+int lv_steps_per_launch(qew_batch b, int dtype, const void *h, const double *anchor, int nsteps,
+                        const double *drift, const double *diff, uint64_t seed, uint64_t step0) {
+    if (const char *env = std::getenv("QEW_LANGEVIN_STEPS_PER_LAUNCH")) {
+        const int k = std::atoi(env);
+        QEW_REQUIRE(k == 8 || k == 16 || k == 32, "QEW_LANGEVIN_STEPS_PER_LAUNCH must be 8, 16 or 32");
+        return k;
+    }
+    if (b->lv_k) return b->lv_k;
+    if (nsteps < 3 * kLvMaxSteps) return 16;
+    hipStream_t s = b->ctx->stream;
+    const size_t es = dtype == QEW_F32 ? sizeof(float) : sizeof(double);
+    const size_t bytes = es * (size_t)b->total;
+    b->d_tune.ensure(2 * bytes);
+    char *t0 = b->d_tune.as<char>();
+    void *tb[2] = {t0, t0 + bytes};
+    hipEvent_t e0, e1;
+    QEW_HIP(hipEventCreate(&e0));
+    QEW_HIP(hipEventCreate(&e1));
+    int best = 16;
+    float best_ms = 0;
+    for (int K : {8, 16, kLvMaxSteps}) {
+        QEW_HIP(hipMemcpyAsync(tb[0], h, bytes, hipMemcpyDeviceToDevice, s));
+        ensure_tiles(b, K);
+        QEW_HIP(hipEventRecord(e0, s));
+        multi_steps(b, K, dtype, tb, anchor, kLvMaxSteps, drift, diff, seed, step0);
+        QEW_HIP(hipEventRecord(e1, s));
+        QEW_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        QEW_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (best_ms == 0 || ms < best_ms) {
+            best_ms = ms;
+            best = K;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->lv_k = best;
+    return best;
 }
 
 }  // namespace
@@ -347,47 +450,12 @@ qew_status qew_batch_langevin(qew_batch b, int dtype, void *h, const double *anc
         }
         b->d_scratch.ensure(es * (size_t)b->total);
         void *bufs[2] = {h, b->d_scratch.ptr};
-        const int table_dtype0 = b->noises[0].dtype;
         if (nsteps >= 2) {
-            // kLvSteps steps per launch on extended tiles
-            if (b->ltiles == 0) {
-                std::vector<int2> tiles;
-                for (int l = 0; l < b->nlines; ++l)
-                    for (int o = 0; o < b->lines[l].n; o += kLvOut) tiles.push_back(make_int2(l, o));
-                b->d_ltile.ensure(sizeof(int2) * tiles.size());
-                QEW_HIP(hipMemcpy(b->d_ltile.ptr, tiles.data(), sizeof(int2) * tiles.size(),
-                                  hipMemcpyHostToDevice));
-                b->ltiles = (int)tiles.size();
-            }
-            MultiArgs m{};
-            m.lines = b->d_lines.as<LineDesc>();
-            m.noises = b->d_noises.as<NoiseDesc>();
-            m.tiles = b->d_ltile.as<int2>();
-            m.anchor = anchor;
-            m.seed = seed;
-            int launches = 0;
-            for (int k0 = 0; k0 < nsteps; k0 += kLvSteps, ++launches) {
-                m.nsteps = std::min(kLvSteps, nsteps - k0);
-                for (int s = 0; s < m.nsteps; ++s) {
-                    m.drift[s] = drift[k0 + s];
-                    m.diff[s] = diff[k0 + s];
-                }
-                m.step0 = step0 + (uint64_t)k0;
-                m.h_in = bufs[launches & 1];
-                m.h_out = bufs[(launches + 1) & 1];
-                if (dtype == QEW_F32)
-                    hipLaunchKernelGGL(langevin32_multi, dim3(b->ltiles), dim3(kLvExt), 0, ctx->stream, m);
-                else if (table_dtype0 == 1)
-                    hipLaunchKernelGGL(langevin64_multi<float>, dim3(b->ltiles), dim3(kLvExt), 0,
-                                       ctx->stream, m);
-                else
-                    hipLaunchKernelGGL(langevin64_multi<double>, dim3(b->ltiles), dim3(kLvExt), 0,
-                                       ctx->stream, m);
-                QEW_HIP(hipGetLastError());
-            }
-            if (launches & 1)
-                QEW_HIP(hipMemcpyAsync(h, b->d_scratch.ptr, es * (size_t)b->total,
-                                       hipMemcpyDeviceToDevice, ctx->stream));
+            const int K = lv_steps_per_launch(b, dtype, h, anchor, nsteps, drift, diff, seed, step0);
+            void *res = multi_steps(b, K, dtype, bufs, anchor, nsteps, drift, diff, seed, step0);
+            if (res != h)
+                QEW_HIP(hipMemcpyAsync(h, res, es * (size_t)b->total, hipMemcpyDeviceToDevice,
+                                       ctx->stream));
             return;
         }
         StepArgs a{};

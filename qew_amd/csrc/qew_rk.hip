@@ -17,6 +17,7 @@
 // cooperatively (forward_zero_group). The line state, the advance sums, the
 // v <= 0 tests and the bisection are FP64; the tables may be FP64 or FP32.
 #include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -558,27 +559,30 @@ RkScratch rk_scratch(char *w, int nlines, int64_t total) {
 
 int rk_tiles(int max_n) { return std::max(1, ((max_n + 1) / 2 + kRkBlock - 1) / kRkBlock); }
 
-// Whether the LDS path serves this batch. QEW_RK_KERNEL=line|launch
-// overrides the choice (tuning and tests).
-bool use_line_kernel(qew_ctx ctx, int nlines, int max_n) {
+// Whether lines of up to max_n sites fit the LDS path on this device.
+bool line_kernel_fits(qew_ctx ctx, int max_n) {
     int lds_max = 0, lds_optin = 0;
     QEW_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
     QEW_HIP(hipDeviceGetAttribute(&lds_optin, hipDeviceAttributeSharedMemPerBlockOptin, ctx->device));
     lds_max = std::max(lds_max, lds_optin);
     const size_t need = (sizeof(double) + 1) * (size_t)max_n + 8192;
-    const bool fits = max_n <= kRkLdsMaxN && need <= (size_t)lds_max;
-    const char *env = std::getenv("QEW_RK_KERNEL");
-    if (env && std::strcmp(env, "launch") == 0) return false;
-    if (env && std::strcmp(env, "line") == 0) return fits;
-    // one workgroup per line leaves CUs idle for small batches, where the
-    // per-colour launches spread each line over many workgroups
-    return fits && nlines >= ctx->num_cus;
+    return max_n <= kRkLdsMaxN && need <= (size_t)lds_max;
 }
 
-// Runs the sweeps for `nlines` lines whose descriptors are on the device.
+// QEW_RK_KERNEL=line|launch forces a path (tests, tuning): 1 line, 2 launch,
+// 0 not set.
+int rk_env_path() {
+    const char *env = std::getenv("QEW_RK_KERNEL");
+    if (env && std::strcmp(env, "line") == 0) return 1;
+    if (env && std::strcmp(env, "launch") == 0) return 2;
+    return 0;
+}
+
+// Runs the sweeps for `nlines` lines whose descriptors are on the device, on
+// the LDS path (`line`) or with per-colour launches.
 void run_rk(qew_ctx ctx, const LineDesc *d_lines, const NoiseDesc *d_noises, int nlines, int max_n,
             int table_dtype, const qew_rk_params &rp, double *h, const RkScratch &w,
-            unsigned *running_host, std::vector<RkState> &out) {
+            unsigned *running_host, std::vector<RkState> &out, bool line) {
     // a march step of zero (or NaN) with a positive advance cap never ends;
     // the reference would spin forever, the device must not
     QEW_REQUIRE(rp.dstep > 0 || !(rp.max_advance > 0),
@@ -586,7 +590,7 @@ void run_rk(qew_ctx ctx, const LineDesc *d_lines, const NoiseDesc *d_noises, int
     hipStream_t s = ctx->stream;
     const int tiles = rk_tiles(max_n);
     RkArgs a{d_lines, d_noises, h, w.moved, w.partial, tiles, w.state, rp};
-    if (use_line_kernel(ctx, nlines, max_n)) {
+    if (line) {
         const size_t lds = (sizeof(double) + 1) * (size_t)max_n;
         QEW_HIP(hipMemsetAsync(w.state, 0, sizeof(RkState) * (size_t)nlines, s));
         if (table_dtype == 1) {
@@ -676,8 +680,10 @@ qew_status qew_rk_block(qew_ctx ctx, qew_noise noise, const qew_params *p,
                                ctx->stream));
         std::vector<RkState> out;
         const int dt = noise->desc.kind == TABLE ? noise->desc.dtype : 0;
+        // one line: a single workgroup leaves the device idle unless forced
+        const bool line = rk_env_path() == 1 && line_kernel_fits(ctx, n);
         run_rk(ctx, reinterpret_cast<LineDesc *>(w), reinterpret_cast<NoiseDesc *>(w + off_noise), 1, n,
-               dt, *rp, h, rk_scratch(w + off_rk, 1, n), ctx->pinned.as<unsigned>(), out);
+               dt, *rp, h, rk_scratch(w + off_rk, 1, n), ctx->pinned.as<unsigned>(), out, line);
         *result = to_result(out[0]);
     });
 }
@@ -700,9 +706,40 @@ qew_status qew_batch_rk_block(qew_batch b, const qew_rk_params *rp, double *h,
         ctx->pinned.ensure(sizeof(double) * 64);
         std::vector<RkState> out;
         const int dt = b->noises[0].kind == TABLE ? b->noises[0].dtype : 0;
-        run_rk(ctx, b->d_lines.as<LineDesc>(), b->d_noises.as<NoiseDesc>(), b->nlines, b->max_n, dt,
-               *rp, h, rk_scratch(b->d_scratch.as<char>(), b->nlines, b->total),
-               ctx->pinned.as<unsigned>(), out);
+        const RkScratch sc = rk_scratch(b->d_scratch.as<char>(), b->nlines, b->total);
+        auto run = [&](double *hh, const qew_rk_params &p, bool line) {
+            run_rk(ctx, b->d_lines.as<LineDesc>(), b->d_noises.as<NoiseDesc>(), b->nlines, b->max_n,
+                   dt, p, hh, sc, ctx->pinned.as<unsigned>(), out, line);
+        };
+        // Path: forced by QEW_RK_KERNEL, else per-colour launches when the
+        // lines do not fit in LDS, else the faster of the two timed on the
+        // batch's first call (16 sweeps each on a copy of the line state; both
+        // paths give identical results) and kept for later calls.
+        bool line = false;
+        const int env = rk_env_path();
+        if (env) {
+            line = env == 1 && line_kernel_fits(ctx, b->max_n);
+        } else if (line_kernel_fits(ctx, b->max_n)) {
+            if (b->rk_path == 0) {
+                qew_rk_params trial = *rp;
+                trial.max_sweeps = std::min(rp->max_sweeps, 16);
+                const size_t bytes = sizeof(double) * (size_t)b->total;
+                b->d_tune.ensure(bytes);
+                double *copy = b->d_tune.as<double>();
+                double t_path[2];
+                for (int k = 0; k < 2; ++k) {
+                    QEW_HIP(hipMemcpyAsync(copy, h, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+                    QEW_HIP(hipStreamSynchronize(ctx->stream));
+                    const auto t0 = std::chrono::steady_clock::now();
+                    run(copy, trial, k == 0);
+                    t_path[k] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0)
+                                    .count();
+                }
+                b->rk_path = t_path[0] < t_path[1] ? 1 : 2;
+            }
+            line = b->rk_path == 1;
+        }
+        run(h, *rp, line);
         for (int l = 0; l < b->nlines; ++l) results[l] = to_result(out[l]);
     });
 }

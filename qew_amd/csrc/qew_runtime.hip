@@ -807,6 +807,7 @@ qew_status qew_batch_destroy(qew_batch b) {
         b->d_partial.release();
         b->d_scratch.release();
         b->d_ltile.release();
+        b->d_tune.release();
         delete b;
     });
 }

@@ -169,11 +169,17 @@ def test_batch_langevin_matches_single(q):
     assert np.max(np.abs(host(hb)[n:] - host(hs))) > 0  # line 1 uses seed + 1
 
 
-@pytest.mark.parametrize("table", ["f64", "f32"])
-def test_batch_langevin_multistep_tiles_bitwise(q, table):
+@pytest.mark.parametrize("table,k,steps", [("f64", "16", 37), ("f32", "16", 37), ("f64", "8", 37),
+                                           ("f64", "32", 37), ("f64", None, 100)])
+def test_batch_langevin_multistep_tiles_bitwise(q, monkeypatch, table, k, steps):
     # several steps per launch on tiles with a halo (lines shorter and longer
-    # than a tile, both models, a partial last launch, steps without noise):
+    # than a tile, both models, a partial last launch, steps without noise),
+    # for each steps-per-launch value and for the timed choice (>= 96 steps):
     # every line equals single steps of the same stream bit for bit
+    if k is None:
+        monkeypatch.delenv("QEW_LANGEVIN_STEPS_PER_LAUNCH", raising=False)
+    else:
+        monkeypatch.setenv("QEW_LANGEVIN_STEPS_PER_LAUNCH", k)
     field, coef = O.filtered_noise(256, 64, 2.56, 1.0, 0.5, 0.2, 0.2, 6)
     gn = q.FilteredNoise.from_coefficients(coef, 0.01, 1.0 / 64)
     if table == "f32":
@@ -183,7 +189,6 @@ def test_batch_langevin_multistep_tiles_bitwise(q, table):
     b = q.Batch(lengths, [gn] * len(lengths), params)
     rng = np.random.default_rng(12)
     h0 = [rng.uniform(0.3, 0.7, n) for n in lengths]
-    steps = 37
     drift = np.full(steps, 0.002)
     diff = np.where(np.arange(steps) % 5 == 3, 0.0, 0.01)
     hb = dev(np.concatenate(h0))
@@ -302,9 +307,10 @@ def test_rk_site_velocity_is_minus_gradient(q):
         assert np.max(np.abs(host(v) + g)) < 1e-12
 
 
-@pytest.mark.parametrize("path", ["line", "launch"])
+@pytest.mark.parametrize("path", ["line", "launch", None])
 def test_batch_rk_matches_single_lines(q, monkeypatch, path):
-    # the batch runs on the chosen path, the single lines on the default one
+    # the batch runs on the chosen path (None: the path timed on the batch's
+    # first call), the single lines on the default one
     field, coef = O.filtered_noise(64, 64, 3.2, 1.0, 0.5, 0.2, 0.2, 7)
     gn = q.FilteredNoise.from_coefficients(coef, 0.05, 1.0 / 64)
     rp = q.RkParams(0.5 / 64, 1.0, 1e-11, 1e-9, 3.0)
@@ -313,9 +319,12 @@ def test_batch_rk_matches_single_lines(q, monkeypatch, path):
     params = [q.Params(3.2, 1.0, f, q.Model.Linear) for f in forces]
     b = q.Batch(lengths, [gn] * len(lengths), params)
     hb = dev(np.full(sum(lengths), 0.5))
-    monkeypatch.setenv("QEW_RK_KERNEL", path)
+    if path is None:
+        monkeypatch.delenv("QEW_RK_KERNEL", raising=False)
+    else:
+        monkeypatch.setenv("QEW_RK_KERNEL", path)
     res = b.rk_block(hb, rp)
-    monkeypatch.delenv("QEW_RK_KERNEL")
+    monkeypatch.delenv("QEW_RK_KERNEL", raising=False)
     off = np.concatenate([[0], np.cumsum(lengths)])
     for k, f in enumerate(forces):
         h = dev(np.full(lengths[k], 0.5))
