@@ -15,7 +15,13 @@
 namespace qew_dev {
 
 constexpr int kWave = 64;
-constexpr int kTile = 256;  // sites per workgroup tile in the eval kernels
+// Eval and one-step Langevin tiles: kTile threads per workgroup cover
+// kTileSites = kTile - 1 consecutive sites. Thread t evaluates the edge to
+// the right of site first - 1 + t, so thread 0's edge is the tile's left halo
+// (the left edge of its first site) and is gathered in the same round as the
+// others: no wave of the tile waits on a second, dependent gather.
+constexpr int kTile = 256;
+constexpr int kTileSites = kTile - 1;
 
 enum Kind { TABLE = 0, SINCOS = 1, CONST = 2, ZERO = 3, HARMONIC = 4, WASHBOARD = 5 };
 

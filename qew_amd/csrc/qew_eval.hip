@@ -1,8 +1,9 @@
 // Fused gradient + energy of ragged batches of lines (the qew_model.h hot
-// path: src/qew_model.h:53-201). One workgroup = one tile of 256 consecutive
-// sites of one line; each thread owns one site and its right edge, so every
-// edge's 4x4 gather runs exactly once. The left-neighbour edge comes through
-// LDS; thread 0 also evaluates the tile's left halo edge. Each tile writes
+// path: src/qew_model.h:53-201). One workgroup of 256 threads = one tile of
+// 255 consecutive sites of one line: thread t >= 1 owns site first - 1 + t
+// and its right edge, thread 0 evaluates the edge left of the first site (the
+// halo), so every wave makes one round of 4x4 gathers. The left-neighbour
+// edge of each site comes through LDS. Each tile writes
 // its energy partial; a second, small launch sums every line's partials in a
 // fixed order (deterministic). The kernel boundary orders the two: no
 // device-scope fence or atomic is needed per tile, which matters on a
@@ -66,7 +67,7 @@ __global__ __launch_bounds__(kTile) void eval64_kernel(qew_host::EvalArgs a, Lin
         N = a.noises[L.noise];
     }
     const int n = L.n;
-    const int start = (tile - L.tile0) * kTile;
+    const int first = (tile - L.tile0) * kTileSites;
     const double *h = static_cast<const double *>(a.h) + L.off;
     const double *dd = TRIAL ? static_cast<const double *>(a.d) + L.off : nullptr;
     const double alpha = a.alpha;
@@ -74,10 +75,14 @@ __global__ __launch_bounds__(kTile) void eval64_kernel(qew_host::EvalArgs a, Lin
     const double dx = L.L / (double)n;
     const double lt = L.lt, f = L.f;
     const bool arc = (L.model == 1);
-    const int i = start + tid;
-    const bool active = i < n;
+    // this thread's edge is (i, i + 1 mod n), i = first - 1 + tid; it owns
+    // site i for tid >= 1, and thread 0's edge is the halo the gradient needs
+    const int gl = first - 1 + tid;
+    const bool owns = tid >= 1 && gl < n;
+    const bool has_edge = owns || (GRAD && tid == 0);
+    const int i = gl < 0 ? n - 1 : gl;
     double e = 0, hi = 0, hip = 0;
-    if (active) {
+    if (has_edge) {
         const int ip = (i + 1 == n) ? 0 : i + 1;
         hi = at(i);
         hip = at(ip);
@@ -88,43 +93,30 @@ __global__ __launch_bounds__(kTile) void eval64_kernel(qew_host::EvalArgs a, Lin
         else if (ENERGY) v = sample_v64<C>(N, xc, hc);
         else dvdy = sample_dy64<C>(N, xc, hc);
         const double dh = (hip - hi) / dx;
-        if (ENERGY) {
+        if (ENERGY && owns) {
             double le;
             if (!arc) le = lt * dh * dh * 0.5;
             else le = lt * (sqrt(1.0 + dh * dh) - 1.0);
             e = dx * (le + v - f * hi);
         }
         if (GRAD) {
-            s_dvdy[tid + 1] = dvdy;
-            s_h[tid + 1] = hi;
-            if (arc) s_slope[tid + 1] = dh / sqrt(1.0 + dh * dh);
-            if (tid == 0) {
-                // left halo: edge (im, i)
-                const int im = (i == 0) ? n - 1 : i - 1;
-                const double hm = at(im);
-                const double xcm = (double)(im + i) * 0.5 * dx;
-                const double hcm = (hi + hm) * 0.5;
-                s_dvdy[0] = sample_dy64<C>(N, xcm, hcm);
-                s_h[0] = hm;
-                if (arc) {
-                    const double dhm = (hi - hm) / dx;
-                    s_slope[0] = dhm / sqrt(1.0 + dhm * dhm);
-                }
-            }
+            s_dvdy[tid] = dvdy;
+            s_h[tid] = hi;
+            if (arc) s_slope[tid] = dh / sqrt(1.0 + dh * dh);
         }
     }
     if (GRAD) {
         __syncthreads();
-        if (active) {
+        if (owns) {
             double line_force;
             if (!arc) {
-                const double d2h = s_h[tid] + hip - 2.0 * hi;
+                const double d2h = s_h[tid - 1] + hip - 2.0 * hi;
                 line_force = -lt * d2h / (dx * dx);
             } else {
-                line_force = lt * (s_slope[tid] - s_slope[tid + 1]) / dx;
+                line_force = lt * (s_slope[tid - 1] - s_slope[tid]) / dx;
             }
             static_cast<double *>(a.grad)[L.off + i] =
-                dx * (line_force + 0.5 * (s_dvdy[tid] + s_dvdy[tid + 1]) - f);
+                dx * (line_force + 0.5 * (s_dvdy[tid - 1] + s_dvdy[tid]) - f);
         }
     }
     if (ENERGY) {
@@ -146,7 +138,7 @@ __global__ __launch_bounds__(kTile) void eval32_kernel(qew_host::EvalArgs a) {
     const LineDesc L = a.lines[line];
     const NoiseDesc N = a.noises[L.noise];
     const int n = L.n;
-    const int start = (tile - L.tile0) * kTile;
+    const int first = (tile - L.tile0) * kTileSites;
     const float *h = static_cast<const float *>(a.h) + L.off;
     const double anchor = a.anchor ? a.anchor[line] : 0.0;
     const LineGeom G = make_geom(L, N, anchor);
@@ -157,10 +149,13 @@ __global__ __launch_bounds__(kTile) void eval32_kernel(qew_host::EvalArgs a) {
         edge32<true, true>(G, e, ep, he, hep, v, dvdy);
     };
 
-    const int i = start + tid;
-    const bool active = i < n;
+    // edge (i, i + 1 mod n), i = first - 1 + tid; site i is this thread's
+    // for tid >= 1, thread 0's edge is the tile's left halo
+    const int gl = first - 1 + tid;
+    const bool owns = tid >= 1 && gl < n;
+    const int i = gl < 0 ? n - 1 : gl;
     float e = 0.f, hi = 0.f, hip = 0.f;
-    if (active) {
+    if (owns || tid == 0) {
         const int ip = (i + 1 == n) ? 0 : i + 1;
         hi = h[i];
         hip = h[ip];
@@ -170,28 +165,18 @@ __global__ __launch_bounds__(kTile) void eval32_kernel(qew_host::EvalArgs a) {
         const float s2 = dh * dh;
         const float root = sqrtf(1.0f + s2);
         const float le = arc ? lt * (s2 / (root + 1.0f)) : 0.5f * lt * s2;
-        e = dxf * (le + v - f * hi);
-        s_dvdy[tid + 1] = dvdy;
-        s_h[tid + 1] = hi;
-        s_slope[tid + 1] = dh / root;
-        if (tid == 0) {
-            const int im = (i == 0) ? n - 1 : i - 1;
-            const float hm = h[im];
-            float vm, dvm;
-            edge(im, i, hm, hi, vm, dvm);
-            s_dvdy[0] = dvm;
-            s_h[0] = hm;
-            const float dhm = (hi - hm) * inv_dx;
-            s_slope[0] = dhm / sqrtf(1.0f + dhm * dhm);
-        }
+        if (owns) e = dxf * (le + v - f * hi);
+        s_dvdy[tid] = dvdy;
+        s_h[tid] = hi;
+        s_slope[tid] = dh / root;
     }
     __syncthreads();
-    if (active && a.grad) {
+    if (owns && a.grad) {
         float lf;
-        if (!arc) lf = -lt * ((s_h[tid] + hip) - 2.0f * hi) * (inv_dx * inv_dx);
-        else lf = lt * (s_slope[tid] - s_slope[tid + 1]) * inv_dx;
+        if (!arc) lf = -lt * ((s_h[tid - 1] + hip) - 2.0f * hi) * (inv_dx * inv_dx);
+        else lf = lt * (s_slope[tid - 1] - s_slope[tid]) * inv_dx;
         static_cast<float *>(a.grad)[L.off + i] =
-            dxf * (lf + 0.5f * (s_dvdy[tid] + s_dvdy[tid + 1]) - f);
+            dxf * (lf + 0.5f * (s_dvdy[tid - 1] + s_dvdy[tid]) - f);
     }
     if (a.energy) {
         const KSum k = block_ksum(KSum{e, 0.f}, red);
@@ -287,7 +272,7 @@ LineDesc single_line(int n, const qew_params *p) {
     l.noise = 0;
     l.model = p->model;
     l.tile0 = 0;
-    l.ntiles = (n + kTile - 1) / kTile;
+    l.ntiles = (n + kTileSites - 1) / kTileSites;
     l.L = p->physical_size;
     l.lt = p->line_tension;
     l.f = p->driving_force;

@@ -49,46 +49,39 @@ __global__ __launch_bounds__(kTile) void langevin64_kernel(StepArgs a, LineDesc 
         N = a.noises[L.noise];
     }
     const int n = L.n;
-    const int i = (tile - L.tile0) * kTile + tid;
-    const bool active = i < n;
+    // edge (i, i + 1 mod n), i = first - 1 + tid: site i for tid >= 1, the
+    // tile's left halo for tid 0
+    const int gl = (tile - L.tile0) * kTileSites - 1 + tid;
+    const bool owns = tid >= 1 && gl < n;
+    const int i = gl < 0 ? n - 1 : gl;
     const double *h = static_cast<const double *>(a.h_in) + L.off;
     const double dx = L.L / (double)n;
     const double lt = L.lt, f = L.f;
     const bool arc = (L.model == 1);
     double hi = 0, hip = 0;
-    if (active) {
+    if (owns || tid == 0) {
         const int ip = (i + 1 == n) ? 0 : i + 1;
         hi = h[i];
         hip = h[ip];
         const double xc = (double)(i + ip) * 0.5 * dx;
         const double hc = (hip + hi) * 0.5;
-        s_dvdy[tid + 1] = sample_dy64<C>(N, xc, hc);
-        s_h[tid + 1] = hi;
+        s_dvdy[tid] = sample_dy64<C>(N, xc, hc);
+        s_h[tid] = hi;
         if (arc) {
             const double dh = (hip - hi) / dx;
-            s_slope[tid + 1] = dh / sqrt(1.0 + dh * dh);
-        }
-        if (tid == 0) {
-            const int im = (i == 0) ? n - 1 : i - 1;
-            const double hm = h[im];
-            s_dvdy[0] = sample_dy64<C>(N, (double)(im + i) * 0.5 * dx, (hi + hm) * 0.5);
-            s_h[0] = hm;
-            if (arc) {
-                const double dhm = (hi - hm) / dx;
-                s_slope[0] = dhm / sqrt(1.0 + dhm * dhm);
-            }
+            s_slope[tid] = dh / sqrt(1.0 + dh * dh);
         }
     }
     __syncthreads();
-    if (!active) return;
+    if (!owns) return;
     double line_force;
     if (!arc) {
-        const double d2h = s_h[tid] + hip - 2.0 * hi;
+        const double d2h = s_h[tid - 1] + hip - 2.0 * hi;
         line_force = -lt * d2h / (dx * dx);
     } else {
-        line_force = lt * (s_slope[tid] - s_slope[tid + 1]) / dx;
+        line_force = lt * (s_slope[tid - 1] - s_slope[tid]) / dx;
     }
-    const double g = dx * (line_force + 0.5 * (s_dvdy[tid] + s_dvdy[tid + 1]) - f);
+    const double g = dx * (line_force + 0.5 * (s_dvdy[tid - 1] + s_dvdy[tid]) - f);
     double hn;
     if (a.diff > 0)
         hn = hi + (-a.drift * g + a.diff * philox_normal(a.seed + (uint64_t)line, a.step, (uint64_t)i));
@@ -108,8 +101,9 @@ __global__ __launch_bounds__(kTile) void langevin32_kernel(StepArgs a) {
     const LineDesc L = a.lines[line];
     const NoiseDesc N = a.noises[L.noise];
     const int n = L.n;
-    const int i = (tile - L.tile0) * kTile + tid;
-    const bool active = i < n;
+    const int gl = (tile - L.tile0) * kTileSites - 1 + tid;
+    const bool owns = tid >= 1 && gl < n;
+    const int i = gl < 0 ? n - 1 : gl;
     const float *h = static_cast<const float *>(a.h_in) + L.off;
     const double anchor = a.anchor ? a.anchor[line] : 0.0;
     const LineGeom G = make_geom(L, N, anchor);
@@ -123,29 +117,21 @@ __global__ __launch_bounds__(kTile) void langevin32_kernel(StepArgs a) {
     };
 
     float hi = 0.f, hip = 0.f;
-    if (active) {
+    if (owns || tid == 0) {
         const int ip = (i + 1 == n) ? 0 : i + 1;
         hi = h[i];
         hip = h[ip];
-        s_dvdy[tid + 1] = edge_dy(i, ip, hi, hip);
-        s_h[tid + 1] = hi;
+        s_dvdy[tid] = edge_dy(i, ip, hi, hip);
+        s_h[tid] = hi;
         const float dh = (hip - hi) * inv_dx;
-        s_slope[tid + 1] = dh * rsqrtf(1.0f + dh * dh);
-        if (tid == 0) {
-            const int im = (i == 0) ? n - 1 : i - 1;
-            const float hm = h[im];
-            s_dvdy[0] = edge_dy(im, i, hm, hi);
-            s_h[0] = hm;
-            const float dhm = (hi - hm) * inv_dx;
-            s_slope[0] = dhm * rsqrtf(1.0f + dhm * dhm);
-        }
+        s_slope[tid] = dh * rsqrtf(1.0f + dh * dh);
     }
     __syncthreads();
-    if (!active) return;
+    if (!owns) return;
     float lf;
-    if (!arc) lf = -lt * ((s_h[tid] + hip) - 2.0f * hi) * (inv_dx * inv_dx);
-    else lf = lt * (s_slope[tid] - s_slope[tid + 1]) * inv_dx;
-    const float g = dxf * (lf + 0.5f * (s_dvdy[tid] + s_dvdy[tid + 1]) - f);
+    if (!arc) lf = -lt * ((s_h[tid - 1] + hip) - 2.0f * hi) * (inv_dx * inv_dx);
+    else lf = lt * (s_slope[tid - 1] - s_slope[tid]) * inv_dx;
+    const float g = dxf * (lf + 0.5f * (s_dvdy[tid - 1] + s_dvdy[tid]) - f);
     float hn = hi - (float)a.drift * g;
     if (a.diff > 0)
         hn += (float)(a.diff * philox_normal(a.seed + (uint64_t)line, a.step, (uint64_t)i));
