@@ -509,8 +509,11 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
     }
 }
 
-__device__ __forceinline__ double philox_normal(uint64_t seed, uint64_t step, uint64_t site) {
-    uint32_t c[4] = {(uint32_t)(site >> 1), (uint32_t)step, (uint32_t)(step >> 32), 0u};
+// Both normals of the Philox block of sites 2 pair and 2 pair + 1: zc (cos,
+// the even site) and zs (sin, the odd site), from one sincos.
+__device__ __forceinline__ void philox_normal_pair(uint64_t seed, uint64_t step, uint64_t pair,
+                                                   double &zc, double &zs) {
+    uint32_t c[4] = {(uint32_t)pair, (uint32_t)step, (uint32_t)(step >> 32), 0u};
     philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
     const uint64_t k1 = ((((uint64_t)c[0]) << 32) | c[1]) >> 11;
     const uint64_t k2 = ((((uint64_t)c[2]) << 32) | c[3]) >> 11;
@@ -518,7 +521,16 @@ __device__ __forceinline__ double philox_normal(uint64_t seed, uint64_t step, ui
     const double u2 = (double)k2 * 0x1.0p-53;
     const double rad = sqrt(-2.0 * log(u1));
     const double th = 6.283185307179586 * u2;
-    return (site & 1) ? rad * sin(th) : rad * cos(th);
+    double sn, cs;
+    sincos(th, &sn, &cs);
+    zc = rad * cs;
+    zs = rad * sn;
+}
+
+__device__ __forceinline__ double philox_normal(uint64_t seed, uint64_t step, uint64_t site) {
+    double zc, zs;
+    philox_normal_pair(seed, step, site >> 1, zc, zs);
+    return (site & 1) ? zs : zc;
 }
 
 }  // namespace qew_dev

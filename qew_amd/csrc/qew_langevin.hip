@@ -174,11 +174,55 @@ __device__ __forceinline__ int ext_site(int out0, int t, int n) {
     return gi < 0 ? gi + n : gi;
 }
 
+// Kicks diff[s] * N(0,1) of steps [s0, s1) of this launch for the tile's
+// sites, passed to store(s, t, kick) for every step with diff[s] > 0. Sites
+// 2p and 2p + 1 share Philox block (key, step, p): when threads t and t ^ 1
+// hold the two sites of one pair they split the steps (even thread: even
+// steps, odd thread: odd steps) and each block feeds both sites (cos to the
+// even site, sin to the odd one), so a thread draws half as many blocks as
+// it has steps, and the draws of different steps are independent. A thread
+// whose partner holds another pair (the line wraps inside the tile) draws
+// its own site's blocks. s0 is even.
+template <class Store>
+__device__ __forceinline__ void draw_kicks(const MultiArgs &a, uint64_t key, int t, int gi, int gj,
+                                           int s0, int s1, Store store) {
+    if (gi != gj && (gi >> 1) == (gj >> 1)) {
+        const int te = (gi & 1) ? (t ^ 1) : t;  // thread holding the even site
+        for (int s2 = s0; s2 < s1; s2 += 2) {
+            // s2 is even and below s1 <= kLvMaxSteps, so s2 + 1 < kLvMaxSteps
+            const int s = s2 + (t & 1);
+            const double df = (t & 1) ? a.diff[s2 + 1] : a.diff[s2];
+            if (s < s1 && df > 0) {
+                double zc, zs;
+                philox_normal_pair(key, a.step0 + (uint64_t)s, (uint64_t)(gi >> 1), zc, zs);
+                store(s, te, df * zc);
+                store(s, te ^ 1, df * zs);
+            }
+        }
+    } else {
+        for (int s = s0; s < s1; ++s) {
+            const double df = a.diff[s];
+            if (df > 0) store(s, t, df * philox_normal(key, a.step0 + (uint64_t)s, (uint64_t)gi));
+        }
+    }
+}
+
+// Steps whose kicks one draw pass keeps in LDS: all K for FP32 state (4 B
+// per kick), at most 16 for FP64 state (8 B), so the kick table stays within
+// 32 KB per workgroup.
+template <int K, int BYTES>
+constexpr int lv_kick_chunk() { return (BYTES == 4 || K < 16) ? K : 16; }
+
+// Step loop shared by both precisions (two barriers per step): kicks are
+// drawn at the start of each chunk of KC steps, heights ping-pong between
+// two LDS rows so a step's writes never meet the reads of the same step.
 template <typename C, int K>
 __global__ __launch_bounds__(kLvExt) void langevin64_multi(MultiArgs a) {
-    __shared__ double s_h[kLvExt];
+    constexpr int KC = lv_kick_chunk<K, 8>();
+    __shared__ double s_h[2][kLvExt];
     __shared__ double s_dvdy[kLvExt];   // [t + 1]: edge between positions t, t + 1
     __shared__ double s_slope[kLvExt];
+    __shared__ double s_kick[KC][kLvExt];
     const int t = threadIdx.x;
     const int2 tl = a.tiles[blockIdx.x];
     const int line = tl.x;
@@ -189,14 +233,22 @@ __global__ __launch_bounds__(kLvExt) void langevin64_multi(MultiArgs a) {
     const double lt = L.lt, f = L.f;
     const bool arc = (L.model == 1);
     const int gi = ext_site<K>(tl.y, t, n);
+    const int gj = ext_site<K>(tl.y, t ^ 1, n);
     const int gp = (gi + 1 == n) ? 0 : gi + 1;
     const double xc = (double)(gi + gp) * 0.5 * dx;
-    s_h[t] = static_cast<const double *>(a.h_in)[L.off + gi];
-    __syncthreads();
+    const uint64_t key = a.seed + (uint64_t)line;
+    auto store = [&](int s, int u, double k) { s_kick[s % KC][u] = k; };
+    s_h[0][t] = static_cast<const double *>(a.h_in)[L.off + gi];
     for (int s = 0; s < a.nsteps; ++s) {
-        const double hi = s_h[t];
+        const int cur = s & 1;
+        if (s % KC == 0) {
+            if (s > 0) __syncthreads();  // the previous chunk's kicks have been read
+            draw_kicks(a, key, t, gi, gj, s, min(s + KC, a.nsteps), store);
+        }
+        __syncthreads();
+        const double hi = s_h[cur][t];
         if (t + 1 < kLvExt) {
-            const double hp = s_h[t + 1];
+            const double hp = s_h[cur][t + 1];
             s_dvdy[t + 1] = sample_dy64<C>(N, xc, (hp + hi) * 0.5);
             if (arc) {
                 const double dh = (hp - hi) / dx;
@@ -207,30 +259,28 @@ __global__ __launch_bounds__(kLvExt) void langevin64_multi(MultiArgs a) {
         double hn = hi;
         if (t >= 1 && t + 1 < kLvExt) {
             double line_force;
-            if (!arc) line_force = -lt * (s_h[t - 1] + s_h[t + 1] - 2.0 * hi) / (dx * dx);
+            if (!arc) line_force = -lt * (s_h[cur][t - 1] + s_h[cur][t + 1] - 2.0 * hi) / (dx * dx);
             else line_force = lt * (s_slope[t] - s_slope[t + 1]) / dx;
             const double g = dx * (line_force + 0.5 * (s_dvdy[t] + s_dvdy[t + 1]) - f);
-            const double dr = a.drift[s], df = a.diff[s];
-            if (df > 0)
-                hn = hi + (-dr * g + df * philox_normal(a.seed + (uint64_t)line, a.step0 + (uint64_t)s,
-                                                        (uint64_t)gi));
-            else
-                hn = hi + -dr * g;
+            const double dr = a.drift[s];
+            if (a.diff[s] > 0) hn = hi + (-dr * g + s_kick[s % KC][t]);
+            else hn = hi + -dr * g;
         }
-        __syncthreads();
-        s_h[t] = hn;
-        __syncthreads();
+        s_h[cur ^ 1][t] = hn;
     }
+    // each thread reads back the row it wrote last
     const int o = t - K;
     if (o >= 0 && o < lv_out<K>() && tl.y + o < n)
-        static_cast<double *>(a.h_out)[L.off + tl.y + o] = s_h[t];
+        static_cast<double *>(a.h_out)[L.off + tl.y + o] = s_h[a.nsteps & 1][t];
 }
 
 template <int K>
 __global__ __launch_bounds__(kLvExt) void langevin32_multi(MultiArgs a) {
-    __shared__ float s_h[kLvExt];
+    constexpr int KC = lv_kick_chunk<K, 4>();
+    __shared__ float s_h[2][kLvExt];
     __shared__ float s_dvdy[kLvExt];
     __shared__ float s_slope[kLvExt];
+    __shared__ float s_kick[KC][kLvExt];
     const int t = threadIdx.x;
     const int2 tl = a.tiles[blockIdx.x];
     const int line = tl.x;
@@ -243,13 +293,21 @@ __global__ __launch_bounds__(kLvExt) void langevin32_multi(MultiArgs a) {
     const float lt = (float)L.lt, f = (float)L.f;
     const bool arc = (L.model == 1);
     const int gi = ext_site<K>(tl.y, t, n);
+    const int gj = ext_site<K>(tl.y, t ^ 1, n);
     const int gp = (gi + 1 == n) ? 0 : gi + 1;
-    s_h[t] = static_cast<const float *>(a.h_in)[L.off + gi];
-    __syncthreads();
+    const uint64_t key = a.seed + (uint64_t)line;
+    auto store = [&](int s, int u, double k) { s_kick[s % KC][u] = (float)k; };
+    s_h[0][t] = static_cast<const float *>(a.h_in)[L.off + gi];
     for (int s = 0; s < a.nsteps; ++s) {
-        const float hi = s_h[t];
+        const int cur = s & 1;
+        if (s % KC == 0) {
+            if (s > 0) __syncthreads();  // the previous chunk's kicks have been read
+            draw_kicks(a, key, t, gi, gj, s, min(s + KC, a.nsteps), store);
+        }
+        __syncthreads();
+        const float hi = s_h[cur][t];
         if (t + 1 < kLvExt) {
-            const float hp = s_h[t + 1];
+            const float hp = s_h[cur][t + 1];
             float v, dy;
             edge32<false, true>(G, gi, gp, hi, hp, v, dy);
             s_dvdy[t + 1] = dy;
@@ -260,21 +318,17 @@ __global__ __launch_bounds__(kLvExt) void langevin32_multi(MultiArgs a) {
         float hn = hi;
         if (t >= 1 && t + 1 < kLvExt) {
             float lf;
-            if (!arc) lf = -lt * ((s_h[t - 1] + s_h[t + 1]) - 2.0f * hi) * (inv_dx * inv_dx);
+            if (!arc) lf = -lt * ((s_h[cur][t - 1] + s_h[cur][t + 1]) - 2.0f * hi) * (inv_dx * inv_dx);
             else lf = lt * (s_slope[t] - s_slope[t + 1]) * inv_dx;
             const float g = dxf * (lf + 0.5f * (s_dvdy[t] + s_dvdy[t + 1]) - f);
             hn = hi - (float)a.drift[s] * g;
-            if (a.diff[s] > 0)
-                hn += (float)(a.diff[s] * philox_normal(a.seed + (uint64_t)line,
-                                                        a.step0 + (uint64_t)s, (uint64_t)gi));
+            if (a.diff[s] > 0) hn += s_kick[s % KC][t];
         }
-        __syncthreads();
-        s_h[t] = hn;
-        __syncthreads();
+        s_h[cur ^ 1][t] = hn;
     }
     const int o = t - K;
     if (o >= 0 && o < lv_out<K>() && tl.y + o < n)
-        static_cast<float *>(a.h_out)[L.off + tl.y + o] = s_h[t];
+        static_cast<float *>(a.h_out)[L.off + tl.y + o] = s_h[a.nsteps & 1][t];
 }
 
 // (line, first output site) per tile for K steps per launch

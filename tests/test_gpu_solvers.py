@@ -204,14 +204,16 @@ def test_batch_langevin_multistep_tiles_bitwise(q, monkeypatch, table, k, steps)
 
 
 def test_batch_langevin_fp32_multistep_matches_single_steps(q):
-    # FP32 line state: a multi-step call equals the same steps issued one per call
+    # FP32 line state: a multi-step call equals the same steps issued one per
+    # call (two lines over several tiles, one odd line shorter than a tile)
     n, ny = 700, 256
     field, coef = O.filtered_noise(n, ny, float(n), float(ny), 1.0, 4.0, 4.0, 5)
     g32 = q.FilteredNoise.from_coefficients(coef, 1.0, 1.0, dtype=q.F32)
-    p = [q.Params(float(n), 1.0, 0.05, q.Model.Linear), q.Params(float(n), 1.0, 0.05, q.Model.Arclength)]
-    b = q.Batch([n, n], [g32, g32], p)
-    anchor = dev(np.array([ny / 2.0, ny / 2.0 + 0.25]))
-    o0 = np.random.default_rng(2).normal(0, 1, 2 * n).astype(np.float32)
+    p = [q.Params(float(n), 1.0, 0.05, q.Model.Linear), q.Params(float(n), 1.0, 0.05, q.Model.Arclength),
+         q.Params(37.0, 1.0, 0.05, q.Model.Linear)]
+    b = q.Batch([n, n, 37], [g32, g32, g32], p)
+    anchor = dev(np.array([ny / 2.0, ny / 2.0 + 0.25, ny / 2.0 - 0.5]))
+    o0 = np.random.default_rng(2).normal(0, 1, 2 * n + 37).astype(np.float32)
     steps = 40
     drift = np.full(steps, 0.01)
     diff = np.full(steps, np.sqrt(2 * 0.01 * 0.05))
