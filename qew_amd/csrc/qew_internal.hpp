@@ -129,11 +129,11 @@ struct qew_batch_s {
     std::vector<int64_t> offsets;
     qew_host::DevBuf d_lines, d_noises, d_tile_line, d_partial;
     qew_host::DevBuf d_scratch;  // solver workspaces (grown on demand)
-    // multi-step Langevin tiling: (line, first output site) per tile for
-    // ltile_k steps per launch, built on first use; lv_k = steps per launch
-    // picked by timing (0: not yet tuned); d_tune = line-state copies the
-    // timing runs on
-    int ltiles = 0, ltile_k = 0, lv_k = 0;
+    // multi-step Langevin tiling: (line, first output site) per tile of
+    // ltile_out output sites, built on first use; lv_k = launch plan picked by
+    // timing (K > 0: K steps per launch on workgroup tiles, -K: on wave
+    // tiles; 0: not yet tuned); d_tune = line-state copies the timing runs on
+    int ltiles = 0, ltile_out = 0, lv_k = 0;
     // Rosso-Krauth path picked by timing: 0 not yet, 1 LDS line kernel,
     // 2 per-colour launches
     int rk_path = 0;

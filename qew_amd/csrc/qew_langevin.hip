@@ -146,7 +146,8 @@ __global__ __launch_bounds__(kTile) void langevin32_kernel(StepArgs a) {
 // operation order and Philox key (seed + line, step, site)), so a launch
 // equals K single steps bit for bit, whatever K; the halo work is redundant
 // (kLvExt / (kLvExt - 2K)) and saves K - 1 launches and round trips through
-// memory. K (8, 16 or 32) is picked per batch by timing (lv_steps_per_launch).
+// memory. K and the tile kind (these workgroup tiles or the wave tiles
+// below) are picked per batch by timing (lv_plan).
 // Short lines (n < kLvExt) alias sites within the tile: the copies compute
 // identical values and each site is written once.
 constexpr int kLvExt = 256;
@@ -163,6 +164,7 @@ struct MultiArgs {
     const double *anchor;
     double drift[kLvMaxSteps], diff[kLvMaxSteps];
     int nsteps;     // steps of this launch, 1..K
+    int ntiles;     // entries of tiles
     uint64_t seed;  // line l uses seed + l
     uint64_t step0;
 };
@@ -331,10 +333,146 @@ __global__ __launch_bounds__(kLvExt) void langevin32_multi(MultiArgs a) {
         static_cast<float *>(a.h_out)[L.off + tl.y + o] = s_h[a.nsteps & 1][t];
 }
 
-// (line, first output site) per tile for K steps per launch
-void ensure_tiles(qew_batch b, int K) {
-    if (b->ltile_k == K) return;
-    const int out = kLvExt - 2 * K;
+// Wave tiles: one wavefront holds kWave consecutive sites (mod n) in
+// registers -- kWave - 2K output sites and a K-site halo each side -- and
+// advances them K steps. Neighbour heights, edge derivatives and slopes come
+// from cross-lane shuffles, so a step has no barrier and no LDS round trip; a
+// workgroup is four independent wave tiles. Each site's update uses the same
+// operations, operand order and Philox keys as the workgroup tiles above
+// (bit-identical results). Kicks are held in registers, so the step loops
+// are unrolled.
+template <int K>
+constexpr int lv_wave_out() { return kWave - 2 * K; }
+constexpr int kLvWavesPerBlock = kLvExt / kWave;
+
+// kick[s] = diff[s] * N(0,1) for steps 0 .. K-1 of this launch (entries of
+// steps at or beyond nsteps are not used). Pairs of lanes holding sites 2p
+// and 2p + 1 split the steps as in draw_kicks and swap the second normal of
+// each block through a lane-xor shuffle; other lanes draw their own.
+template <int K, typename T>
+__device__ __forceinline__ void wave_kicks(const MultiArgs &a, uint64_t key, int lane, int gi,
+                                           int gj, T (&kick)[K]) {
+    const bool paired = gi != gj && (gi >> 1) == (gj >> 1);
+    const int odd = lane & 1;
+#pragma unroll
+    for (int s2 = 0; s2 < K; s2 += 2) {
+        const int s = paired ? s2 + odd : s2;
+        double zc, zs;
+        philox_normal_pair(key, a.step0 + (uint64_t)s, (uint64_t)(gi >> 1), zc, zs);
+        const double mine = (gi & 1) ? zs : zc;   // this lane's site, step s
+        const double other = (gi & 1) ? zc : zs;  // the pair's other site, step s
+        const double recv = __shfl_xor(other, 1); // paired: this site, step s ^ 1
+        const double d0 = a.diff[s2], d1 = a.diff[s2 + 1];
+        if (paired) {
+            kick[s2] = (T)(d0 * (odd ? recv : mine));
+            kick[s2 + 1] = (T)(d1 * (odd ? mine : recv));
+        } else {
+            kick[s2] = (T)(d0 * mine);
+            kick[s2 + 1] = (T)(d1 * philox_normal(key, a.step0 + (uint64_t)(s2 + 1), (uint64_t)gi));
+        }
+    }
+}
+
+template <int K>
+__global__ __launch_bounds__(kLvExt) void langevin32_wave(MultiArgs a) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wt = blockIdx.x * kLvWavesPerBlock + (int)(threadIdx.x / kWave);
+    if (wt >= a.ntiles) return;  // the whole wavefront
+    const int2 tl = a.tiles[wt];
+    const int line = tl.x;
+    const LineDesc L = a.lines[line];
+    const NoiseDesc N = a.noises[L.noise];
+    const int n = L.n;
+    const double anchor = a.anchor ? a.anchor[line] : 0.0;
+    const LineGeom G = make_geom(L, N, anchor);
+    const float dxf = G.dxf, inv_dx = G.inv_dx;
+    const float lt = (float)L.lt, f = (float)L.f;
+    const bool arc = (L.model == 1);
+    const int gi = ext_site<K>(tl.y, lane, n);
+    const int gj = ext_site<K>(tl.y, lane ^ 1, n);
+    const int gp = (gi + 1 == n) ? 0 : gi + 1;
+    const bool upd = lane >= 1 && lane + 1 < kWave;
+    float kick[K];
+    wave_kicks<K>(a, a.seed + (uint64_t)line, lane, gi, gj, kick);
+    float h = static_cast<const float *>(a.h_in)[L.off + gi];
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+        if (s >= a.nsteps) break;
+        // lane 63 has no right neighbour: its edge is evaluated and discarded
+        const float hp = __shfl_down(h, 1);
+        const float hm = __shfl_up(h, 1);
+        float v, dy;
+        edge32<false, true>(G, gi, gp, h, hp, v, dy);
+        const float dh = (hp - h) * inv_dx;
+        const float dyl = __shfl_up(dy, 1);
+        float lf;
+        if (!arc) {
+            lf = -lt * ((hm + hp) - 2.0f * h) * (inv_dx * inv_dx);
+        } else {
+            const float sl = dh * rsqrtf(1.0f + dh * dh);
+            lf = lt * (__shfl_up(sl, 1) - sl) * inv_dx;
+        }
+        const float g = dxf * (lf + 0.5f * (dyl + dy) - f);
+        float hn = h - (float)a.drift[s] * g;
+        if (a.diff[s] > 0) hn += kick[s];
+        if (upd) h = hn;
+    }
+    const int o = lane - K;
+    if (o >= 0 && o < lv_wave_out<K>() && tl.y + o < n)
+        static_cast<float *>(a.h_out)[L.off + tl.y + o] = h;
+}
+
+template <typename C, int K>
+__global__ __launch_bounds__(kLvExt) void langevin64_wave(MultiArgs a) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wt = blockIdx.x * kLvWavesPerBlock + (int)(threadIdx.x / kWave);
+    if (wt >= a.ntiles) return;  // the whole wavefront
+    const int2 tl = a.tiles[wt];
+    const int line = tl.x;
+    const LineDesc L = a.lines[line];
+    const NoiseDesc N = a.noises[L.noise];
+    const int n = L.n;
+    const double dx = L.L / (double)n;
+    const double lt = L.lt, f = L.f;
+    const bool arc = (L.model == 1);
+    const int gi = ext_site<K>(tl.y, lane, n);
+    const int gj = ext_site<K>(tl.y, lane ^ 1, n);
+    const int gp = (gi + 1 == n) ? 0 : gi + 1;
+    const double xc = (double)(gi + gp) * 0.5 * dx;
+    const bool upd = lane >= 1 && lane + 1 < kWave;
+    double kick[K];
+    wave_kicks<K>(a, a.seed + (uint64_t)line, lane, gi, gj, kick);
+    double h = static_cast<const double *>(a.h_in)[L.off + gi];
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+        if (s >= a.nsteps) break;
+        const double hp = __shfl_down(h, 1);
+        const double hm = __shfl_up(h, 1);
+        const double dy = sample_dy64<C>(N, xc, (hp + h) * 0.5);
+        const double dyl = __shfl_up(dy, 1);
+        double line_force;
+        if (!arc) {
+            line_force = -lt * (hm + hp - 2.0 * h) / (dx * dx);
+        } else {
+            const double dh = (hp - h) / dx;
+            const double sl = dh / sqrt(1.0 + dh * dh);
+            line_force = lt * (__shfl_up(sl, 1) - sl) / dx;
+        }
+        const double g = dx * (line_force + 0.5 * (dyl + dy) - f);
+        const double dr = a.drift[s];
+        double hn;
+        if (a.diff[s] > 0) hn = h + (-dr * g + kick[s]);
+        else hn = h + -dr * g;
+        if (upd) h = hn;
+    }
+    const int o = lane - K;
+    if (o >= 0 && o < lv_wave_out<K>() && tl.y + o < n)
+        static_cast<double *>(a.h_out)[L.off + tl.y + o] = h;
+}
+
+// (line, first output site) per tile of `out` output sites
+void ensure_tiles(qew_batch b, int out) {
+    if (b->ltile_out == out) return;
     std::vector<int2> tiles;
     for (int l = 0; l < b->nlines; ++l)
         for (int o = 0; o < b->lines[l].n; o += out) tiles.push_back(make_int2(l, o));
@@ -343,11 +481,16 @@ void ensure_tiles(qew_batch b, int K) {
                            hipMemcpyHostToDevice, b->ctx->stream));
     QEW_HIP(hipStreamSynchronize(b->ctx->stream));
     b->ltiles = (int)tiles.size();
-    b->ltile_k = K;
+    b->ltile_out = out;
 }
 
+// Launch plan: plan > 0 runs `plan` steps per launch on workgroup tiles,
+// plan < 0 runs -plan steps per launch on wave tiles.
+int plan_steps(int plan) { return plan > 0 ? plan : -plan; }
+int plan_out(int plan) { return plan > 0 ? kLvExt - 2 * plan : kWave + 2 * plan; }
+
 template <int K>
-void launch_multi(qew_batch b, int dtype, const MultiArgs &m) {
+void launch_block(qew_batch b, int dtype, const MultiArgs &m) {
     const dim3 grid(b->ltiles), block(kLvExt);
     hipStream_t s = b->ctx->stream;
     if (dtype == QEW_F32) hipLaunchKernelGGL(langevin32_multi<K>, grid, block, 0, s, m);
@@ -356,15 +499,27 @@ void launch_multi(qew_batch b, int dtype, const MultiArgs &m) {
     QEW_HIP(hipGetLastError());
 }
 
-// nsteps steps, K per launch, ping-ponging between bufs[0] (input) and
-// bufs[1]; returns the buffer that holds the result
-void *multi_steps(qew_batch b, int K, int dtype, void *bufs[2], const double *anchor, int nsteps,
+template <int K>
+void launch_wave(qew_batch b, int dtype, const MultiArgs &m) {
+    const dim3 grid((b->ltiles + kLvWavesPerBlock - 1) / kLvWavesPerBlock), block(kLvExt);
+    hipStream_t s = b->ctx->stream;
+    if (dtype == QEW_F32) hipLaunchKernelGGL(langevin32_wave<K>, grid, block, 0, s, m);
+    else if (b->noises[0].dtype == 1) hipLaunchKernelGGL((langevin64_wave<float, K>), grid, block, 0, s, m);
+    else hipLaunchKernelGGL((langevin64_wave<double, K>), grid, block, 0, s, m);
+    QEW_HIP(hipGetLastError());
+}
+
+// nsteps steps with launch plan `plan`, ping-ponging between bufs[0] (input)
+// and bufs[1]; returns the buffer that holds the result
+void *multi_steps(qew_batch b, int plan, int dtype, void *bufs[2], const double *anchor, int nsteps,
                   const double *drift, const double *diff, uint64_t seed, uint64_t step0) {
-    ensure_tiles(b, K);
+    const int K = plan_steps(plan);
+    ensure_tiles(b, plan_out(plan));
     MultiArgs m{};
     m.lines = b->d_lines.as<LineDesc>();
     m.noises = b->d_noises.as<NoiseDesc>();
     m.tiles = b->d_ltile.as<int2>();
+    m.ntiles = b->ltiles;
     m.anchor = anchor;
     m.seed = seed;
     int launches = 0;
@@ -377,24 +532,37 @@ void *multi_steps(qew_batch b, int K, int dtype, void *bufs[2], const double *an
         m.step0 = step0 + (uint64_t)k0;
         m.h_in = bufs[launches & 1];
         m.h_out = bufs[(launches + 1) & 1];
-        if (K == 8) launch_multi<8>(b, dtype, m);
-        else if (K == 16) launch_multi<16>(b, dtype, m);
-        else launch_multi<kLvMaxSteps>(b, dtype, m);
+        switch (plan) {
+            case 8: launch_block<8>(b, dtype, m); break;
+            case 16: launch_block<16>(b, dtype, m); break;
+            case kLvMaxSteps: launch_block<kLvMaxSteps>(b, dtype, m); break;
+            case -8: launch_wave<8>(b, dtype, m); break;
+            case -16: launch_wave<16>(b, dtype, m); break;
+            default: throw qew_host::Error(QEW_ERR_INVALID_ARGUMENT, "bad Langevin launch plan");
+        }
     }
     return bufs[launches & 1];
 }
 
-// Steps per launch for this batch: QEW_LANGEVIN_STEPS_PER_LAUNCH (8, 16 or
-// 32) if set; else the value timed on the batch's first call of at least 96
-// steps (32 steps of each candidate on a copy of the line state, so the
-// caller's state is untouched; all candidates give identical results); 16
-// before that.
-int lv_steps_per_launch(qew_batch b, int dtype, const void *h, const double *anchor, int nsteps,
-                        const double *drift, const double *diff, uint64_t seed, uint64_t step0) {
-    if (const char *env = std::getenv("QEW_LANGEVIN_STEPS_PER_LAUNCH")) {
-        const int k = std::atoi(env);
+// Launch plan for this batch. QEW_LANGEVIN_TILE ("block" or "wave") and
+// QEW_LANGEVIN_STEPS_PER_LAUNCH (8, 16 or 32; 32 for block tiles only) pin it
+// (16 steps when only the tile kind is given, block tiles when only the step
+// count is); otherwise the plan timed on the batch's first call of at least
+// 96 steps (32 steps of each candidate on a copy of the line state, so the
+// caller's state is untouched; every plan gives identical results); 16
+// steps on block tiles before that.
+int lv_plan(qew_batch b, int dtype, const void *h, const double *anchor, int nsteps,
+            const double *drift, const double *diff, uint64_t seed, uint64_t step0) {
+    const char *env_k = std::getenv("QEW_LANGEVIN_STEPS_PER_LAUNCH");
+    const char *env_t = std::getenv("QEW_LANGEVIN_TILE");
+    if (env_k || env_t) {
+        const int k = env_k ? std::atoi(env_k) : 16;
         QEW_REQUIRE(k == 8 || k == 16 || k == 32, "QEW_LANGEVIN_STEPS_PER_LAUNCH must be 8, 16 or 32");
-        return k;
+        const std::string tile = env_t ? env_t : "block";
+        QEW_REQUIRE(tile == "block" || tile == "wave", "QEW_LANGEVIN_TILE must be block or wave");
+        if (tile == "block") return k;
+        QEW_REQUIRE(k != 32, "wave tiles take 8 or 16 steps per launch");
+        return -k;
     }
     if (b->lv_k) return b->lv_k;
     if (nsteps < 3 * kLvMaxSteps) return 16;
@@ -409,18 +577,18 @@ int lv_steps_per_launch(qew_batch b, int dtype, const void *h, const double *anc
     QEW_HIP(hipEventCreate(&e1));
     int best = 16;
     float best_ms = 0;
-    for (int K : {8, 16, kLvMaxSteps}) {
+    for (int plan : {8, 16, kLvMaxSteps, -8, -16}) {
         QEW_HIP(hipMemcpyAsync(tb[0], h, bytes, hipMemcpyDeviceToDevice, s));
-        ensure_tiles(b, K);
+        ensure_tiles(b, plan_out(plan));
         QEW_HIP(hipEventRecord(e0, s));
-        multi_steps(b, K, dtype, tb, anchor, kLvMaxSteps, drift, diff, seed, step0);
+        multi_steps(b, plan, dtype, tb, anchor, kLvMaxSteps, drift, diff, seed, step0);
         QEW_HIP(hipEventRecord(e1, s));
         QEW_HIP(hipEventSynchronize(e1));
         float ms = 0;
         QEW_HIP(hipEventElapsedTime(&ms, e0, e1));
         if (best_ms == 0 || ms < best_ms) {
             best_ms = ms;
-            best = K;
+            best = plan;
         }
     }
     (void)hipEventDestroy(e0);
@@ -491,8 +659,8 @@ qew_status qew_batch_langevin(qew_batch b, int dtype, void *h, const double *anc
         b->d_scratch.ensure(es * (size_t)b->total);
         void *bufs[2] = {h, b->d_scratch.ptr};
         if (nsteps >= 2) {
-            const int K = lv_steps_per_launch(b, dtype, h, anchor, nsteps, drift, diff, seed, step0);
-            void *res = multi_steps(b, K, dtype, bufs, anchor, nsteps, drift, diff, seed, step0);
+            const int plan = lv_plan(b, dtype, h, anchor, nsteps, drift, diff, seed, step0);
+            void *res = multi_steps(b, plan, dtype, bufs, anchor, nsteps, drift, diff, seed, step0);
             if (res != h)
                 QEW_HIP(hipMemcpyAsync(h, res, es * (size_t)b->total, hipMemcpyDeviceToDevice,
                                        ctx->stream));

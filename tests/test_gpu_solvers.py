@@ -169,17 +169,21 @@ def test_batch_langevin_matches_single(q):
     assert np.max(np.abs(host(hb)[n:] - host(hs))) > 0  # line 1 uses seed + 1
 
 
-@pytest.mark.parametrize("table,k,steps", [("f64", "16", 37), ("f32", "16", 37), ("f64", "8", 37),
-                                           ("f64", "32", 37), ("f64", None, 100)])
-def test_batch_langevin_multistep_tiles_bitwise(q, monkeypatch, table, k, steps):
-    # several steps per launch on tiles with a halo (lines shorter and longer
-    # than a tile, both models, a partial last launch, steps without noise),
-    # for each steps-per-launch value and for the timed choice (>= 96 steps):
-    # every line equals single steps of the same stream bit for bit
-    if k is None:
-        monkeypatch.delenv("QEW_LANGEVIN_STEPS_PER_LAUNCH", raising=False)
-    else:
-        monkeypatch.setenv("QEW_LANGEVIN_STEPS_PER_LAUNCH", k)
+@pytest.mark.parametrize("table,k,steps,tile", [
+    ("f64", "16", 37, "block"), ("f32", "16", 37, "block"), ("f64", "8", 37, "block"),
+    ("f64", "32", 37, "block"), ("f64", "8", 37, "wave"), ("f64", "16", 37, "wave"),
+    ("f32", "8", 37, "wave"), ("f32", "16", 37, "wave"), ("f64", None, 100, None)])
+def test_batch_langevin_multistep_tiles_bitwise(q, monkeypatch, table, k, steps, tile):
+    # several steps per launch on workgroup or wave tiles with a halo (lines
+    # shorter and longer than a tile, odd lengths, both models, a partial last
+    # launch, steps without noise), for each launch plan and for the timed
+    # choice (>= 96 steps): every line equals single steps of the same stream
+    # bit for bit
+    for name, val in (("QEW_LANGEVIN_STEPS_PER_LAUNCH", k), ("QEW_LANGEVIN_TILE", tile)):
+        if val is None:
+            monkeypatch.delenv(name, raising=False)
+        else:
+            monkeypatch.setenv(name, val)
     field, coef = O.filtered_noise(256, 64, 2.56, 1.0, 0.5, 0.2, 0.2, 6)
     gn = q.FilteredNoise.from_coefficients(coef, 0.01, 1.0 / 64)
     if table == "f32":
@@ -203,9 +207,12 @@ def test_batch_langevin_multistep_tiles_bitwise(q, monkeypatch, table, k, steps)
         np.testing.assert_array_equal(host(hb)[off[k]:off[k + 1]], host(hs), err_msg=f"line {k} n={n}")
 
 
-def test_batch_langevin_fp32_multistep_matches_single_steps(q):
+@pytest.mark.parametrize("tile", ["block", "wave"])
+def test_batch_langevin_fp32_multistep_matches_single_steps(q, monkeypatch, tile):
     # FP32 line state: a multi-step call equals the same steps issued one per
     # call (two lines over several tiles, one odd line shorter than a tile)
+    monkeypatch.setenv("QEW_LANGEVIN_TILE", tile)
+    monkeypatch.delenv("QEW_LANGEVIN_STEPS_PER_LAUNCH", raising=False)
     n, ny = 700, 256
     field, coef = O.filtered_noise(n, ny, float(n), float(ny), 1.0, 4.0, 4.0, 5)
     g32 = q.FilteredNoise.from_coefficients(coef, 1.0, 1.0, dtype=q.F32)
