@@ -442,6 +442,11 @@ def main():
     head = r32 if r32 else r64
     head_dtype = "f32" if r32 else "f64"
     traffic = traffic_from_profiles()
+    traffic_launch = None
+    if traffic and head_dtype in traffic.get("detail", {}):
+        d = traffic["detail"][head_dtype]
+        # HBM bytes of one launch of B evaluations (rocprof FETCH_SIZE + WRITE_SIZE per evaluation)
+        traffic_launch = (d["fetch_bytes_per_eval"] + d["write_bytes_per_eval"]) * B
     line = {
         "metric": "grad evals/s + % HBM peak at L=65536; batched L-BFGS minima/s, FP32 vs FP64",
         "value": head["evals_per_s"],
@@ -460,7 +465,7 @@ def main():
                    "L": L, "noise_grid": [L, NY], "xi": XI, "evals_per_step": B},
         "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": head["achieved_GBs"] / HBM_PEAK_GBS,
-                     "traffic": traffic.get(head_dtype) if traffic else None},
+                     "traffic": traffic_launch},
         "cpu_baseline": cpu,
     }
     if r64:
