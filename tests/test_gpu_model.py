@@ -49,7 +49,7 @@ def test_sampler_analytic(q):
         np.testing.assert_allclose(host(g), r, atol=1e-14)
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 64, 255, 256, 257, 1000, 4096])
+@pytest.mark.parametrize("n", [1, 2, 3, 64, 254, 255, 256, 257, 510, 511, 1000, 4096])
 @pytest.mark.parametrize("model", [0, 1])
 def test_gradient_and_energy_match_oracle(q, n, model):
     nx, ny = max(n, 8), 64
@@ -138,7 +138,8 @@ def test_noise_from_field_golden(q):
 
 def test_batch_eval_ragged_equals_single_lines(q):
     torch = torch_cuda()
-    lengths = [1024, 77, 2048, 256, 4096, 3]
+    # tile boundaries at multiples of 255 sites: 255, 256, 510, 511
+    lengths = [1024, 77, 2048, 256, 4096, 3, 255, 511, 510]
     lines, noises, params, oracle = [], [], [], []
     for k, n in enumerate(lengths):
         on, gn, _ = table_pair(q, max(n, 8), 32, max(n, 8) * 1.0, 32.0, 100 + k, xi=4.0)

@@ -270,6 +270,66 @@ def test_rk_washboard_critical_force(q):
     assert r.runaway and not r.blocked and r.mean_h > 3.0
 
 
+def test_fc_bisection_to_1e10_matches_oracle(q):
+    """The depinning driver's force ramp and f_c bisection (the facade's
+    depinning_ramp: ramp f = k f_step carrying h over, then bisection of
+    [last blocked, first unblocked] warm-started from the highest blocked
+    configuration) run to 1e-10 relative width with the GPU rk_block and
+    with the oracle's: identical decisions on every probe, identical bracket
+    bits and critical configuration. Probes are capped at 2000 sweeps (a
+    capped probe counts as not blocked, as in the driver), which defines the
+    same critical force on both sides and keeps the run to seconds."""
+    n = 32
+    field, coef = O.filtered_noise(n, 64, n * 0.05, 1.0, 0.5, 0.2, 0.2, 7)
+    on = O.OracleNoise.table(coef, n * 0.05, 1.0)
+    gn = q.FilteredNoise.from_coefficients(coef, 0.05, 1.0 / 64)
+    kw = dict(dstep=0.5 / 64, max_advance=1.0, root_tol=1e-11, sweep_tol=1e-9, runaway=1.0,
+              max_sweeps=2000)
+    rpo = O.rk_params(**kw)
+    rp = q.RkParams(kw["dstep"], kw["max_advance"], kw["root_tol"], kw["sweep_tol"], kw["runaway"],
+                    kw["max_sweeps"])
+
+    def gpu(h, f):
+        hd = dev(h)
+        r = q.rk_block(hd, q.Params(n * 0.05, 1.0, f, q.Model.Linear), gn, rp)
+        return bool(r.blocked), host(hd)
+
+    def oracle(h, f):
+        hh = h.copy()
+        r = O.rk_block(hh, O.params(n * 0.05, 1.0, f, 0), on, rpo)
+        return bool(r.blocked), hh
+
+    def depin(rk, f_step=0.1, rel_tol=1e-10):
+        h, f_lo, f_hi, h_lo, trace = np.full(n, 0.5), None, None, None, []
+        for k in range(400):
+            f = k * f_step
+            blocked, h = rk(h, f)
+            trace.append(blocked)
+            if not blocked:
+                f_hi = f
+                break
+            f_lo, h_lo = f, h.copy()
+        lo, hi = f_lo, f_hi
+        while hi - lo > rel_tol * max(abs(hi), abs(lo)):
+            mid = 0.5 * (lo + hi)
+            if not (lo < mid < hi):
+                break
+            blocked, hm = rk(h_lo, mid)
+            trace.append(blocked)
+            if blocked:
+                lo, h_lo = mid, hm
+            else:
+                hi = mid
+        return lo, hi, h_lo, trace
+
+    lo_g, hi_g, hc_g, tr_g = depin(gpu)
+    lo_o, hi_o, hc_o, tr_o = depin(oracle)
+    assert tr_g == tr_o and len(tr_g) > 60
+    assert (lo_g, hi_g) == (lo_o, hi_o)
+    assert 0 < hi_g - lo_g <= 1e-10 * hi_g
+    np.testing.assert_array_equal(hc_g, hc_o)
+
+
 @pytest.mark.parametrize("path", ["line", "launch"])
 @pytest.mark.parametrize("n,lt,f,max_sweeps", [
     (16384, 0.005, 0.1, 3000),   # largest line the LDS path holds; runs to blocked
