@@ -128,7 +128,7 @@ def bench_lbfgs(log, nlines=512, plain_sample=8):
     # ftol 1e-5: above the Armijo roundoff floor of the longest lines (L=16384,
     # |E| ~ 1e4), where the reference algorithm converges in ~100-300 iterations
     # FP32: the line offsets' resolution floors the FP32 gradient near 1e-4, so
-    # FP32 stops there and is judged by its energy against the FP64 minimum
+    # FP32 stops there and is compared by its energy against the FP64 minimum
     for name, noises, dtype, ftol in (("f64", n64, q.F64, 1e-5), ("f32", n32, q.F32, 1e-4)):
         b = q.Batch(lengths, noises, params)
         b.set_precond_shifts(shifts)
@@ -343,7 +343,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(this build has no multi-GPU path)", file=sys.stderr)
+              "(the engine runs on one GPU)", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -688,8 +688,8 @@ void orc_lbfgs(int n, double *h, const orc_params *p, const orc_noise *nz,
 // Philox4x32-10 (Salmon et al., SC'11 "Parallel random numbers: as easy as
 // 1, 2, 3") and the counter-based normal used by the Langevin step. The
 // reference draws from a Kokkos XorShift64 pool (src/dynamics.h:29,49-58),
-// whose stream is scheduling-dependent; this build keys every normal by
-// (seed, step, site) instead.
+// whose stream is scheduling-dependent; the GPU engine keys every normal by
+// (seed, step, site) instead, and so does this restatement.
 // ===========================================================================
 void orc_philox4x32_10(const uint32_t *ctr_in, const uint32_t *key_in, uint32_t *out) {
     uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
@@ -838,7 +838,7 @@ int orc_rk_block(int n, double *h, const orc_params *p, const orc_noise *nz,
     return 0;
 }
 
-// Number of OpenMP threads this build uses (1 for the sequential oracle).
+// Number of OpenMP threads this library uses (1 for the sequential oracle).
 int orc_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

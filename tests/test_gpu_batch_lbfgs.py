@@ -136,7 +136,7 @@ def test_batch_lbfgs_long_lines_match_oracle(q):
         h = dev(np.concatenate(h0))
         # FP32 line offsets resolve ~ulp(|offset|) ~ 4e-6 cells at 50 cells from the
         # anchor, which floors the FP32 gradient near 1e-4 x curvature scale:
-        # FP32 runs stop at ftol 1e-4 and are judged by the energy they reach
+        # FP32 runs stop at ftol 1e-4 and are checked by the energy they reach
         ftol = 1e-5 if dtype == q.F64 else 1e-4
         res = b.lbfgs(h, q.LbfgsParams(m=10, ftol=ftol, max_iter=4000, precondition=True),
                       dtype=dtype)
