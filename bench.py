@@ -124,7 +124,8 @@ def bench_lbfgs(log, nlines=512, plain_sample=8):
     shifts = [0.5 / xi ** 2 for xi in xis]
     h0 = torch.full((sum(lengths),), ny / 2.0, dtype=torch.float64, device="cuda:0")
     out = {}
-    finals = {}
+    finals, gmax = {}, {}
+    offs = np.concatenate([[0], np.cumsum(lengths)])
     # ftol 1e-5: above the Armijo roundoff floor of the longest lines (L=16384,
     # |E| ~ 1e4), where the reference algorithm converges in ~100-300 iterations
     # FP32: the line offsets' resolution floors the FP32 gradient near 1e-4, so
@@ -138,9 +139,11 @@ def bench_lbfgs(log, nlines=512, plain_sample=8):
         t1 = time.perf_counter()
         res = b.lbfgs(h, lp, dtype=dtype)
         dt = time.perf_counter() - t1
-        # energies of the minima, evaluated in FP64 on the FP64 tables
-        e, _ = q.Batch(lengths, n64, params).eval(h)
+        # energies and gradients of the minima, evaluated in FP64 on the FP64 tables
+        e, g = q.Batch(lengths, n64, params).eval(h)
         finals[name] = e.cpu().numpy()
+        ga = g.abs().cpu().numpy()
+        gmax[name] = np.array([ga[o0:o1].max() for o0, o1 in zip(offs[:-1], offs[1:])])
         its = [r.iterations for r in res]
         out[name] = {"minima_per_s": nlines / dt, "seconds": dt, "ftol": ftol,
                      "converged": int(sum(r.converged for r in res)),
@@ -154,6 +157,9 @@ def bench_lbfgs(log, nlines=512, plain_sample=8):
     out["f32_vs_f64_energy_rel_err"] = {"max": float(rel_err.max()),
                                         "median": float(np.median(rel_err)),
                                         "lines_within_1e-6": int((rel_err <= 1e-6).sum())}
+    # max |grad| per line at each precision's minima, from the FP64 gradient
+    out["max_grad_fp64_at_minima"] = {k: {"median": float(np.median(v)), "max": float(v.max())}
+                                      for k, v in gmax.items()}
     # plain FP64 host-driven path, one line at a time, on a sample of lines
     idx = list(range(0, nlines, max(1, nlines // plain_sample)))[:plain_sample]
     t2 = time.perf_counter()
