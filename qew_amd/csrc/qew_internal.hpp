@@ -77,6 +77,30 @@ struct DevBuf {
     }
 };
 
+// Two timing events, destroyed on every exit path (launch-plan timing).
+struct EventPair {
+    hipEvent_t start = nullptr, stop = nullptr;
+    EventPair() {
+        QEW_HIP(hipEventCreate(&start));
+        hipError_t e = hipEventCreate(&stop);
+        if (e != hipSuccess) {
+            (void)hipEventDestroy(start);
+            QEW_HIP(e);
+        }
+    }
+    ~EventPair() {
+        (void)hipEventDestroy(start);
+        (void)hipEventDestroy(stop);
+    }
+    EventPair(const EventPair &) = delete;
+    EventPair &operator=(const EventPair &) = delete;
+    float elapsed_ms() const {
+        float ms = 0;
+        QEW_HIP(hipEventElapsedTime(&ms, start, stop));
+        return ms;
+    }
+};
+
 // Pinned host staging for small results (host scalars returned by
 // objective(), dot products in the host-driven L-BFGS, ...).
 struct PinnedBuf {

@@ -572,27 +572,22 @@ int lv_plan(qew_batch b, int dtype, const void *h, const double *anchor, int nst
     b->d_tune.ensure(2 * bytes);
     char *t0 = b->d_tune.as<char>();
     void *tb[2] = {t0, t0 + bytes};
-    hipEvent_t e0, e1;
-    QEW_HIP(hipEventCreate(&e0));
-    QEW_HIP(hipEventCreate(&e1));
+    const qew_host::EventPair ev;
     int best = 16;
     float best_ms = 0;
     for (int plan : {8, 16, kLvMaxSteps, -8, -16}) {
         QEW_HIP(hipMemcpyAsync(tb[0], h, bytes, hipMemcpyDeviceToDevice, s));
         ensure_tiles(b, plan_out(plan));
-        QEW_HIP(hipEventRecord(e0, s));
+        QEW_HIP(hipEventRecord(ev.start, s));
         multi_steps(b, plan, dtype, tb, anchor, kLvMaxSteps, drift, diff, seed, step0);
-        QEW_HIP(hipEventRecord(e1, s));
-        QEW_HIP(hipEventSynchronize(e1));
-        float ms = 0;
-        QEW_HIP(hipEventElapsedTime(&ms, e0, e1));
+        QEW_HIP(hipEventRecord(ev.stop, s));
+        QEW_HIP(hipEventSynchronize(ev.stop));
+        const float ms = ev.elapsed_ms();
         if (best_ms == 0 || ms < best_ms) {
             best_ms = ms;
             best = plan;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     b->lv_k = best;
     return best;
 }
