@@ -68,7 +68,7 @@ __global__ __launch_bounds__(kTile) void langevin64_kernel(StepArgs a, LineDesc 
         s_dvdy[tid] = sample_dy64<C>(N, xc, hc);
         s_h[tid] = hi;
         if (arc) {
-            const double dh = (hip - hi) / dx;
+            const double dh = div1(hip - hi, dx);
             s_slope[tid] = dh / sqrt(1.0 + dh * dh);
         }
     }
@@ -77,9 +77,9 @@ __global__ __launch_bounds__(kTile) void langevin64_kernel(StepArgs a, LineDesc 
     double line_force;
     if (!arc) {
         const double d2h = s_h[tid - 1] + hip - 2.0 * hi;
-        line_force = -lt * d2h / (dx * dx);
+        line_force = div1(-lt * d2h, dx * dx);
     } else {
-        line_force = lt * (s_slope[tid - 1] - s_slope[tid]) / dx;
+        line_force = div1(lt * (s_slope[tid - 1] - s_slope[tid]), dx);
     }
     const double g = dx * (line_force + 0.5 * (s_dvdy[tid - 1] + s_dvdy[tid]) - f);
     double hn;
@@ -253,7 +253,7 @@ __global__ __launch_bounds__(kLvExt) void langevin64_multi(MultiArgs a) {
             const double hp = s_h[cur][t + 1];
             s_dvdy[t + 1] = sample_dy64<C>(N, xc, (hp + hi) * 0.5);
             if (arc) {
-                const double dh = (hp - hi) / dx;
+                const double dh = div1(hp - hi, dx);
                 s_slope[t + 1] = dh / sqrt(1.0 + dh * dh);
             }
         }
@@ -261,8 +261,8 @@ __global__ __launch_bounds__(kLvExt) void langevin64_multi(MultiArgs a) {
         double hn = hi;
         if (t >= 1 && t + 1 < kLvExt) {
             double line_force;
-            if (!arc) line_force = -lt * (s_h[cur][t - 1] + s_h[cur][t + 1] - 2.0 * hi) / (dx * dx);
-            else line_force = lt * (s_slope[t] - s_slope[t + 1]) / dx;
+            if (!arc) line_force = div1(-lt * (s_h[cur][t - 1] + s_h[cur][t + 1] - 2.0 * hi), dx * dx);
+            else line_force = div1(lt * (s_slope[t] - s_slope[t + 1]), dx);
             const double g = dx * (line_force + 0.5 * (s_dvdy[t] + s_dvdy[t + 1]) - f);
             const double dr = a.drift[s];
             if (a.diff[s] > 0) hn = hi + (-dr * g + s_kick[s % KC][t]);
@@ -452,11 +452,11 @@ __global__ __launch_bounds__(kLvExt) void langevin64_wave(MultiArgs a) {
         const double dyl = __shfl_up(dy, 1);
         double line_force;
         if (!arc) {
-            line_force = -lt * (hm + hp - 2.0 * h) / (dx * dx);
+            line_force = div1(-lt * (hm + hp - 2.0 * h), dx * dx);
         } else {
-            const double dh = (hp - h) / dx;
+            const double dh = div1(hp - h, dx);
             const double sl = dh / sqrt(1.0 + dh * dh);
-            line_force = lt * (__shfl_up(sl, 1) - sl) / dx;
+            line_force = div1(lt * (__shfl_up(sl, 1) - sl), dx);
         }
         const double g = dx * (line_force + 0.5 * (dyl + dy) - f);
         const double dr = a.drift[s];
