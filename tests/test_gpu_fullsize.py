@@ -1,14 +1,16 @@
-"""Parity at the full size of the gradient/objective throughput configuration:
-L = 65536 sites on a 65536 x 8192 FilteredNoise table (xi = 8 grid units,
-seed 1), arclength elasticity, a seeded unit-variance random walk started at
-Nh/2 -- the workload bench.py times. The table is generated on the GPU and
-its coefficients are handed to the oracle, so both sides see the same table.
+"""Parity at the full sizes of the benchmark configurations; the tables are
+generated on the GPU and their coefficients handed to the oracle, so both
+sides see the same table.
 
-Bar: FP64 gradient bit-identical to the oracle, energy within 1e-12
-relative; FP32-compensated batch evaluation within the stated error
-(gradient < 2e-4 of max|grad|, energy < 1e-5 relative); evaluation r of a
-batch of shifted walks equals the single evaluation of that walk bit for
-bit. Host memory: one 4.3 GB copy of the table.
+- gradient/objective throughput: L = 65536 sites on a 65536 x 8192 table
+  (xi = 8, seed 1), arclength, a seeded unit-variance random walk started at
+  Nh/2 -- the workload bench.py times. FP64 gradient bit-identical to the
+  oracle, energy within 1e-12 relative; FP32-compensated batch evaluation
+  within the stated error (gradient < 2e-4 of max|grad|, energy < 1e-5
+  relative); evaluation r of a batch of shifted walks equals the single
+  evaluation of that walk bit for bit. Host memory: one 4.3 GB table copy.
+- Langevin anneal: L = 65536 on a 65536 x 4096 table, 20 steps.
+- Rosso-Krauth: one L = 16384 line on a 16384 x 1024 table, bit for bit.
 """
 import numpy as np
 import pytest
@@ -93,3 +95,31 @@ def test_full_size_langevin_matches_oracle(q):
     h32 = host(off).astype(np.float64) + ny / 2
     assert rel(np.mean(h32 - ny / 2), np.mean(h_ref - ny / 2)) < 1e-5
     assert rel(np.var(h32), np.var(h_ref)) < 1e-5
+
+
+def test_full_size_rk_matches_oracle(q):
+    """One line of the batched Rosso-Krauth configuration (L = 16384 on a
+    16384 x 1024 table, period M = 1024 in h, xi = 4, seed 1, linear, line
+    tension 1, the depinning tool's knobs): relaxation at f = 0 and the first
+    ramp probe f = 0.25 from the flat line at M/2, on the FP64 table and on
+    its FP32 copy (the oracle then runs on the FP32-rounded coefficients):
+    heights, sweep counts and evaluation counts identical to the oracle."""
+    n, ny = 16384, 1024
+    n64 = q.FilteredNoise(n, ny, float(n), float(ny), 1.0, 4.0, 4.0, 1, dtype=q.F64)
+    coef = n64.coefficients()
+    kw = dict(dstep=0.5, max_advance=float(ny), root_tol=1e-10, sweep_tol=1e-8,
+              runaway=2.0 * ny, max_sweeps=200000)
+    rpo = O.rk_params(**kw)
+    rp = q.RkParams(kw["dstep"], kw["max_advance"], kw["root_tol"], kw["sweep_tol"], kw["runaway"],
+                    kw["max_sweeps"])
+    for gn, c in ((n64, coef), (n64.convert(q.F32), coef.astype(np.float32).astype(np.float64))):
+        on = O.OracleNoise.table(c, float(n), float(ny))
+        h_ref = np.full(n, ny / 2.0)
+        hd = dev(h_ref)
+        for f in (0.0, 0.25):
+            r_ref = O.rk_block(h_ref, O.params(float(n), 1.0, f, O.LINEAR), on, rpo, omp=True)
+            r = q.rk_block(hd, q.Params(float(n), 1.0, f, q.Model.Linear), gn, rp)
+            assert (r.blocked, r.runaway, r.sweeps) == (bool(r_ref.blocked), bool(r_ref.runaway),
+                                                         r_ref.sweeps), f
+            assert r.blocked and r.n_velocity_evals == r_ref.n_velocity_evals
+            np.testing.assert_array_equal(host(hd), h_ref)
