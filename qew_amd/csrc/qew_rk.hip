@@ -281,8 +281,9 @@ __device__ bool sweep_status(RkState &st, double total, int n, const qew_rk_para
 // evaluation rounds instead of the ~40 of a sequential march and bisection.
 // The advance sum is formed per site owner in a fixed order. Registers are
 // capped for 3 waves per SIMD (2 uncapped): the extra latency hiding in
-// sweeps where most sites move outweighs the few spills (measured: 6 % less
-// time on the 64 x 16384 depinning batch; a 4-wave cap spills more and loses).
+// sweeps where most sites move outweighs the few spills (measured on the
+// 64 x 16384 depinning batch: 1.5-6 % less time, same probes and sweeps; a
+// 4-wave cap spills more and loses).
 template <typename C>
 __global__ __launch_bounds__(kRkBlock, 3) void rk_color_kernel(RkArgs a, int color) {
     __shared__ int s_list[kRkBlock];
