@@ -181,6 +181,36 @@ __device__ __forceinline__ void row4_64(const NoiseDesc &nd, int ix, int iy, dou
     }
 }
 
+// The 4 x 4 coefficient block of a cell, c[a][b] = coef(x-row a, y level b),
+// held in the table's storage type.
+template <typename C>
+__device__ __forceinline__ void gather_block64(const NoiseDesc &nd, int ix, const int iyn[4],
+                                               C c[4][4]) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        double r[4];
+        row4_64<C>(nd, ix, iyn[b], r);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) c[a][b] = (C)r[a];
+    }
+}
+
+// V from a held block and the weights (the reference's summation order,
+// sample_value, src/filtered_noise.h:138-155): rows over y first, then over x.
+template <typename C>
+__device__ __forceinline__ double v_from_block(const C (&c)[4][4], const double wx[4],
+                                               const double wy[4]) {
+    double vv = 0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        double row = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) row += (double)c[a][b] * wy[b];
+        vv += wx[a] * row;
+    }
+    return vv;
+}
+
 // Analytic potentials: the reference tests' functors, evaluated with the
 // device math library (ocml).
 __device__ inline void analytic_sample(const NoiseDesc &nd, double x, double y, double &v,
@@ -296,23 +326,9 @@ __device__ __forceinline__ double sample_v64(const NoiseDesc &nd, double x, doub
     double wx[4], wy[4];
     bspline_w(fx, wx);
     bspline_w(fy, wy);
-    double c[4][4];
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        double r[4];
-        row4_64<C>(nd, ixn[1], iyn[b], r);
-#pragma unroll
-        for (int a = 0; a < 4; ++a) c[a][b] = r[a];
-    }
-    double vv = 0;
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        double row = 0;
-#pragma unroll
-        for (int b = 0; b < 4; ++b) row += c[a][b] * wy[b];
-        vv += wx[a] * row;
-    }
-    return vv;
+    C c[4][4];
+    gather_block64<C>(nd, ixn[1], iyn, c);
+    return v_from_block(c, wx, wy);
 }
 
 // Full sample with both derivatives (src/filtered_noise.h:103-134).

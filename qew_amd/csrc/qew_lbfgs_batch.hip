@@ -29,6 +29,7 @@
 // stay FP64.
 #include <algorithm>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 #include "qew_internal.hpp"
@@ -254,6 +255,109 @@ __device__ double wg_objective(const LineGeom &G, const T *x, const T *d, double
     return block_sum(acc, r.d);
 }
 
+// Table potentials: the kLS trial heights of one edge nearly always lie in
+// one y cell (trial steps are small against a cell), so the edge's 4 x 4
+// block is gathered once, for trial 0, and every trial in that cell takes V
+// from it (same locate, weights and summation order as sample_v64, so the
+// same value); a trial in another cell is sampled on its own. One block per
+// edge instead of kLS: FP32 state (FP32 blocks) keeps two edges in flight,
+// FP64 state one (measured on the 512-line batch: +14 % and +5 % minima/s;
+// more edges spill). Per-thread accumulation runs over the thread's edges in
+// order, as in the one-edge-at-a-time loop.
+template <typename T>
+constexpr int kUT = sizeof(T) == 8 ? 1 : 2;
+
+template <typename T, typename C>
+__device__ __forceinline__ void objective_trials_table(const LineGeom &G, const T *x, const T *d,
+                                                       const double *alpha, double (&acc)[kLS]) {
+    const int n = G.n;
+    constexpr int K = kLS;
+    using B = typename std::conditional<sizeof(T) == 8, C, float>::type;  // block storage
+    // trial heights of edge (i, ip) for step alpha[j]: FP64 state adds
+    // alpha d; FP32 state forms the offset in FP32 and adds the anchor
+    auto heights = [&](const P4<T> &v, int j, double &hi, double &hip) {
+        if constexpr (sizeof(T) == 8) {
+            hi = v.a + alpha[j] * v.b;
+            hip = v.c + alpha[j] * v.d;
+        } else {
+            const float al = (float)alpha[j];
+            hi = G.anchor + (double)__builtin_fmaf(al, v.b, v.a);
+            hip = G.anchor + (double)__builtin_fmaf(al, v.d, v.c);
+        }
+    };
+    struct Item {
+        P4<T> v;
+        int iy;
+        B c[4][4];
+    };
+    stream<kUT<T>>(
+        n,
+        [&](int i) {
+            const int ip = (i + 1 == n) ? 0 : i + 1;
+            Item it;
+            it.v = P4<T>{x[i], d[i], x[ip], d[ip]};
+            double hi, hip, fx, fy;
+            heights(it.v, 0, hi, hip);
+            int ixn[4], iyn[4];
+            locate64(G.nd, (double)(i + ip) * 0.5 * G.dx, (hip + hi) * 0.5, ixn, iyn, fx, fy);
+            it.iy = iyn[1];
+            gather_block64<B>(G.nd, ixn[1], iyn, it.c);
+            return it;
+        },
+        [&](int i, const Item &it) {
+            const int ip = (i + 1 == n) ? 0 : i + 1;
+            const double xc = (double)(i + ip) * 0.5 * G.dx;
+            // trials in the held cell first; the others are sampled after the
+            // held block is dead (each trial has its own accumulator)
+            double vt[K];
+            bool held[K];
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                double hi, hip;
+                heights(it.v, j, hi, hip);
+                int ixn[4], iyn[4];
+                double fx, fy;
+                locate64(G.nd, xc, (hip + hi) * 0.5, ixn, iyn, fx, fy);
+                held[j] = iyn[1] == it.iy;
+                vt[j] = 0;
+                if (held[j]) {
+                    double wx[4], wy[4];
+                    bspline_w(fx, wx);
+                    bspline_w(fy, wy);
+                    vt[j] = v_from_block(it.c, wx, wy);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                if (!held[j]) {
+                    double hi, hip;
+                    heights(it.v, j, hi, hip);
+                    vt[j] = sample_v64<B>(G.nd, xc, (hip + hi) * 0.5);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                double hi, hip;
+                heights(it.v, j, hi, hip);
+                const double vv = vt[j];
+                double le;
+                if constexpr (sizeof(T) == 8) {
+                    // src/qew_model.h:53-99 (TrialAt: x0(i) + alpha * d(i))
+                    const double dh = div1(hip - hi, G.dx);
+                    le = (G.model == 0) ? G.lt * dh * dh * 0.5 : G.lt * (sqrt(1.0 + dh * dh) - 1.0);
+                } else {
+                    const float al = (float)alpha[j];
+                    const double oi = (double)__builtin_fmaf(al, it.v.b, it.v.a);
+                    const double oip = (double)__builtin_fmaf(al, it.v.d, it.v.c);
+                    const double dh = div1(oip - oi, G.dx);
+                    const double s2 = dh * dh;
+                    le = (G.model == 0) ? G.lt * s2 * 0.5 : G.lt * (s2 / (sqrt(1.0 + s2) + 1.0));
+                }
+                acc[j] += G.dx * (le + vv - G.f * hi);
+            }
+        });
+}
+
 // Energies at the kLS trial points x + alpha[j] d in one pass over the line.
 // Each site term, each thread's summation order and the workgroup sum are
 // those of wg_objective, so phi[j] equals wg_objective(G, x, d, alpha[j]).
@@ -265,6 +369,13 @@ __device__ __forceinline__ void wg_objective_trials(const LineGeom &G, const T *
     double acc[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) acc[j] = 0;
+    if (G.nd.kind == TABLE) {
+        objective_trials_table<T, C>(G, x, d, alpha, acc);
+        block_sumk<K>(acc, r.d);
+#pragma unroll
+        for (int j = 0; j < K; ++j) phi[j] = acc[j];
+        return;
+    }
     stream<1>(
         n, [&](int i) {
             const int ip = (i + 1 == n) ? 0 : i + 1;
