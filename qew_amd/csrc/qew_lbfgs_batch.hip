@@ -124,13 +124,13 @@ template <>
 struct Acc<double> {
     double s = 0;
     __device__ void add(double p) { s += p; }
-    __device__ double reduce(Red r) { return block_sum(s, r.d); }
+    __device__ __forceinline__ double reduce(Red r) { return block_sum(s, r.d); }
 };
 template <>
 struct Acc<float> {
     KSum s{0.f, 0.f};
     __device__ void add(float p) { s = ksum_add(s, KSum{p, 0.f}); }
-    __device__ double reduce(Red r) {
+    __device__ __forceinline__ double reduce(Red r) {
         const KSum k = block_ksum(s, r.k);
         return (double)k.s + (double)k.c;
     }
@@ -184,7 +184,7 @@ __device__ __forceinline__ void block_sumk(double *v, double *red) {
 }
 
 template <typename T>
-__device__ double wg_dot(const T *a, const T *b, int n, Red r) {
+__device__ __forceinline__ double wg_dot(const T *a, const T *b, int n, Red r) {
     Acc<T> acc;
     stream(n, [&](int i) { return P2<T>{a[i], b[i]}; },
            [&](int, P2<T> v) { acc.add(v.a * v.b); });
@@ -192,7 +192,7 @@ __device__ double wg_dot(const T *a, const T *b, int n, Red r) {
 }
 
 template <typename T>
-__device__ double wg_maxabs(const T *a, int n, Red r) {
+__device__ __forceinline__ double wg_maxabs(const T *a, int n, Red r) {
     double m = 0;
     stream(n, [&](int i) { return a[i]; }, [&](int, T v) {
         const double f = fabs((double)v);
@@ -413,7 +413,7 @@ __device__ __forceinline__ void wg_objective_trials(const LineGeom &G, const T *
 }
 
 template <typename T, typename C>
-__device__ void wg_gradient(const LineGeom &G, const T *x, T *g, T *edy, T *esl) {
+__device__ __forceinline__ void wg_gradient(const LineGeom &G, const T *x, T *g, T *edy, T *esl) {
     const int n = G.n;
     const bool arc = G.model == 1;
     stream<kUG<T>>(n, [&](int i) {
@@ -543,10 +543,11 @@ __device__ void lds_fft(C2<T> *buf, int N, int sign, const C2<T> *tw, int M) {
 }
 
 // d = irfft(rfft(q) / eig), eig[k] = (kappa/dx) 4 sin^2(pi k / n) + shift
-// (src/lbfgs.h:116-146, 173-179); n = 2N. `buf` holds the packed q on entry.
+// (src/lbfgs.h:116-146, 173-179); n = 2N. `buf` holds the packed q on entry
+// and d, as n reals, on exit.
 template <typename T>
-__device__ void wg_precondition(T *d, int n, double kdx, double shift, C2<T> *buf,
-                                const C2<T> *tw, int M) {
+__device__ void wg_precondition(int n, double kdx, double shift, C2<T> *buf, const C2<T> *tw,
+                                int M) {
     const int N = n / 2;
     lds_fft(buf, N, -1, tw, M);
     const int e_w = M / n;        // W_n^k = conj(tw[k e_w])
@@ -589,17 +590,17 @@ __device__ void wg_precondition(T *d, int n, double kdx, double shift, C2<T> *bu
     }
     __syncthreads();
     lds_fft(buf, N, +1, tw, M);
+    // complex j is reals 2j, 2j + 1: each thread scales its own entries in place
     const T inv = (T)(1.0 / (double)N);
     for (int j = threadIdx.x; j < N; j += blockDim.x) {
         const C2<T> z = buf[j];
-        d[2 * j] = z.x * inv;
-        d[2 * j + 1] = z.y * inv;
+        buf[j] = {z.x * inv, z.y * inv};
     }
     __syncthreads();
 }
 
 template <typename T, typename C>
-__device__ void solve_line(const BatchArgs &a, int line, Red r, double *rho, double *la,
+__device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, double *rho, double *la,
                            int *slot, int *sbuf, C2<T> *fftbuf) {
     const qew_lbfgs_params &lp = a.lp;
     const LineDesc L = a.lines[line];
@@ -642,21 +643,30 @@ __device__ void solve_line(const BatchArgs &a, int line, Red r, double *rho, dou
     bool finished = false;
     for (int it = 0; it < lp.max_iter; ++it) {
         const int hs = nslots;
+        // With the preconditioner, q and then d live in the LDS FFT buffer
+        // (n reals) from the first pass to the second loop's last pass, which
+        // writes d to global memory: the two-loop passes read only their
+        // S and Y rows from memory. Without it, q and d are global vectors.
+        // Values and each thread's operation order are the same either way.
+        const bool pre = lp.precondition != 0;
+        T *fb = reinterpret_cast<T *>(fftbuf);
         // one pass: max|g|, q = g and the first two-loop dot S_newest . q
         double gmax = 0, al_dot = 0;
-        {
+        auto first_pass = [&](T *qv) {
             const T *S0 = hs > 0 ? Sb(sbuf[slot[hs - 1]]) : nullptr;
             Acc<T> acc;
             stream(n, [&](int i) { return P2<T>{g[i], S0 ? S0[i] : (T)0}; },
                    [&](int i, P2<T> v) {
                        const double f = fabs((double)v.a);
                        gmax = f > gmax ? f : gmax;
-                       q[i] = v.a;
+                       qv[i] = v.a;
                        if (S0) acc.add(v.b * v.a);
                    });
             gmax = block_max(gmax, r.d);
             if (S0) al_dot = acc.reduce(r);
-        }
+        };
+        if (pre) first_pass(fb);
+        else first_pass(q);
         if (gmax < lp.ftol) {
             res.iterations = it;
             res.max_force = gmax;
@@ -665,49 +675,41 @@ __device__ void solve_line(const BatchArgs &a, int line, Red r, double *rho, dou
             break;
         }
         // two-loop recursion, first loop: q -= al Y for the newest pair first;
-        // each pass also forms the next (older) pair's S . q. The last pass
-        // writes q packed into the FFT buffer when the preconditioner follows.
-        const bool pre = lp.precondition != 0;
-        for (int idx = hs - 1; idx >= 0; --idx) {
-            const int row = slot[idx];
-            const double al = rho[row] * al_dot;
-            if (tid == 0) la[idx] = al;
-            const T c = (T)(-al);
-            const T *Y = Yb(sbuf[row]);
-            const T *Sn = idx > 0 ? Sb(sbuf[slot[idx - 1]]) : nullptr;
-            Acc<T> acc;
-            stream(n, [&](int i) { return P3<T>{q[i], Y[i], Sn ? Sn[i] : (T)0}; },
-                   [&](int i, P3<T> v) {
-                       const T qn = v.a + c * v.b;
-                       if (idx == 0 && pre) {
-                           T *fb = reinterpret_cast<T *>(fftbuf);
-                           fb[i] = qn;
-                       } else {
-                           q[i] = qn;
-                       }
-                       if (Sn) acc.add(v.c * qn);
-                   });
-            if (Sn) al_dot = acc.reduce(r);
-            else __syncthreads();
-        }
-        if (pre) {
-            if (hs == 0) {
-                T *fb = reinterpret_cast<T *>(fftbuf);
-                for (int i = tid; i < n; i += blockDim.x) fb[i] = q[i];
-                __syncthreads();
+        // each pass also forms the next (older) pair's S . q
+        auto first_loop = [&](T *qv) {
+            for (int idx = hs - 1; idx >= 0; --idx) {
+                const int row = slot[idx];
+                const double al = rho[row] * al_dot;
+                if (tid == 0) la[idx] = al;
+                const T c = (T)(-al);
+                const T *Y = Yb(sbuf[row]);
+                const T *Sn = idx > 0 ? Sb(sbuf[slot[idx - 1]]) : nullptr;
+                Acc<T> acc;
+                stream(n, [&](int i) { return P3<T>{qv[i], Y[i], Sn ? Sn[i] : (T)0}; },
+                       [&](int i, P3<T> v) {
+                           const T qn = v.a + c * v.b;
+                           qv[i] = qn;
+                           if (Sn) acc.add(v.c * qn);
+                       });
+                if (Sn) al_dot = acc.reduce(r);
+                else __syncthreads();
             }
-            wg_precondition<T>(d, n, kdx, shift, fftbuf, tw, a.tw_m);
+        };
+        if (pre) {
+            first_loop(fb);
+            wg_precondition<T>(n, kdx, shift, fftbuf, tw, a.tw_m);
         } else {
+            first_loop(q);
             const T g0 = (T)(nslots == 0 ? 1.0 : gamma);
             stream(n, [&](int i) { return q[i]; }, [&](int i, T v) { d[i] = g0 * v; });
             __syncthreads();
         }
         // second loop, oldest pair first: d += (la - b) S with b = rho Y . d;
-        // the last pass also negates d and forms g . d
+        // the last pass also negates d, forms g . d and stores d to memory
         double dphi0 = 0;
-        {
+        auto second_loop = [&](T *dv) {
             double b_dot = 0;
-            if (hs > 0) b_dot = wg_dot(Yb(sbuf[slot[0]]), d, n, r);
+            if (hs > 0) b_dot = wg_dot(Yb(sbuf[slot[0]]), dv, n, r);
             for (int idx = 0; idx < hs; ++idx) {
                 const int row = slot[idx];
                 const double bb = rho[row] * b_dot;
@@ -716,20 +718,26 @@ __device__ void solve_line(const BatchArgs &a, int line, Red r, double *rho, dou
                 const bool last = idx == hs - 1;
                 const T *Yn = last ? g : Yb(sbuf[slot[idx + 1]]);
                 Acc<T> acc;
-                stream(n, [&](int i) { return P3<T>{d[i], S[i], Yn[i]}; },
-                       [&](int i, P3<T> v) {
-                           T dn = v.a + c * v.b;
-                           if (last) dn = -dn;
-                           d[i] = dn;
-                           acc.add(v.c * dn);
-                       });
+                // separate call sites for the two destinations keep each
+                // store in its own address space
+                auto pass = [&](T *dout, bool neg) {
+                    stream(n, [&](int i) { return P3<T>{dv[i], S[i], Yn[i]}; },
+                           [&](int i, P3<T> v) {
+                               T dn = v.a + c * v.b;
+                               if (neg) dn = -dn;
+                               dout[i] = dn;
+                               acc.add(v.c * dn);
+                           });
+                };
+                if (last) pass(d, true);
+                else pass(dv, false);
                 const double s = acc.reduce(r);
                 if (last) dphi0 = s;
                 else b_dot = s;
             }
             if (hs == 0) {
                 Acc<T> acc;
-                stream(n, [&](int i) { return P2<T>{d[i], g[i]}; },
+                stream(n, [&](int i) { return P2<T>{dv[i], g[i]}; },
                        [&](int i, P2<T> v) {
                            const T dn = -v.a;
                            d[i] = dn;
@@ -737,7 +745,9 @@ __device__ void solve_line(const BatchArgs &a, int line, Red r, double *rho, dou
                        });
                 dphi0 = acc.reduce(r);
             }
-        }
+        };
+        if (pre) second_loop(fb);
+        else second_loop(d);
         if (dphi0 >= 0) {
             nslots = 0;
             Acc<T> acc;
