@@ -124,7 +124,7 @@ def bench_lbfgs(log, nlines=512, plain_sample=8):
     shifts = [0.5 / xi ** 2 for xi in xis]
     h0 = torch.full((sum(lengths),), ny / 2.0, dtype=torch.float64, device="cuda:0")
     out = {}
-    finals, gmax = {}, {}
+    finals, gmax, minima = {}, {}, {}
     offs = np.concatenate([[0], np.cumsum(lengths)])
     # ftol 1e-5: above the Armijo roundoff floor of the longest lines (L=16384,
     # |E| ~ 1e4), where the reference algorithm converges in ~100-300 iterations
@@ -142,6 +142,7 @@ def bench_lbfgs(log, nlines=512, plain_sample=8):
         # energies and gradients of the minima, evaluated in FP64 on the FP64 tables
         e, g = q.Batch(lengths, n64, params).eval(h)
         finals[name] = e.cpu().numpy()
+        minima[name] = h.cpu().numpy()
         ga = g.abs().cpu().numpy()
         gmax[name] = np.array([ga[o0:o1].max() for o0, o1 in zip(offs[:-1], offs[1:])])
         its = [r.iterations for r in res]
@@ -160,6 +161,25 @@ def bench_lbfgs(log, nlines=512, plain_sample=8):
     # max |grad| per line at each precision's minima, from the FP64 gradient
     out["max_grad_fp64_at_minima"] = {k: {"median": float(np.median(v)), "max": float(v.max())}
                                       for k, v in gmax.items()}
+    # roughness of the minima: squared width per line, and the roughness
+    # exponent from the height-difference ACF averaged over the lines of one
+    # length (fit over 16 <= ell <= L/16 grid units), FP32 against FP64
+    from qew_amd import analysis
+    w2 = {k: np.array([analysis.width2(v[o0:o1]) for o0, o1 in zip(offs[:-1], offs[1:])])
+          for k, v in minima.items()}
+    w2_rel = np.abs(w2["f32"] - w2["f64"]) / w2["f64"]
+    zeta = {}
+    for L_ in sorted(set(lengths)):
+        rows = [k for k in range(nlines) if lengths[k] == L_]
+        z = {name: analysis.roughness_exponent(
+                 np.stack([v[offs[k]:offs[k + 1]] for k in rows]), float(L_), 16.0, L_ / 16.0)
+             for name, v in minima.items()}
+        zeta[str(L_)] = {"f64": z["f64"], "f32": z["f32"], "abs_err": abs(z["f32"] - z["f64"]),
+                         "lines": len(rows)}
+    out["f32_vs_f64_roughness"] = {
+        "W2_rel_err": {"median": float(np.median(w2_rel)), "max": float(w2_rel.max())},
+        "zeta_by_length": zeta,
+        "zeta_max_abs_err": max(v["abs_err"] for v in zeta.values())}
     # plain FP64 host-driven path, one line at a time, on a sample of lines
     idx = list(range(0, nlines, max(1, nlines // plain_sample)))[:plain_sample]
     t2 = time.perf_counter()
@@ -189,7 +209,8 @@ def bench_langevin(log, nsteps=100000):
     parameterisation (drift = T dt, diffusion = T sqrt(2 dt) for A = tau = 1,
     with the drivers' one-step lag, src/qew_runs.h:154-170), Philox normals
     keyed by (seed 2, step, site). FP64 and FP32 (anchored offsets, FP32
-    table) run the same noise; the mean velocity and W^2 are compared."""
+    table) run the same noise; the mean velocity, W^2 and the roughness
+    exponent of the final line are compared."""
     import torch
     import qew_amd as q
     Lc, ny = 65536, 4096
@@ -225,14 +246,19 @@ def bench_langevin(log, nsteps=100000):
         torch.cuda.synchronize()
         sec = time.perf_counter() - t1
         hh = h.double().cpu().numpy() + (ny / 2.0 if dtype == q.F32 else 0.0)
+        # roughness exponent of the final line from its height-difference
+        # ACF over 16 <= ell <= 4096 grid units
+        from qew_amd import analysis
         out[name] = {"site_steps_per_s": nsteps * Lc / sec, "us_per_step": 1e6 * sec / nsteps,
                      "seconds": sec, "mean_velocity": float((hh.mean() - ny / 2.0) / total),
-                     "W2": float(hh.var())}
+                     "W2": float(hh.var()),
+                     "zeta": float(analysis.roughness_exponent(hh, float(Lc), 16.0, 4096.0))}
         log(f"langevin {name}: {json.dumps(out[name])}")
     out["f32_vs_f64_rel_err"] = {
         "mean_velocity": abs(out["f32"]["mean_velocity"] - out["f64"]["mean_velocity"])
         / abs(out["f64"]["mean_velocity"]),
-        "W2": abs(out["f32"]["W2"] - out["f64"]["W2"]) / out["f64"]["W2"]}
+        "W2": abs(out["f32"]["W2"] - out["f64"]["W2"]) / out["f64"]["W2"],
+        "zeta_abs": abs(out["f32"]["zeta"] - out["f64"]["zeta"])}
     out["workload"] = (f"L={Lc}, table {Lc}x{ny}, xi=4, linear, f=0.5*f_c(est)={f:g}, "
                        f"dt={dt}, {nsteps} steps, T 0.5 -> 0.01")
     log(f"langevin: {json.dumps(out)}")
