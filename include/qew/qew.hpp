@@ -10,6 +10,7 @@
 //                                      SinusoidNoise, AnalyticNoise
 //   objective / gradient               qew::objective / qew::gradient
 //   lbfgs_minimize                     qew::lbfgs_minimize
+//   fire_minimize                      qew::fire_minimize
 //   LangevinDynamics                   qew::LangevinDynamics
 //   rk_site_velocity / rk_block        qew::rk_block
 //   static_sweep / dynamic_anneal /
@@ -325,6 +326,40 @@ LbfgsResult lbfgs_minimize(const View1D &h, const Params &p, const Noise &noise,
     check(qew_lbfgs_minimize(context(), as_device(noise).handle(), &c, &l, (int)h.extent(0),
                              h.data(), &r));
     return from_c(r);
+}
+
+// ---- FIRE (src/fire.h) ----------------------------------------------------------
+struct FireParams {
+    real_t dt_max = 0;  // required, > 0
+    real_t dt_init = 0;
+    real_t ftol = 1e-6;
+    int max_iter = 100000;
+    int n_min = 5;
+    real_t f_inc = 1.1;
+    real_t f_dec = 0.5;
+    real_t alpha_start = 0.1;
+    real_t f_alpha = 0.99;
+    qew_fire_params c() const {
+        return qew_fire_params{dt_max, dt_init, ftol,        max_iter, n_min,
+                               f_inc,  f_dec,   alpha_start, f_alpha};
+    }
+};
+
+struct FireResult {
+    int iterations = 0;
+    real_t max_force = 0;
+    bool converged = false;
+};
+
+template <class Noise>
+FireResult fire_minimize(const View1D &h, const Params &p, const Noise &noise,
+                         const FireParams &fp) {
+    const qew_params c = p.c();
+    const qew_fire_params f = fp.c();
+    qew_fire_result r{};
+    check(qew_fire_minimize(context(), as_device(noise).handle(), &c, &f, (int)h.extent(0),
+                            h.data(), &r));
+    return FireResult{r.iterations, r.max_force, r.converged != 0};
 }
 
 // ---- Langevin (src/dynamics.h) -------------------------------------------------

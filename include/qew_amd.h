@@ -100,6 +100,27 @@ typedef struct qew_lbfgs_result {
     long n_ls_restarts;
 } qew_lbfgs_result;
 
+/* qew::FireParams (src/fire.h:23-33); same defaults via
+ * qew_fire_default_params, except that dt_max has none and must be > 0. */
+typedef struct qew_fire_params {
+    double dt_max;
+    double dt_init;
+    double ftol;
+    int max_iter;
+    int n_min;
+    double f_inc;
+    double f_dec;
+    double alpha_start;
+    double f_alpha;
+} qew_fire_params;
+
+/* qew::FireResult (src/fire.h:35-39). */
+typedef struct qew_fire_result {
+    int iterations;
+    double max_force;
+    int converged;
+} qew_fire_result;
+
 /* qew::RkParams / qew::RkResult (src/rosso_krauth.h:34-49). */
 typedef struct qew_rk_params {
     double dstep;
@@ -244,6 +265,20 @@ qew_status qew_lbfgs_minimize(qew_ctx ctx, qew_noise noise, const qew_params *p,
  * of nlines. Line lengths must be powers of two when lp->precondition. */
 qew_status qew_batch_lbfgs(qew_batch batch, int dtype, const qew_lbfgs_params *lp,
                            double *h, qew_lbfgs_result *results);
+
+/* ---- FIRE (src/fire.h:41-131) ------------------------------------------- */
+
+qew_fire_params qew_fire_default_params(void);
+/* fire_minimize(h, p, noise, fp): one line, FP64, h (device FP64[n]) relaxed
+ * in place; the whole relaxation runs in one launch. */
+qew_status qew_fire_minimize(qew_ctx ctx, qew_noise noise, const qew_params *p,
+                             const qew_fire_params *fp, int n, double *h,
+                             qew_fire_result *result);
+/* fire_minimize on every line of the batch in one persistent launch (one
+ * workgroup per line). h: device FP64 line state in/out; results: host array
+ * of nlines. Tables may be FP64 or FP32 (read as FP64 values). */
+qew_status qew_batch_fire(qew_batch batch, const qew_fire_params *fp, double *h,
+                          qew_fire_result *results);
 
 /* ---- Langevin (src/dynamics.h:27-70) ------------------------------------ */
 

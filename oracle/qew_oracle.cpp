@@ -22,6 +22,9 @@
 //       stream drawn before the mask (statistical + structural checks only)
 //   L-BFGS                               : lbfgs_golden e_min/h_min, harmonic
 //       and flat minima, branch tests, preconditioner closed form
+//   FIRE                                 : flat and harmonic minima, energy
+//       decrease, agreement with L-BFGS, the first step against its closed
+//       form; iteration counts unpinned (no reference trajectory available)
 //   Philox4x32-10                        : published known-answer vectors
 //   Langevin                             : equipartition, T=0 descent,
 //       force-shifted equilibrium
@@ -89,6 +92,18 @@ struct orc_lbfgs_result {
     long n_func_evals;
     long n_curvature_skips;
     long n_ls_restarts;
+};
+
+struct orc_fire_params {
+    double dt_max, dt_init, ftol;
+    int max_iter, n_min;
+    double f_inc, f_dec, alpha_start, f_alpha;
+};
+
+struct orc_fire_result {
+    int iterations;
+    double max_force;
+    int converged;
 };
 
 struct orc_rk_params {
@@ -679,6 +694,58 @@ void orc_lbfgs(int n, double *h, const orc_params *p, const orc_noise *nz,
         std::swap(g, g_new);
     }
     res.iterations = lp->max_iter;
+    res.max_force = max_abs(n, g.data());
+    res.converged = 0;
+    *res_out = res;
+}
+
+// FIRE, semi-implicit Euler variant (src/fire.h:41-131): v += dt F, the power
+// P = F.v adapts dt and alpha (and zeroes v when P <= 0), v is mixed toward
+// F with weight alpha |v| / |F|, then h += dt v; F = -grad. Sums are
+// sequential in i.
+void orc_fire(int n, double *h, const orc_params *p, const orc_noise *nz,
+              const orc_fire_params *fp, orc_fire_result *res_out) {
+    std::vector<double> g(n), v(n, 0.0);
+    double dt = fp->dt_init > 0 ? fp->dt_init : 0.1 * fp->dt_max;
+    double alpha = fp->alpha_start;
+    int n_pos = 0;
+    orc_gradient(n, h, p, nz, g.data());
+    orc_fire_result res{};
+    for (int it = 0; it < fp->max_iter; ++it) {
+        const double max_f = max_abs(n, g.data());
+        if (max_f < fp->ftol) {
+            res.iterations = it; res.max_force = max_f; res.converged = 1;
+            *res_out = res;
+            return;
+        }
+        for (int i = 0; i < n; ++i) v[i] -= dt * g[i];
+        double P = 0;
+        for (int i = 0; i < n; ++i) P += -g[i] * v[i];
+        if (P > 0) {
+            if (n_pos > fp->n_min) {
+                dt = std::fmin(dt * fp->f_inc, fp->dt_max);
+                alpha *= fp->f_alpha;
+            }
+            ++n_pos;
+        } else {
+            for (int i = 0; i < n; ++i) v[i] = 0.0;
+            dt *= fp->f_dec;
+            alpha = fp->alpha_start;
+            n_pos = 0;
+        }
+        double nv2 = 0, nf2 = 0;
+        for (int i = 0; i < n; ++i) nv2 += v[i] * v[i];
+        for (int i = 0; i < n; ++i) nf2 += g[i] * g[i];
+        const double nf = std::sqrt(nf2);
+        if (nf > 0) {
+            const double a = alpha;
+            const double scale = alpha * std::sqrt(nv2) / nf;
+            for (int i = 0; i < n; ++i) v[i] = (1.0 - a) * v[i] - scale * g[i];
+        }
+        for (int i = 0; i < n; ++i) h[i] += dt * v[i];
+        orc_gradient(n, h, p, nz, g.data());
+    }
+    res.iterations = fp->max_iter;
     res.max_force = max_abs(n, g.data());
     res.converged = 0;
     *res_out = res;

@@ -11,6 +11,8 @@ the reference's C++ API:
                                     reference tests' Noise functors
   LbfgsParams, LbfgsResult,
   lbfgs_minimize                    src/lbfgs.h:32-287
+  FireParams, FireResult,
+  fire_minimize                     src/fire.h:21-131
   LangevinDynamics                  src/dynamics.h:27-70
   RkParams, RkResult, rk_block,
   rk_site_velocity                  src/rosso_krauth.h:34-194
@@ -30,7 +32,8 @@ from ._lib import (ARCLENGTH, BF16, F32, F64, LINEAR, QewError, QewInvalidArgume
                    load)
 
 __all__ = [
-    "Model", "Params", "LbfgsParams", "LbfgsResult", "RkParams", "RkResult",
+    "Model", "Params", "LbfgsParams", "LbfgsResult", "FireParams", "FireResult",
+    "fire_minimize", "RkParams", "RkResult",
     "Context", "context", "FilteredNoise", "AnalyticNoise", "ZeroNoise", "ConstNoise",
     "HarmonicWell", "SinusoidNoise", "objective", "objective_trial", "gradient",
     "evaluate", "lbfgs_minimize", "LangevinDynamics", "rk_block", "rk_site_velocity",
@@ -89,6 +92,37 @@ class LbfgsResult:
     def from_c(cls, r):
         return cls(r.iterations, r.max_force, bool(r.converged), r.n_func_evals,
                    r.n_curvature_skips, r.n_ls_restarts)
+
+
+@dataclass
+class FireParams:
+    """qew::FireParams (src/fire.h:23-33), same defaults; dt_max has no
+    default there either and must be set > 0."""
+    dt_max: float = 0.0
+    dt_init: float = 0.0
+    ftol: float = 1e-6
+    max_iter: int = 100000
+    n_min: int = 5
+    f_inc: float = 1.1
+    f_dec: float = 0.5
+    alpha_start: float = 0.1
+    f_alpha: float = 0.99
+
+    def c(self):
+        return _lib.FireParams(self.dt_max, self.dt_init, self.ftol, self.max_iter, self.n_min,
+                               self.f_inc, self.f_dec, self.alpha_start, self.f_alpha)
+
+
+@dataclass
+class FireResult:
+    """qew::FireResult (src/fire.h:35-39)."""
+    iterations: int = 0
+    max_force: float = 0.0
+    converged: bool = False
+
+    @classmethod
+    def from_c(cls, r):
+        return cls(r.iterations, r.max_force, bool(r.converged))
 
 
 @dataclass
@@ -372,6 +406,17 @@ def lbfgs_minimize(h, p, noise, lp=None):
     return LbfgsResult.from_c(r)
 
 
+def fire_minimize(h, p, noise, fp):
+    """fire_minimize (src/fire.h:41): relaxes h (FP64, on the GPU) in place;
+    the whole relaxation is one launch."""
+    torch = _torch()
+    r = _lib.FireResult()
+    pc, fc = p.c(), fp.c()
+    check(_lib.load().qew_fire_minimize(noise.ctx.handle, noise.handle, C.byref(pc), C.byref(fc),
+                                        h.numel(), _ptr(h, torch.float64), C.byref(r)))
+    return FireResult.from_c(r)
+
+
 class LangevinDynamics:
     """LangevinDynamics(n, seed) (src/dynamics.h:27-70); normals from
     Philox4x32-10 keyed by (seed, step, site)."""
@@ -475,6 +520,14 @@ class Batch:
         check(_lib.load().qew_batch_lbfgs(self.handle, dtype, C.byref(lc), _ptr(h, torch.float64),
                                           res))
         return [LbfgsResult.from_c(r) for r in res]
+
+    def fire(self, h, fp):
+        """FIRE on every line (FP64 line state h, one persistent launch)."""
+        torch = _torch()
+        res = (_lib.FireResult * self.nlines)()
+        fc = fp.c()
+        check(_lib.load().qew_batch_fire(self.handle, C.byref(fc), _ptr(h, torch.float64), res))
+        return [FireResult.from_c(r) for r in res]
 
     def langevin(self, h, drift, diff, seed, step0=0, dtype=F64, anchor=None):
         torch = _torch()

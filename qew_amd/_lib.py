@@ -33,6 +33,16 @@ class LbfgsResult(C.Structure):
                 ("n_curvature_skips", C.c_long), ("n_ls_restarts", C.c_long)]
 
 
+class FireParams(C.Structure):
+    _fields_ = [("dt_max", C.c_double), ("dt_init", C.c_double), ("ftol", C.c_double),
+                ("max_iter", C.c_int), ("n_min", C.c_int), ("f_inc", C.c_double),
+                ("f_dec", C.c_double), ("alpha_start", C.c_double), ("f_alpha", C.c_double)]
+
+
+class FireResult(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("max_force", C.c_double), ("converged", C.c_int)]
+
+
 class RkParams(C.Structure):
     _fields_ = [("dstep", C.c_double), ("max_advance", C.c_double),
                 ("root_tol", C.c_double), ("sweep_tol", C.c_double),
@@ -85,6 +95,9 @@ SIGNATURES = [
     ("qew_lbfgs_minimize", [VP, VP, C.POINTER(Params), C.POINTER(LbfgsParams), I, VP,
                             C.POINTER(LbfgsResult)]),
     ("qew_batch_lbfgs", [VP, I, C.POINTER(LbfgsParams), VP, C.POINTER(LbfgsResult)]),
+    ("qew_fire_minimize", [VP, VP, C.POINTER(Params), C.POINTER(FireParams), I, VP,
+                           C.POINTER(FireResult)]),
+    ("qew_batch_fire", [VP, C.POINTER(FireParams), VP, C.POINTER(FireResult)]),
     ("qew_langevin_step", [VP, VP, C.POINTER(Params), I, VP, D, D, U64, U64]),
     ("qew_batch_langevin", [VP, I, VP, VP, I, DP, DP, U64, U64]),
     ("qew_rk_block", [VP, VP, C.POINTER(Params), C.POINTER(RkParams), I, VP, C.POINTER(RkResult)]),
@@ -96,6 +109,7 @@ _RESTYPE = {
     "qew_last_error": C.c_char_p,
     "qew_version": C.c_char_p,
     "qew_lbfgs_default_params": LbfgsParams,
+    "qew_fire_default_params": FireParams,
 }
 
 _lib = None

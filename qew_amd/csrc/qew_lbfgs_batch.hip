@@ -22,6 +22,9 @@
 // the eig sines come from one table of exp(2 pi i j / M), M = 4 * the
 // largest transform length, filled once per launch.
 //
+// The same file holds the batched FIRE relaxation (src/fire.h), which reuses
+// the per-line gradient, element loops and reductions.
+//
 // Precision modes: FP64 (the reference's arithmetic; sums are workgroup tree
 // sums) and FP32 (line offsets from a per-line anchor, FP32 tables, FP32
 // vectors, Kahan-compensated FP32 dot products and energies) in which the
@@ -907,6 +910,120 @@ __global__ void twiddle_kernel(C2<T> *tw, int M) {
 
 __global__ void zero_u32(unsigned *p) { *p = 0u; }
 
+// ---- FIRE (src/fire.h:41-131), one workgroup per line ----
+// The reference's passes per iteration are max|F|; v += dt F; P = F.v;
+// (v = 0 on P <= 0); |v|^2, |F|^2; mix; h += dt v; gradient. Here |F|^2 is
+// summed in the max|F| pass (F does not change in between), |v|^2 in the P
+// pass (v changes after it only when it is zeroed, and then |v|^2 = 0), and
+// the zeroing, the mix and the position step are one element-wise pass.
+// Every element update is the reference's expression; the sums are
+// workgroup tree sums.
+struct FireArgs {
+    const LineDesc *lines;
+    const NoiseDesc *noises;
+    const int *order;
+    int nlines;
+    unsigned *next;
+    double *h;
+    double *work;
+    qew_fire_result *results;
+    qew_fire_params fp;
+};
+
+template <typename C>
+__device__ __forceinline__ void fire_line(const FireArgs &a, int line, Red r) {
+    const qew_fire_params &fp = a.fp;
+    const LineDesc L = a.lines[line];
+    const NoiseDesc nd = a.noises[L.noise];
+    const int n = L.n;
+    const LineGeom G = make_geom(L, nd, 0.0);
+    double *x = a.h + L.off;
+    double *ws = a.work + (size_t)L.off * 4;
+    double *g = ws, *v = ws + n, *edy = ws + 2 * (size_t)n, *esl = ws + 3 * (size_t)n;
+    // every pass maps element i to thread i mod kLB, so element-wise passes
+    // need no barrier between them; the gradient reads neighbours
+    stream(n, [&](int) { return 0; }, [&](int i, int) { v[i] = 0.0; });
+    double dt = fp.dt_init > 0 ? fp.dt_init : 0.1 * fp.dt_max;
+    double alpha = fp.alpha_start;
+    int n_pos = 0;
+    __syncthreads();
+    wg_gradient<double, C>(G, x, g, edy, esl);
+    qew_fire_result res{};
+    res.iterations = fp.max_iter;
+    res.converged = 0;
+    bool done = false;
+    for (int it = 0; it < fp.max_iter; ++it) {
+        double max_f = 0, nf2 = 0;
+        stream(n, [&](int i) { return g[i]; }, [&](int, double gi) {
+            const double f = fabs(gi);
+            max_f = f > max_f ? f : max_f;
+            nf2 += gi * gi;
+        });
+        max_f = block_max(max_f, r.d);
+        if (max_f < fp.ftol) {
+            res.iterations = it;
+            res.max_force = max_f;
+            res.converged = 1;
+            done = true;
+            break;
+        }
+        nf2 = block_sum(nf2, r.d);
+        double P = 0, nv2 = 0, unused = 0;
+        stream(n, [&](int i) { return P2<double>{g[i], v[i]}; }, [&](int i, P2<double> e) {
+            const double vi = e.b - dt * e.a;
+            v[i] = vi;
+            P += -e.a * vi;
+            nv2 += vi * vi;
+        });
+        block_sum3(P, nv2, unused, r.d);
+        const bool zero = !(P > 0);
+        if (!zero) {
+            if (n_pos > fp.n_min) {
+                dt = fmin(dt * fp.f_inc, fp.dt_max);
+                alpha *= fp.f_alpha;
+            }
+            ++n_pos;
+        } else {
+            dt *= fp.f_dec;
+            alpha = fp.alpha_start;
+            n_pos = 0;
+            nv2 = 0.0;
+        }
+        const double nf = sqrt(nf2);
+        const bool mix = nf > 0;
+        const double am = alpha;
+        const double scale = mix ? alpha * sqrt(nv2) / nf : 0.0;
+        stream(n, [&](int i) { return P3<double>{g[i], v[i], x[i]}; },
+               [&](int i, P3<double> e) {
+                   double vi = zero ? 0.0 : e.b;
+                   if (mix) vi = (1.0 - am) * vi - scale * e.a;
+                   v[i] = vi;
+                   x[i] = e.c + dt * vi;
+               });
+        __syncthreads();
+        wg_gradient<double, C>(G, x, g, edy, esl);
+    }
+    if (!done) res.max_force = wg_maxabs(g, n, r);
+    if (threadIdx.x == 0) a.results[line] = res;
+    __syncthreads();
+}
+
+template <typename C>
+__global__ __launch_bounds__(kLB) void fire_batch_kernel(FireArgs a) {
+    __shared__ double s_red[kLS * (kLB / kWave)];
+    __shared__ KSum s_kred[kLB / kWave];
+    __shared__ int s_line;
+    Red r{s_red, s_kred};
+    for (;;) {
+        if (threadIdx.x == 0) s_line = (int)atomicAdd(a.next, 1u);
+        __syncthreads();
+        const int qi = s_line;
+        __syncthreads();
+        if (qi >= a.nlines) return;
+        fire_line<C>(a, a.order[qi], r);
+    }
+}
+
 }  // namespace
 
 using namespace qew_host;
@@ -994,5 +1111,94 @@ extern "C" qew_status qew_batch_lbfgs(qew_batch b, int dtype, const qew_lbfgs_pa
         QEW_HIP(hipMemcpyAsync(results, a.results, sizeof(qew_lbfgs_result) * b->nlines,
                                hipMemcpyDeviceToHost, ctx->stream));
         QEW_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+qew_fire_params qew_fire_default_params(void) {
+    qew_fire_params p;
+    p.dt_max = 0;
+    p.dt_init = 0;
+    p.ftol = 1e-6;
+    p.max_iter = 100000;
+    p.n_min = 5;
+    p.f_inc = 1.1;
+    p.f_dec = 0.5;
+    p.alpha_start = 0.1;
+    p.f_alpha = 0.99;
+    return p;
+}
+
+extern "C" qew_status qew_batch_fire(qew_batch b, const qew_fire_params *fp, double *h,
+                                     qew_fire_result *results) {
+    return guard([&] {
+        QEW_REQUIRE(b && fp && h && results, "null pointer");
+        QEW_REQUIRE(fp->dt_max > 0, "FireParams.dt_max must be > 0");
+        QEW_REQUIRE(fp->max_iter >= 0, "FireParams.max_iter must be >= 0");
+        require_wide_tables(b->noises);
+        const int tdt = b->noises[0].kind == TABLE ? b->noises[0].dtype : 0;
+        for (const auto &nd : b->noises)
+            QEW_REQUIRE(nd.kind != TABLE || nd.dtype == tdt,
+                        "all tables of a batch must share one storage type");
+        qew_ctx ctx = b->ctx;
+        use_device(ctx);
+        // layout: [order nlines ints][counter][results][work: g v edy esl]
+        auto up = [](size_t v) { return (v + 255) / 256 * 256; };
+        const size_t off_next = up(sizeof(int) * b->nlines);
+        const size_t off_res = off_next + 256;
+        const size_t off_work = off_res + up(sizeof(qew_fire_result) * b->nlines);
+        b->d_scratch.ensure(off_work + sizeof(double) * 4 * (size_t)b->total);
+        char *w = b->d_scratch.as<char>();
+        std::vector<int> order(b->nlines);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int x, int y) { return b->lines[x].n > b->lines[y].n; });
+        QEW_HIP(hipMemcpyAsync(w, order.data(), sizeof(int) * b->nlines, hipMemcpyHostToDevice,
+                               ctx->stream));
+        hipLaunchKernelGGL(zero_u32, dim3(1), dim3(1), 0, ctx->stream,
+                           reinterpret_cast<unsigned *>(w + off_next));
+        QEW_HIP(hipGetLastError());
+        FireArgs a{};
+        a.lines = b->d_lines.as<LineDesc>();
+        a.noises = b->d_noises.as<NoiseDesc>();
+        a.order = reinterpret_cast<int *>(w);
+        a.nlines = b->nlines;
+        a.next = reinterpret_cast<unsigned *>(w + off_next);
+        a.h = h;
+        a.work = reinterpret_cast<double *>(w + off_work);
+        a.results = reinterpret_cast<qew_fire_result *>(w + off_res);
+        a.fp = *fp;
+        auto launch = [&](auto kernel) {
+            int per_cu = 0;
+            QEW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kLB, 0));
+            const int grid = std::max(1, std::min(b->nlines, std::max(1, per_cu) * ctx->num_cus));
+            hipLaunchKernelGGL(kernel, dim3(grid), dim3(kLB), 0, ctx->stream, a);
+            QEW_HIP(hipGetLastError());
+        };
+        if (tdt == 1) launch(fire_batch_kernel<float>);
+        else launch(fire_batch_kernel<double>);
+        QEW_HIP(hipMemcpyAsync(results, a.results, sizeof(qew_fire_result) * b->nlines,
+                               hipMemcpyDeviceToHost, ctx->stream));
+        QEW_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+extern "C" qew_status qew_fire_minimize(qew_ctx ctx, qew_noise noise, const qew_params *p,
+                                        const qew_fire_params *fp, int n, double *h,
+                                        qew_fire_result *result) {
+    return guard([&] {
+        ctx_check(ctx);
+        QEW_REQUIRE(noise && p && fp && h && result, "null pointer");
+        QEW_REQUIRE(n >= 1, "line length must be >= 1");
+        QEW_REQUIRE(fp->dt_max > 0, "FireParams.dt_max must be > 0");
+        auto check = [](qew_status s) {
+            if (s != QEW_OK) throw Error(s, qew_last_error());
+        };
+        qew_batch b = nullptr;
+        check(qew_batch_create(ctx, 1, &n, &noise, p, &b));
+        struct Release {
+            qew_batch b;
+            ~Release() { (void)qew_batch_destroy(b); }
+        } release{b};
+        check(qew_batch_fire(b, fp, h, result));
     });
 }

@@ -212,6 +212,22 @@ static void facade_semantics() {
     const LbfgsResult r = lbfgs_minimize(w, lp, HarmonicWell{2.0, 0.5}, opt);
     CHECK(r.converged);
     for (double v : w.to_host()) CHECK(std::abs(v - 0.5) < 1e-6);
+    // FIRE reaches the same minimum (tests/test_fire.cpp:82-108)
+    View1D fw("fw", 96);
+    fw.fill(0.0);
+    FireParams fopt;
+    fopt.dt_max = 0.1;
+    fopt.ftol = 1e-9;
+    const FireResult fr = fire_minimize(fw, lp, HarmonicWell{2.0, 0.5}, fopt);
+    CHECK(fr.converged && fr.max_force < 1e-9);
+    for (double v : fw.to_host()) CHECK(std::abs(v - 0.5) < 1e-6);
+    bool fire_threw = false;
+    try {
+        fire_minimize(fw, lp, HarmonicWell{2.0, 0.5}, FireParams{});  // dt_max missing
+    } catch (const std::invalid_argument &) {
+        fire_threw = true;
+    }
+    CHECK(fire_threw);
     // LangevinDynamics at zero diffusion is seed independent (tests/test_dynamics.cpp:137-161)
     View1D a("a", 48), b("b", 48);
     a.fill(0.3);

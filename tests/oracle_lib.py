@@ -40,6 +40,23 @@ class LbfgsResult(C.Structure):
                 ("n_curvature_skips", C.c_long), ("n_ls_restarts", C.c_long)]
 
 
+class FireParams(C.Structure):
+    """qew::FireParams (src/fire.h:23-33) with its defaults; dt_max has none."""
+    _fields_ = [("dt_max", C.c_double), ("dt_init", C.c_double), ("ftol", C.c_double),
+                ("max_iter", C.c_int), ("n_min", C.c_int), ("f_inc", C.c_double),
+                ("f_dec", C.c_double), ("alpha_start", C.c_double), ("f_alpha", C.c_double)]
+
+    def __init__(self, dt_max, **kw):
+        d = dict(dt_max=dt_max, dt_init=0.0, ftol=1e-6, max_iter=100000, n_min=5, f_inc=1.1,
+                 f_dec=0.5, alpha_start=0.1, f_alpha=0.99)
+        d.update(kw)
+        super().__init__(**d)
+
+
+class FireResult(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("max_force", C.c_double), ("converged", C.c_int)]
+
+
 class RkParams(C.Structure):
     _fields_ = [("dstep", C.c_double), ("max_advance", C.c_double),
                 ("root_tol", C.c_double), ("sweep_tol", C.c_double),
@@ -84,6 +101,7 @@ def load(omp=False):
     lib.orc_precond_eig.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, _DP]
     lib.orc_precondition.argtypes = [C.c_int, _DP, _DP, _DP]
     lib.orc_lbfgs.argtypes = [C.c_int, _DP, P(Params), P(Noise), P(LbfgsParams), P(LbfgsResult)]
+    lib.orc_fire.argtypes = [C.c_int, _DP, P(Params), P(Noise), P(FireParams), P(FireResult)]
     lib.orc_philox4x32_10.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
     lib.orc_normal.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
     lib.orc_normal.restype = C.c_double
@@ -213,6 +231,13 @@ def lbfgs(h, p, noise, lp, omp=False):
     h = np.array(h, dtype=np.float64)
     r = LbfgsResult()
     load(omp).orc_lbfgs(len(h), dp(h), C.byref(p), noise.ref, C.byref(lp), C.byref(r))
+    return h, r
+
+
+def fire(h, p, noise, fp):
+    h = np.array(h, dtype=np.float64)
+    r = FireResult()
+    load().orc_fire(len(h), dp(h), C.byref(p), noise.ref, C.byref(fp), C.byref(r))
     return h, r
 
 

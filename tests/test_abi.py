@@ -14,13 +14,14 @@ LIB = os.path.join(ROOT, "qew_amd", "libqew_amd.so")
 
 def declared():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:const char \*|qew_status|qew_lbfgs_params)\s*(qew_\w+)\s*\(",
+    return sorted(set(re.findall(r"^\s*(?:const char \*|qew_status|qew_lbfgs_params|qew_fire_params)\s*(qew_\w+)\s*\(",
                                  text, flags=re.M)))
 
 
 def test_header_declares_entry_points():
     names = declared()
     assert "qew_eval" in names and "qew_batch_lbfgs" in names and "qew_rk_block" in names
+    assert "qew_fire_minimize" in names and "qew_fire_default_params" in names
     assert len(names) >= 30
 
 
@@ -50,3 +51,9 @@ def test_errors_without_device_are_reported_not_raised_across_abi():
     assert b"null context" in lib.qew_last_error()
     p = lib.qew_lbfgs_default_params()
     assert (p.m, p.ftol, p.max_iter, p.max_ls) == (8, 1e-6, 10000, 30)
+    f = lib.qew_fire_default_params()
+    # src/fire.h:23-33 defaults; dt_max has none
+    assert (f.dt_max, f.dt_init, f.ftol, f.max_iter, f.n_min) == (0.0, 0.0, 1e-6, 100000, 5)
+    assert (f.f_inc, f.f_dec, f.alpha_start, f.f_alpha) == (1.1, 0.5, 0.1, 0.99)
+    st = lib.qew_fire_minimize(None, None, None, None, 4, None, None)
+    assert st == _lib.QEW_ERR_INVALID_ARGUMENT and b"null context" in lib.qew_last_error()

@@ -184,6 +184,68 @@ def test_lbfgs_preconditioner_on_stiff_line():
     assert r.iterations < r2.iterations
 
 
+# ---- FIRE (src/fire.h; reference tests/test_fire.cpp, tests/test_lbfgs.cpp:246-271) ----
+
+def _fire_params(**kw):
+    return O.FireParams(0.1 * math.sqrt(1.0 / 1.0), **kw)
+
+
+def test_fire_relaxes_to_flat_line():
+    n = 128
+    h0 = 0.5 + 0.3 * np.cos(4 * np.pi * np.arange(n) / n)
+    h, r = O.fire(h0, O.params(n, 1.0, 0.0, O.LINEAR), ZERO, _fire_params(ftol=1e-9))
+    assert r.converged and r.max_force < 1e-9 and np.ptp(h) < 1e-6
+
+
+@pytest.mark.parametrize("model", [O.LINEAR, O.ARCLENGTH])
+def test_fire_harmonic_minimum(model):
+    h, r = O.fire(np.zeros(96), O.params(96, 1.0, 0.0, model), harmonic(2.0, 0.5),
+                  _fire_params(ftol=1e-9))
+    assert r.converged and np.max(np.abs(h - 0.5)) < 1e-6
+
+
+def test_fire_lowers_energy():
+    rng = np.random.default_rng(1)
+    h0 = rng.random(96)
+    p = O.params(96, 1.0, 0.0, O.ARCLENGTH)
+    nz = harmonic(1.5, 0.6)
+    h, r = O.fire(h0, p, nz, _fire_params(ftol=1e-9))
+    assert r.converged and O.objective(h, p, nz) <= O.objective(h0, p, nz) + 1e-12
+
+
+def test_fire_reports_nonconvergence():
+    h, r = O.fire(np.zeros(64), O.params(64, 1.0, 0.0, O.LINEAR), harmonic(2.0, 0.5),
+                  _fire_params(ftol=1e-12, max_iter=3))
+    assert not r.converged and r.iterations == 3 and r.max_force >= 1e-12
+
+
+def test_fire_first_step_closed_form():
+    """From rest, v = -dt g is parallel to F, so the mixing leaves it
+    unchanged and the first step is h1 = h0 - dt^2 g(h0), dt = 0.1 dt_max."""
+    rng = np.random.default_rng(4)
+    h0 = rng.random(24)
+    p = O.params(24, 1.3, 0.05, O.LINEAR)
+    nz = harmonic(0.7, 0.2)
+    fp = O.FireParams(0.3, ftol=0.0, max_iter=1)
+    h, r = O.fire(h0, p, nz, fp)
+    dt = 0.03
+    expect = h0 - dt * dt * O.gradient(h0, p, nz)
+    assert r.iterations == 1 and np.max(np.abs(h - expect)) < 1e-15
+    # dt_init, when given, replaces 0.1 dt_max
+    h, _ = O.fire(h0, p, nz, O.FireParams(0.3, dt_init=0.2, ftol=0.0, max_iter=1))
+    assert np.max(np.abs(h - (h0 - 0.04 * O.gradient(h0, p, nz)))) < 1e-15
+
+
+def test_fire_agrees_with_lbfgs():
+    rng = np.random.default_rng(3)
+    h0 = rng.random(96)
+    p = O.params(96, 1.0, 0.0, O.LINEAR)
+    nz = harmonic(1.5, 0.4)
+    ha, _ = O.lbfgs(h0, p, nz, O.lbfgs_params(ftol=1e-9))
+    hb, r = O.fire(h0, p, nz, _fire_params(ftol=1e-9))
+    assert r.converged and np.max(np.abs(ha - hb)) < 1e-5
+
+
 # ---- Philox / Langevin --------------------------------------------------------
 
 def test_philox4x32_10_known_answers():
