@@ -29,7 +29,7 @@ const char *kUsage =
     "  --f_max           force ramp cap         (default 40)\n"
     "  --fc_rel_tol      bisect f_c to this relative width; 0 = ramp only (default 0)\n"
     "  --seed            noise RNG seed          (default 1)\n"
-    "  --output          output file            (default depinning_<model>.kv)\n"
+    "  --output          output file            (default depinning_<model>.h5; .kv: text)\n"
     "  RK solver knobs (defaults derived from Ly/ny):\n"
     "  --dstep --max_advance --runaway --root_tol --sweep_tol --max_sweeps\n";
 
@@ -61,7 +61,7 @@ int run(int argc, char **argv) {
     cfg.rp.sweep_tol = args.dbl("sweep_tol", 1e-8);
     cfg.rp.max_sweeps = args.integer("max_sweeps", 200000);
     const std::uint64_t seed = args.u64("seed", 1);
-    const std::string fname = args.str("output", std::string("depinning_") + model_name + ".kv");
+    const std::string fname = args.str("output", std::string("depinning_") + model_name + ".h5");
     args.require_all_used();
 
     initialize();
@@ -81,8 +81,7 @@ int run(int argc, char **argv) {
         std::printf("f_c = %.12g  in [%.12g, %.12g] after %d bisection probes\n",
                     static_cast<double>(res.f_c), static_cast<double>(res.f_c_lower),
                     static_cast<double>(res.f_c_upper), res.bisection_probes);
-    KvWriter w(fname);
-    write_depinning(w, cfg, res);
+    write_output(fname, [&](auto &w) { write_depinning(w, cfg, res); });
     std::printf("wrote %s\n", fname.c_str());
     return 0;
 }

@@ -31,7 +31,7 @@ const char *kUsage =
     "  --total_time      integration time       (default 1000; production ~1e6)\n"
     "  --dt              fixed timestep; 0 = heuristic (default 0)\n"
     "  --seed            noise/RNG seed          (default 1)\n"
-    "  --output          output file            (default dynamic_<model>.kv)\n";
+    "  --output          output file            (default dynamic_<model>.h5; .kv: text)\n";
 
 int run(int argc, char **argv) {
     cli::Args args(argc, argv);
@@ -58,7 +58,7 @@ int run(int argc, char **argv) {
     cfg.tl_end = args.dbl("tl_end", 0.0001);
     cfg.total_time = args.dbl("total_time", 1000.0);
     cfg.dt = args.dbl("dt", 0.0);
-    const std::string fname = args.str("output", std::string("dynamic_") + model_name + ".kv");
+    const std::string fname = args.str("output", std::string("dynamic_") + model_name + ".h5");
     args.require_all_used();
 
     initialize();
@@ -69,8 +69,7 @@ int run(int argc, char **argv) {
                 static_cast<long>(res.total_time / res.dt), static_cast<double>(res.line_tension));
     for (const DynamicSnapshot &snap : res.snapshots)
         std::printf("  saved t=%.6g\n", static_cast<double>(snap.t));
-    KvWriter w(fname);
-    write_dynamic(w, cfg, res);
+    write_output(fname, [&](auto &w) { write_dynamic(w, cfg, res); });
     std::printf("wrote %s\n", fname.c_str());
     return 0;
 }

@@ -1,8 +1,8 @@
 // qew_static: L-BFGS relaxation of a flat line for a sweep of pinning lengths
 // (the reference's executables/qew_static/main.cpp:30-116: same options, same
 // defaults, same summary lines, exit code 2 on a bad option). Runs on the
-// MI355X through libqew_amd. Output: the reference's dataset keys in a qew-kv
-// text archive. Extra option: --m (L-BFGS history length, default 8 as in the
+// MI355X through libqew_amd. Output: the reference's dataset keys in an HDF5
+// file (a qew-kv text archive when --output ends in .kv). Extra option: --m (L-BFGS history length, default 8 as in the
 // reference's LbfgsParams).
 #include <cstdint>
 #include <cstdio>
@@ -31,7 +31,7 @@ const char *kUsage =
     "  --max_iter        L-BFGS iteration cap   (default 5000)\n"
     "  --m               L-BFGS history length  (default 8)\n"
     "  --seed            noise RNG seed         (default 1)\n"
-    "  --output          output file            (default static_<model>.kv)\n";
+    "  --output          output file            (default static_<model>.h5; .kv: text)\n";
 
 Model parse_model(const std::string &name) {
     if (name == "linear") return Model::Linear;
@@ -60,7 +60,7 @@ int run(int argc, char **argv) {
     cfg.max_iter = args.integer("max_iter", 5000);
     cfg.m = args.integer("m", 8);
     const std::uint64_t seed = args.u64("seed", 1);
-    const std::string fname = args.str("output", std::string("static_") + model_name + ".kv");
+    const std::string fname = args.str("output", std::string("static_") + model_name + ".h5");
     args.require_all_used();
 
     initialize();
@@ -71,8 +71,7 @@ int run(int argc, char **argv) {
                     prof.pinning_length_over_xi, static_cast<double>(prof.line_tension),
                     prof.result.iterations, static_cast<double>(prof.result.max_force),
                     prof.result.converged ? "converged" : "NOT converged");
-    KvWriter w(fname);
-    write_static(w, cfg, profiles);
+    write_output(fname, [&](auto &w) { write_static(w, cfg, profiles); });
     std::printf("wrote %s\n", fname.c_str());
     return 0;
 }

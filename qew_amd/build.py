@@ -63,6 +63,22 @@ def build(jobs=8, apps=True, verbose=False):
     return LIB
 
 
+def hdf5_flags():
+    """Compiler flags that give the tools HDF5 output: the HDF5 headers (searched
+    after the system's, so nothing else in that prefix shadows them) and the
+    library the tools dlopen by default. HDF5 is never linked: the tools load it
+    at run time and fall back to an error naming it when it is absent. The
+    location is $QEW_HDF5_ROOT, else the first prefix that holds include/hdf5.h
+    and lib/libhdf5.so."""
+    roots = [os.environ.get("QEW_HDF5_ROOT", ""), "/usr/local", "/opt/conda"]
+    for r in roots:
+        if r and os.path.exists(os.path.join(r, "include", "hdf5.h")) and \
+                os.path.exists(os.path.join(r, "lib", "libhdf5.so")):
+            lib = os.path.join(r, "lib", "libhdf5.so")
+            return ["-idirafter", os.path.join(r, "include"), f'-DQEW_HDF5_LIBRARY="{lib}"']
+    return []
+
+
 def build_apps(jobs=8):
     if not os.path.isdir(APPS):
         return []
@@ -73,8 +89,8 @@ def build_apps(jobs=8):
     def one(f):
         out = os.path.join(BIN, f[:-4])
         _run(["g++", "-O2", "-std=c++17", "-I", inc, "-I", "/opt/rocm/include",
-              "-D__HIP_PLATFORM_AMD__", os.path.join(APPS, f), "-o", out,
-              "-L", PKG, "-lqew_amd", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/.."])
+              "-D__HIP_PLATFORM_AMD__", *hdf5_flags(), os.path.join(APPS, f), "-o", out,
+              "-L", PKG, "-lqew_amd", "-ldl", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/.."])
         return out
 
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:

@@ -56,13 +56,14 @@ def test_facade_and_drivers_selftest(tmp_path):
 
 @pytest.mark.gpu
 def test_tools_write_reference_keys(tmp_path):
-    from qew_amd.io import read_kv
-    out = tmp_path / "s.kv"
+    from qew_amd.io import read_kv, read_output
+    # no --output: the reference's default name, static_<model>.h5, in HDF5
     r = subprocess.run([tool("qew_static"), "--nx", "256", "--ny", "64", "--Lx", "2.56",
-                        "--pinning_lengths", "1,10", "--model", "linear", "--output", str(out),
-                        "--kokkos-num-threads=4"], capture_output=True, text=True, timeout=300)
+                        "--pinning_lengths", "1,10", "--model", "linear",
+                        "--kokkos-num-threads=4"], capture_output=True, text=True, timeout=300,
+                       cwd=tmp_path)
     assert r.returncode == 0, r.stderr
-    kv = read_kv(out)
+    kv = read_output(tmp_path / "static_linear.h5")
     assert kv["physical_size"] == 2.56
     for lp in ("1", "10"):
         assert kv[f"driving_force=0/pinning_length={lp}/pinning_length"] == float(lp)
@@ -78,12 +79,12 @@ def test_tools_write_reference_keys(tmp_path):
     assert snaps and all(k.startswith("timestep=") for k in snaps)
     assert all(np.all(np.isfinite(kv[k])) for k in snaps)
 
-    out = tmp_path / "p.kv"
+    out = tmp_path / "p.h5"
     r = subprocess.run([tool("qew_depinning"), "--nx", "64", "--ny", "64", "--f_step", "2",
                         "--fc_rel_tol", "1e-3", "--output", str(out)],
                        capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr
-    kv = read_kv(out)
+    kv = read_output(out)
     assert "f_c_upper_bound" in kv and kv["driving_force=0/h"].shape == (64,)
     assert kv["f_c_lower"] <= kv["f_c"] <= kv["f_c_upper"] <= kv["f_c_upper_bound"]
     assert "Line depinned near" in r.stdout
