@@ -550,6 +550,57 @@ __device__ __forceinline__ double philox_normal(uint64_t seed, uint64_t step, ui
     return (site & 1) ? zs : zc;
 }
 
+// The same two normals with the Box-Muller transform in FP32, for kernels
+// whose line state is FP32: the same Philox block and the same 53-bit
+// uniforms, so each normal is the FP64 one to within a few FP32 ulps (the
+// FP32 state rounds every update to that resolution anyway). ln u1 keeps
+// full relative accuracy: log of u1 itself below 1/2, log1p of the exact
+// u1 - 1 above (where u1 rounded to FP32 would lose it); the angle goes to
+// sincospi as 2 u2 half-turns, which needs no range reduction.
+__device__ __forceinline__ void philox_normal_pair32(uint64_t seed, uint64_t step, uint64_t pair,
+                                                     float &zc, float &zs) {
+    uint32_t c[4] = {(uint32_t)pair, (uint32_t)step, (uint32_t)(step >> 32), 0u};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const uint64_t k1 = ((((uint64_t)c[0]) << 32) | c[1]) >> 11;
+    const uint64_t k2 = ((((uint64_t)c[2]) << 32) | c[3]) >> 11;
+    const double u1 = (double)(k1 + 1) * 0x1.0p-53;
+    const double u2 = (double)k2 * 0x1.0p-53;
+    const float lu = u1 < 0.5 ? logf((float)u1) : log1pf((float)(u1 - 1.0));
+    const float rad = sqrtf(-2.0f * lu);
+    float sn, cs;
+    sincospif(2.0f * (float)u2, &sn, &cs);
+    zc = rad * cs;
+    zs = rad * sn;
+}
+
+__device__ __forceinline__ float philox_normal32(uint64_t seed, uint64_t step, uint64_t site) {
+    float zc, zs;
+    philox_normal_pair32(seed, step, site >> 1, zc, zs);
+    return (site & 1) ? zs : zc;
+}
+
+// The normal pair in the line state's precision (FP64 or FP32 Box-Muller),
+// widened to double: every kernel of one state precision draws its kicks
+// through this, so they agree bit for bit.
+template <typename S>
+__device__ __forceinline__ void state_normal_pair(uint64_t seed, uint64_t step, uint64_t pair,
+                                                  double &zc, double &zs) {
+    if constexpr (sizeof(S) == 8) {
+        philox_normal_pair(seed, step, pair, zc, zs);
+    } else {
+        float c, s;
+        philox_normal_pair32(seed, step, pair, c, s);
+        zc = c;
+        zs = s;
+    }
+}
+
+template <typename S>
+__device__ __forceinline__ double state_normal(uint64_t seed, uint64_t step, uint64_t site) {
+    if constexpr (sizeof(S) == 8) return philox_normal(seed, step, site);
+    else return philox_normal32(seed, step, site);
+}
+
 }  // namespace qew_dev
 
 namespace qew_dev {

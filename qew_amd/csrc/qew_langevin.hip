@@ -134,7 +134,7 @@ __global__ __launch_bounds__(kTile) void langevin32_kernel(StepArgs a) {
     const float g = dxf * (lf + 0.5f * (s_dvdy[tid - 1] + s_dvdy[tid]) - f);
     float hn = hi - (float)a.drift * g;
     if (a.diff > 0)
-        hn += (float)(a.diff * philox_normal(a.seed + (uint64_t)line, a.step, (uint64_t)i));
+        hn += (float)(a.diff * state_normal<float>(a.seed + (uint64_t)line, a.step, (uint64_t)i));
     static_cast<float *>(a.h_out)[L.off + i] = hn;
 }
 
@@ -185,7 +185,7 @@ __device__ __forceinline__ int ext_site(int out0, int t, int n) {
 // it has steps, and the draws of different steps are independent. A thread
 // whose partner holds another pair (the line wraps inside the tile) draws
 // its own site's blocks. s0 is even.
-template <class Store>
+template <typename S, class Store>
 __device__ __forceinline__ void draw_kicks(const MultiArgs &a, uint64_t key, int t, int gi, int gj,
                                            int s0, int s1, Store store) {
     if (gi != gj && (gi >> 1) == (gj >> 1)) {
@@ -196,7 +196,7 @@ __device__ __forceinline__ void draw_kicks(const MultiArgs &a, uint64_t key, int
             const double df = (t & 1) ? a.diff[s2 + 1] : a.diff[s2];
             if (s < s1 && df > 0) {
                 double zc, zs;
-                philox_normal_pair(key, a.step0 + (uint64_t)s, (uint64_t)(gi >> 1), zc, zs);
+                state_normal_pair<S>(key, a.step0 + (uint64_t)s, (uint64_t)(gi >> 1), zc, zs);
                 store(s, te, df * zc);
                 store(s, te ^ 1, df * zs);
             }
@@ -204,7 +204,7 @@ __device__ __forceinline__ void draw_kicks(const MultiArgs &a, uint64_t key, int
     } else {
         for (int s = s0; s < s1; ++s) {
             const double df = a.diff[s];
-            if (df > 0) store(s, t, df * philox_normal(key, a.step0 + (uint64_t)s, (uint64_t)gi));
+            if (df > 0) store(s, t, df * state_normal<S>(key, a.step0 + (uint64_t)s, (uint64_t)gi));
         }
     }
 }
@@ -245,7 +245,7 @@ __global__ __launch_bounds__(kLvExt) void langevin64_multi(MultiArgs a) {
         const int cur = s & 1;
         if (s % KC == 0) {
             if (s > 0) __syncthreads();  // the previous chunk's kicks have been read
-            draw_kicks(a, key, t, gi, gj, s, min(s + KC, a.nsteps), store);
+            draw_kicks<double>(a, key, t, gi, gj, s, min(s + KC, a.nsteps), store);
         }
         __syncthreads();
         const double hi = s_h[cur][t];
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(kLvExt) void langevin32_multi(MultiArgs a) {
         const int cur = s & 1;
         if (s % KC == 0) {
             if (s > 0) __syncthreads();  // the previous chunk's kicks have been read
-            draw_kicks(a, key, t, gi, gj, s, min(s + KC, a.nsteps), store);
+            draw_kicks<float>(a, key, t, gi, gj, s, min(s + KC, a.nsteps), store);
         }
         __syncthreads();
         const float hi = s_h[cur][t];
@@ -358,7 +358,7 @@ __device__ __forceinline__ void wave_kicks(const MultiArgs &a, uint64_t key, int
     for (int s2 = 0; s2 < K; s2 += 2) {
         const int s = paired ? s2 + odd : s2;
         double zc, zs;
-        philox_normal_pair(key, a.step0 + (uint64_t)s, (uint64_t)(gi >> 1), zc, zs);
+        state_normal_pair<T>(key, a.step0 + (uint64_t)s, (uint64_t)(gi >> 1), zc, zs);
         const double mine = (gi & 1) ? zs : zc;   // this lane's site, step s
         const double other = (gi & 1) ? zc : zs;  // the pair's other site, step s
         const double recv = __shfl_xor(other, 1); // paired: this site, step s ^ 1
@@ -368,7 +368,7 @@ __device__ __forceinline__ void wave_kicks(const MultiArgs &a, uint64_t key, int
             kick[s2 + 1] = (T)(d1 * (odd ? mine : recv));
         } else {
             kick[s2] = (T)(d0 * mine);
-            kick[s2 + 1] = (T)(d1 * philox_normal(key, a.step0 + (uint64_t)(s2 + 1), (uint64_t)gi));
+            kick[s2 + 1] = (T)(d1 * state_normal<T>(key, a.step0 + (uint64_t)(s2 + 1), (uint64_t)gi));
         }
     }
 }
