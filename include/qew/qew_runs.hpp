@@ -15,12 +15,17 @@
 //     seed + 1 (:154);
 //   * ramp: f = k f_step with an integer k, warm start from the previous
 //     configuration, stop at the first non-blocked force (:224-241).
+// The static sweep's pinning lengths are independent minimisations of one
+// disorder, so they run together in the persistent batched L-BFGS kernel
+// (the reference minimises them one after the other; StaticConfig::batched
+// = false does the same here).
 #ifndef QEW_RUNS_HPP
 #define QEW_RUNS_HPP
 
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <memory>
 #include <vector>
 
 #include "qew.hpp"
@@ -40,6 +45,12 @@ struct StaticConfig {
     real_t ftol = 1e-6;
     int max_iter = 5000;
     int m = 8;  // history length (the reference fixes the LbfgsParams default)
+    // true: the pinning lengths are minimised together by the persistent
+    // batched L-BFGS kernel (one workgroup per length) when it supports the
+    // line (nx a power of two in [4, 16384], m <= 32); false, or an
+    // unsupported line: one host-driven minimisation per length, as the
+    // reference does. No reference counterpart.
+    bool batched = true;
 };
 
 struct StaticProfile {
@@ -53,29 +64,61 @@ inline real_t larkin_line_tension(double lp_over_xi, real_t amplitude) {
     return std::pow(lp_over_xi, static_cast<real_t>(1.5)) * amplitude;
 }
 
+// True when the persistent batched L-BFGS kernel accepts a preconditioned
+// line of n sites with history m (its FFT works on power-of-two lengths up to
+// 16384 in one workgroup's LDS).
+inline bool batch_lbfgs_supports(int n, int m) {
+    return n >= 4 && n <= 16384 && (n & (n - 1)) == 0 && m >= 1 && m <= 32;
+}
+
 template <class Noise>
 std::vector<StaticProfile> static_sweep(const StaticConfig &c, const Noise &noise) {
-    std::vector<StaticProfile> out;
-    for (double lp : c.pinning_lengths_over_xi) {
-        Params p;
-        p.physical_size = c.Lx;
-        p.line_tension = larkin_line_tension(lp, c.amplitude);
-        p.driving_force = c.driving_force;
-        p.model = c.model;
+    LbfgsParams opt;
+    opt.m = c.m;
+    opt.ftol = c.ftol;
+    opt.max_iter = c.max_iter;
+    opt.precondition = true;
+    opt.precond_shift = static_cast<real_t>(0.5) * (c.Lx / c.nx) * c.amplitude / (c.xi * c.xi);
+    const size_t np = c.pinning_lengths_over_xi.size();
+    std::vector<Params> ps(np);
+    std::vector<StaticProfile> out(np);
+    for (size_t k = 0; k < np; ++k) {
+        const double lp = c.pinning_lengths_over_xi[k];
+        ps[k].physical_size = c.Lx;
+        ps[k].line_tension = larkin_line_tension(lp, c.amplitude);
+        ps[k].driving_force = c.driving_force;
+        ps[k].model = c.model;
+        out[k].pinning_length_over_xi = lp;
+        out[k].line_tension = ps[k].line_tension;
+    }
+    if (np == 0) return out;
+    if (c.batched && batch_lbfgs_supports(c.nx, c.m)) {
+        // every length's line back to back in one buffer, one launch
+        const DeviceNoise dn = as_device(noise);
+        std::vector<qew_noise> nh(np, dn.handle());
+        std::vector<int> len(np, c.nx);
+        std::vector<qew_params> cp(np);
+        for (size_t k = 0; k < np; ++k) cp[k] = ps[k].c();
+        qew_batch b = nullptr;
+        check(qew_batch_create(context(), (int)np, len.data(), nh.data(), cp.data(), &b));
+        std::unique_ptr<qew_batch_s, qew_status (*)(qew_batch)> owner(b, qew_batch_destroy);
+        View1D h("h", np * (size_t)c.nx);
+        deep_copy(h, static_cast<real_t>(0.5) * c.Ly);
+        const qew_lbfgs_params l = opt.c();
+        std::vector<qew_lbfgs_result> r(np);
+        check(qew_batch_lbfgs(b, QEW_F64, &l, h.data(), r.data()));
+        const std::vector<double> all = to_host_vec(h);
+        for (size_t k = 0; k < np; ++k) {
+            out[k].result = from_c(r[k]);
+            out[k].h.assign(all.begin() + k * (size_t)c.nx, all.begin() + (k + 1) * (size_t)c.nx);
+        }
+        return out;
+    }
+    for (size_t k = 0; k < np; ++k) {
         View1D h("h", c.nx);
         deep_copy(h, static_cast<real_t>(0.5) * c.Ly);
-        LbfgsParams opt;
-        opt.m = c.m;
-        opt.ftol = c.ftol;
-        opt.max_iter = c.max_iter;
-        opt.precondition = true;
-        opt.precond_shift = static_cast<real_t>(0.5) * (c.Lx / c.nx) * c.amplitude / (c.xi * c.xi);
-        StaticProfile prof;
-        prof.result = lbfgs_minimize(h, p, noise, opt);
-        prof.pinning_length_over_xi = lp;
-        prof.line_tension = p.line_tension;
-        prof.h = to_host_vec(h);
-        out.push_back(std::move(prof));
+        out[k].result = lbfgs_minimize(h, ps[k], noise, opt);
+        out[k].h = to_host_vec(h);
     }
     return out;
 }

@@ -2,8 +2,10 @@
 // (the reference's executables/qew_static/main.cpp:30-116: same options, same
 // defaults, same summary lines, exit code 2 on a bad option). Runs on the
 // MI355X through libqew_amd. Output: the reference's dataset keys in an HDF5
-// file (a qew-kv text archive when --output ends in .kv). Extra option: --m (L-BFGS history length, default 8 as in the
-// reference's LbfgsParams).
+// file (a qew-kv text archive when --output ends in .kv). Extra options: --m
+// (L-BFGS history length, default 8 as in the reference's LbfgsParams) and
+// --batched (0 minimises the pinning lengths one after the other as the
+// reference does; 1, the default, all in one persistent batched launch).
 #include <cstdint>
 #include <cstdio>
 #include <exception>
@@ -30,6 +32,8 @@ const char *kUsage =
     "  --ftol            L-BFGS tolerance       (default 1e-6)\n"
     "  --max_iter        L-BFGS iteration cap   (default 5000)\n"
     "  --m               L-BFGS history length  (default 8)\n"
+    "  --batched         1: all pinning lengths in one batched launch, 0: one\n"
+    "                    host-driven minimisation per length (default 1)\n"
     "  --seed            noise RNG seed         (default 1)\n"
     "  --output          output file            (default static_<model>.h5; .kv: text)\n";
 
@@ -59,6 +63,9 @@ int run(int argc, char **argv) {
     cfg.ftol = args.dbl("ftol", 1e-6);
     cfg.max_iter = args.integer("max_iter", 5000);
     cfg.m = args.integer("m", 8);
+    const int batched = args.integer("batched", 1);
+    if (batched != 0 && batched != 1) throw std::runtime_error("--batched must be 0 or 1");
+    cfg.batched = batched == 1;
     const std::uint64_t seed = args.u64("seed", 1);
     const std::string fname = args.str("output", std::string("static_") + model_name + ".h5");
     args.require_all_used();

@@ -56,6 +56,49 @@ static void static_sweep_shape_and_mapping() {
     }
 }
 
+// The batched sweep (all pinning lengths in one persistent launch) reaches the
+// minima of the reference's one-length-at-a-time sweep; a line length the
+// batched preconditioner does not take falls back to the sequential sweep.
+static void static_sweep_batched_equals_sequential() {
+    StaticConfig cfg;
+    cfg.model = Model::Linear;
+    cfg.xi = 0.1;
+    cfg.Lx = 2.56;
+    cfg.Ly = 1.0;
+    cfg.nx = 256;
+    cfg.ny = 64;
+    cfg.pinning_lengths_over_xi = {1.0, 4.0, 10.0};
+    cfg.ftol = 1e-7;  // above the energy roundoff floor of these lines
+    cfg.max_iter = 5000;
+    FilteredNoise noise(cfg.nx, cfg.ny, cfg.Lx, cfg.Ly, cfg.amplitude, cfg.xi, cfg.xi, 3);
+    cfg.batched = true;
+    const auto b = static_sweep(cfg, noise);
+    cfg.batched = false;
+    const auto s = static_sweep(cfg, noise);
+    CHECK(b.size() == 3 && s.size() == 3);
+    for (size_t k = 0; k < b.size() && k < s.size(); ++k) {
+        CHECK(b[k].result.converged && s[k].result.converged);
+        CHECK(b[k].line_tension == s[k].line_tension);
+        CHECK(b[k].h.size() == 256u && s[k].h.size() == 256u);
+        double dh = 0;
+        for (size_t i = 0; i < b[k].h.size() && i < s[k].h.size(); ++i)
+            dh = std::max(dh, std::abs(b[k].h[i] - s[k].h[i]));
+        CHECK(dh < 1e-5);
+    }
+    StaticConfig odd;
+    odd.model = Model::Linear;
+    odd.Lx = 1.0;
+    odd.nx = 24;  // not a power of two: the sequential path
+    odd.pinning_lengths_over_xi = {0.5, 2.0};
+    odd.ftol = 1e-8;
+    const auto o = static_sweep(odd, ZeroNoise{});
+    CHECK(o.size() == 2);
+    for (const auto &prof : o) {
+        CHECK(prof.result.converged && prof.h.size() == 24u);
+        for (double v : prof.h) CHECK(std::abs(v - 0.5) < 1e-9);
+    }
+}
+
 static void dynamic_anneal_timestep_and_snapshots() {
     DynamicConfig cfg;
     cfg.model = Model::Linear;
@@ -246,6 +289,7 @@ static void facade_semantics() {
 int main() {
     fmt_specs();
     static_sweep_shape_and_mapping();
+    static_sweep_batched_equals_sequential();
     dynamic_anneal_timestep_and_snapshots();
     dynamic_anneal_matches_step_loop();
     depinning_ramp_brackets_critical_force();

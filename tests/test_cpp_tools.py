@@ -35,7 +35,7 @@ def test_cli_parser_selftest(tmp_path):
     assert r.returncode == 0, r.stdout
 
 
-@pytest.mark.parametrize("name", ["qew_static", "qew_dynamic", "qew_depinning"])
+@pytest.mark.parametrize("name", ["qew_static", "qew_dynamic", "qew_depinning", "qew_benchmark"])
 def test_tools_help_and_option_errors(name):
     exe = tool(name)
     r = subprocess.run([exe, "--help"], capture_output=True, text=True, timeout=60)
@@ -69,6 +69,16 @@ def test_tools_write_reference_keys(tmp_path):
         assert kv[f"driving_force=0/pinning_length={lp}/pinning_length"] == float(lp)
         assert kv[f"driving_force=0/pinning_length={lp}/h"].shape == (256,)
     assert r.stdout.count("converged") == 2
+    # --batched 0: the reference's one minimisation per length, same minima
+    r0 = subprocess.run([tool("qew_static"), "--nx", "256", "--ny", "64", "--Lx", "2.56",
+                         "--pinning_lengths", "1,10", "--model", "linear", "--batched", "0",
+                         "--output", str(tmp_path / "s0.kv")], capture_output=True, text=True,
+                        timeout=300)
+    assert r0.returncode == 0, r0.stderr
+    kv0 = read_kv(tmp_path / "s0.kv")
+    for lp in ("1", "10"):
+        key = f"driving_force=0/pinning_length={lp}/h"
+        assert np.max(np.abs(kv0[key] - kv[key])) < 1e-5
 
     out = tmp_path / "d.kv"
     r = subprocess.run([tool("qew_dynamic"), "--nx", "128", "--ny", "64", "--total_time", "20",
@@ -88,3 +98,20 @@ def test_tools_write_reference_keys(tmp_path):
     assert "f_c_upper_bound" in kv and kv["driving_force=0/h"].shape == (64,)
     assert kv["f_c_lower"] <= kv["f_c"] <= kv["f_c_upper"] <= kv["f_c_upper_bound"]
     assert "Line depinned near" in r.stdout
+
+
+@pytest.mark.gpu
+def test_benchmark_tool_prints_reference_summary():
+    """qew_benchmark restates executables/benchmark/main.cpp: kernel timings on
+    a 262144-site line, then a preconditioned L-BFGS relaxation of a
+    qew_static-shaped line (n = 4096) that converges."""
+    import re
+    r = subprocess.run([tool("qew_benchmark"), "--reps", "3"], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.strip().splitlines()
+    assert re.match(r"device= 0  setup= *[0-9.]+s  grad= *[0-9.]+ ms/ea  obj= *[0-9.]+ ms/ea  "
+                    r"dyn= *[0-9.]+ ms/ea$", lines[0]), lines[0]
+    m = re.match(r"lbfgs\(n=4096, precond\): *[0-9.]+s  iters=(\d+)  feval=(\d+)  converged$", lines[1])
+    assert m, lines[1]
+    assert 0 < int(m.group(1)) <= int(m.group(2))
