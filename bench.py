@@ -61,6 +61,19 @@ def distinct_coefficients(h, n, nx, ny):
     return int(np.unique(np.concatenate(keys)).size)
 
 
+def distinct_mid_values(h, n, nx, ny):
+    """Distinct (x-column, y-cell) values a midpoint-contracted table supplies
+    to the edges of line h (line spacing = table spacing): edge i reads the 4
+    y levels of column (i + ip) // 2; only the wrap edge shares a column."""
+    i = np.arange(n)
+    ip = (i + 1) % n
+    ix = ((i + ip) // 2) % nx
+    hc = (h[ip] + h) * 0.5
+    iy = np.floor(np.mod(hc, ny)).astype(np.int64)
+    keys = [ix * ny + (iy + b) % ny for b in (-1, 0, 1, 2)]
+    return int(np.unique(np.concatenate(keys)).size)
+
+
 def traffic_from_profiles():
     p = os.path.join(ROOT, "profiles", "eval_traffic.json")
     if os.path.exists(p):
@@ -208,15 +221,16 @@ def bench_langevin(log, nsteps=100000):
     10^5 steps, temperature annealed linearly from 0.5 to 0.01 in the drivers'
     parameterisation (drift = T dt, diffusion = T sqrt(2 dt) for A = tau = 1,
     with the drivers' one-step lag, src/qew_runs.h:154-170), Philox normals
-    keyed by (seed 2, step, site). FP64 and FP32 (anchored offsets, FP32
-    table) run the same noise; the mean velocity, W^2 and the roughness
-    exponent of the final line are compared."""
+    keyed by (seed 2, step, site). FP64, FP32 (anchored offsets, FP32 table)
+    and FP32 on the midpoint-contracted table run the same noise; the mean
+    velocity, W^2 and the roughness exponent of the final line are compared."""
     import torch
     import qew_amd as q
     Lc, ny = 65536, 4096
     t0 = time.perf_counter()
     n64 = q.FilteredNoise(Lc, ny, float(Lc), float(ny), 1.0, 4.0, 4.0, 1, dtype=q.F64)
     n32 = n64.convert(q.F32)
+    nmid = n64.convert(q.F32_MID)
     torch.cuda.synchronize()
     log(f"langevin: table generated in {time.perf_counter() - t0:.1f} s")
     f = 0.5 * FC_EST_LANGEVIN
@@ -228,7 +242,7 @@ def bench_langevin(log, nsteps=100000):
     diff = temp * np.sqrt(2 * dt)
     p = q.Params(float(Lc), 1.0, f, q.Model.Linear)
     out = {}
-    for name, nz, dtype in (("f64", n64, q.F64), ("f32", n32, q.F32)):
+    for name, nz, dtype in (("f64", n64, q.F64), ("f32", n32, q.F32), ("f32_mid", nmid, q.F32)):
         b = q.Batch([Lc], [nz], [p])
         if dtype == q.F64:
             h = torch.full((Lc,), ny / 2.0, dtype=torch.float64, device="cuda:0")
@@ -259,6 +273,11 @@ def bench_langevin(log, nsteps=100000):
         / abs(out["f64"]["mean_velocity"]),
         "W2": abs(out["f32"]["W2"] - out["f64"]["W2"]) / out["f64"]["W2"],
         "zeta_abs": abs(out["f32"]["zeta"] - out["f64"]["zeta"])}
+    out["f32_mid_vs_f64_rel_err"] = {
+        "mean_velocity": abs(out["f32_mid"]["mean_velocity"] - out["f64"]["mean_velocity"])
+        / abs(out["f64"]["mean_velocity"]),
+        "W2": abs(out["f32_mid"]["W2"] - out["f64"]["W2"]) / out["f64"]["W2"],
+        "zeta_abs": abs(out["f32_mid"]["zeta"] - out["f64"]["zeta"])}
     out["workload"] = (f"L={Lc}, table {Lc}x{ny}, xi=4, linear, f=0.5*f_c(est)={f:g}, "
                        f"dt={dt}, {nsteps} steps, T 0.5 -> 0.01")
     log(f"langevin: {json.dumps(out)}")
@@ -360,6 +379,8 @@ def main():
     ap.add_argument("--lbfgs-lines", type=int, default=512)
     ap.add_argument("--only", choices=["both", "f32", "f64"], default="both")
     ap.add_argument("--no-bf16", action="store_true", help="skip the BF16-table evaluation leg")
+    ap.add_argument("--no-mid", action="store_true",
+                    help="skip the midpoint-contracted FP32 table evaluation leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (reference point for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE scaling) and exit")
@@ -388,15 +409,24 @@ def main():
     shifts = torch.tensor((np.arange(B) % NSHIFT) * SHIFT, dtype=torch.float64, device=dev)
     u_sum = distinct_coefficients(h0, L, L, NY)
     log(f"sum_r U_r = {u_sum} ({u_sum / L:.3f} per row)")
+    u_mid = distinct_mid_values(h0, L, L, NY)
 
     results = {}
     params = q.Params(float(L), 1.0, 0.0, q.Model.Arclength)
 
     def run(table):
-        # table storage "f64" (FP64 line state), "f32" or "bf16" (FP32 line state)
+        # table storage "f64" (FP64 line state), "f32", "bf16" or "f32_mid"
+        # (FP32 line state; f32_mid: the FP64 table contracted at the edge
+        # midpoints' x-fraction 1/2)
         t_gen = time.perf_counter()
         dtype = q.F64 if table == "f64" else q.F32
-        noise = q.FilteredNoise(L, NY, float(L), float(NY), 1.0, XI, XI, 1, dtype=dtype)
+        if table == "f32_mid":
+            base = results.get("table64") or q.FilteredNoise(L, NY, float(L), float(NY), 1.0, XI,
+                                                              XI, 1, dtype=q.F64)
+            noise = base.convert(q.F32_MID)
+            del base
+        else:
+            noise = q.FilteredNoise(L, NY, float(L), float(NY), 1.0, XI, XI, 1, dtype=dtype)
         if table == "bf16":
             noise = noise.convert(q.BF16)
         torch.cuda.synchronize()
@@ -426,9 +456,10 @@ def main():
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         kms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
-        s_c = {"f64": 8, "f32": 4, "bf16": 2}[table]  # coefficient bytes
+        s_c = {"f64": 8, "f32": 4, "bf16": 2, "f32_mid": 4}[table]  # table bytes per value
         s_h = 8 if dtype == q.F64 else 4               # line state and gradient bytes
-        bytes_eval = s_c * u_sum + L * 2 * s_h
+        values = u_mid if table == "f32_mid" else u_sum
+        bytes_eval = s_c * values + L * 2 * s_h
         res = {
             "evals_per_s": B * args.steps / wall,
             "ms_per_step": 1e3 * wall / args.steps,
@@ -455,6 +486,10 @@ def main():
     if args.only == "both" and not args.no_bf16:
         r16, first16 = run("bf16")
         log(f"bf16: {json.dumps(r16)}")
+    rmid = None
+    if args.only == "both" and not args.no_mid:
+        rmid, firstmid = run("f32_mid")
+        log(f"f32_mid: {json.dumps(rmid)}")
 
     def error_vs_f64(first):
         g64 = first64[1].cpu().numpy()
@@ -516,6 +551,14 @@ def main():
                               "roofline_achieved_GBs": r16["achieved_GBs"],
                               "roofline_frac": r16["achieved_GBs"] / HBM_PEAK_GBS,
                               "error_vs_f64": error_vs_f64(first16) if r64 else None}
+    if rmid:
+        # midpoint-contracted FP32 table: 4 values per edge instead of the
+        # 4 x 4 window (every edge of this line lies at x-fraction 1/2)
+        line["f32_mid_table"] = {"value": rmid["evals_per_s"], "ms_per_step": rmid["ms_per_step"],
+                                 "bytes_per_eval": rmid["bytes_per_eval"],
+                                 "roofline_achieved_GBs": rmid["achieved_GBs"],
+                                 "roofline_frac": rmid["achieved_GBs"] / HBM_PEAK_GBS,
+                                 "error_vs_f64": error_vs_f64(firstmid) if r64 else None}
     if not args.no_lbfgs:
         torch.cuda.empty_cache()
         line["batched_lbfgs"] = bench_lbfgs(log, nlines=args.lbfgs_lines)

@@ -35,11 +35,18 @@ typedef enum qew_status {
 } qew_status;
 
 /* Arithmetic / storage type of line state and coefficient tables. QEW_BF16
- * is a table storage type only (bfloat16 coefficients, rounded to nearest
- * even from FP64 via FP32): such tables are read by qew_batch_eval and
- * qew_batch_langevin with QEW_F32 line state; every other entry point
- * rejects them with QEW_ERR_INVALID_ARGUMENT. */
-typedef enum qew_dtype { QEW_F64 = 0, QEW_F32 = 1, QEW_BF16 = 2 } qew_dtype;
+ * and QEW_F32_MID are table storage types only:
+ *  - QEW_BF16: bfloat16 coefficients, rounded to nearest even from FP64 via
+ *    FP32;
+ *  - QEW_F32_MID: one FP32 value per cell, T(ix, iy) = sum_a w_a c(ix-1+a, iy)
+ *    with the cubic B-spline weights at x-fraction 1/2 (1, 23, 23, 1)/48,
+ *    contracted in FP64 and rounded once. Every edge midpoint of a line whose
+ *    site spacing equals the table's x spacing (and whose length is even)
+ *    lies at x-fraction 1/2, so its 4 x 4 gather reduces to 4 values of T.
+ * Such tables are read by qew_batch_eval and qew_batch_langevin with QEW_F32
+ * line state (QEW_F32_MID only for lines with that geometry); every other
+ * entry point rejects them with QEW_ERR_INVALID_ARGUMENT. */
+typedef enum qew_dtype { QEW_F64 = 0, QEW_F32 = 1, QEW_BF16 = 2, QEW_F32_MID = 3 } qew_dtype;
 
 /* qew::Model (src/qew_model.h:18). */
 typedef enum qew_model { QEW_LINEAR = 0, QEW_ARCLENGTH = 1 } qew_model;
@@ -178,7 +185,9 @@ qew_status qew_noise_generate_batch(qew_ctx ctx, int count, const int *nx, const
                                     const double *xi_y, const uint64_t *seed, int dtype,
                                     qew_noise *out);
 /* A copy of a potential with its table stored as `dtype` (FP64 -> FP32
- * rounds each coefficient once). No reference counterpart. */
+ * rounds each coefficient once; QEW_F32_MID contracts the x-weights at
+ * fraction 1/2 in FP64 and rounds each cell's value once; BF16 and
+ * QEW_F32_MID tables are not converted further). No reference counterpart. */
 qew_status qew_noise_convert(qew_noise noise, int dtype, qew_noise *out);
 /* FilteredNoise(field, nx, ny, Lx, Ly) (src/filtered_noise.cpp:84-90): field
  * is host, row-major field[i*ny + j]. */
@@ -198,7 +207,8 @@ qew_status qew_noise_info(qew_noise noise, int *kind, int *dtype, int *nx, int *
 /* Host copy of noise_on_grid (FilteredNoise::field(), src/filtered_noise.h:195)
  * for generated tables, row-major nx*ny. */
 qew_status qew_noise_field(qew_noise noise, double *field_host);
-/* Host copy of the coefficients, row-major coef[i*ny + j], as FP64. */
+/* Host copy of the coefficients, row-major coef[i*ny + j], as FP64 (for a
+ * QEW_F32_MID table: its per-cell contracted values T(i, j)). */
 qew_status qew_noise_coefficients(qew_noise noise, double *coef_host);
 /* DeviceNoise::sample at m device points (src/filtered_noise.h:103-134);
  * x, y, v, dvdx, dvdy are device FP64 arrays. */

@@ -11,6 +11,9 @@ LIB_PATH = os.path.join(_PKG, "libqew_amd.so")
 
 QEW_OK, QEW_ERR_INVALID_ARGUMENT, QEW_ERR_HIP, QEW_ERR_OUT_OF_MEMORY, QEW_ERR_UNSUPPORTED = range(5)
 F64, F32, BF16 = 0, 1, 2  # BF16: table storage only (read with F32 line state)
+# table storage only: FP32 values contracted with the B-spline x-weights at
+# fraction 1/2 (read with F32 line state, lines with spacing = table dx, even n)
+F32_MID = 3
 LINEAR, ARCLENGTH = 0, 1
 TABLE, SINCOS, CONST, ZERO, HARMONIC, WASHBOARD = range(6)
 

@@ -689,9 +689,50 @@ __device__ __forceinline__ void edge32_combine(const LineGeom &G, const Edge32Lo
 
 // FP32 edge (e, ep) of a line whose heights are FP32 offsets from the
 // anchor: value and/or y-derivative of the potential at the edge midpoint.
+// Midpoint-contracted table (dtype 3): T(ix, iy) already holds the x-sum
+// with the weights at x-fraction 1/2, which is every edge's fraction on the
+// lines such tables are accepted for (site spacing = table x spacing, even
+// n). The edge (e, ep) lies in x cell (e + ep) / 2 (integer division: e + ep
+// is odd, or n - 1 for the wrap edge), and V and dV/dy need only the 4
+// values of that column at the edge's y levels.
+template <bool WANT_V, bool WANT_DY>
+__device__ __forceinline__ void edge32_mid(const LineGeom &G, int e, int ep, float he, float hep,
+                                           float &v, float &dvdy) {
+    const int nx = G.nd.nx, ny = G.nd.ny;
+    int ix = (e + ep) >> 1;
+    if (ix >= nx) ix -= (ix / nx) * nx;
+    const float t = __builtin_fmaf((he + hep) * 0.5f, G.inv_dy, G.ya.frac);
+    const float kf = floorf(t);
+    const float fy = t - kf;
+    int iy = G.ya.cell + (int)kf;
+    if (iy < 0 || iy >= ny) iy = ((iy % ny) + ny) % ny;
+    const int y2 = (iy + 1 == ny) ? 0 : iy + 1;
+    const int ys[4] = {(iy == 0) ? ny - 1 : iy - 1, iy, y2, (y2 + 1 == ny) ? 0 : y2 + 1};
+    const float *tab = static_cast<const float *>(G.nd.coef);
+    float c[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+        c[b] = *(const QEW_GLOBAL float *)(tab + (size_t)ys[b] * G.nd.pitch + ix);
+    float wy[4], dwy[4];
+    if (WANT_V) bspline_wf(fy, wy);
+    if (WANT_DY) bspline_dwf(fy, dwy);
+    float vv = 0.f, dd = 0.f;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        if (WANT_V) vv = __builtin_fmaf(c[b], wy[b], vv);
+        if (WANT_DY) dd = __builtin_fmaf(c[b], dwy[b], dd);
+    }
+    v = vv;
+    dvdy = dd * G.inv_dy;
+}
+
 template <bool WANT_V, bool WANT_DY>
 __device__ __forceinline__ void edge32(const LineGeom &G, int e, int ep, float he, float hep,
                                        float &v, float &dvdy) {
+    if (G.nd.dtype == 3) {
+        edge32_mid<WANT_V, WANT_DY>(G, e, ep, he, hep, v, dvdy);
+        return;
+    }
     const Edge32Loc l = edge32_locate(G, e, ep, he, hep);
     const int ny = G.nd.ny;
     const int y2 = (l.iy + 1 == ny) ? 0 : l.iy + 1;

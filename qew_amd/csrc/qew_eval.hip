@@ -377,9 +377,7 @@ qew_status qew_batch_eval(qew_batch b, int dtype, const void *h, const double *a
         a.energy = energy;
         a.partial = b->d_partial.ptr;
         if (dtype == QEW_F32) {
-            for (const auto &nd : b->noises)
-                QEW_REQUIRE(nd.kind == TABLE && (nd.dtype == QEW_F32 || nd.dtype == QEW_BF16),
-                            "QEW_F32 evaluation needs FP32 or BF16 coefficient tables");
+            require_f32_tables(b->lines, b->noises, "QEW_F32 evaluation");
             launch_eval_f32(ctx->stream, a, b->nlines);
         } else {
             QEW_REQUIRE(!anchor, "anchors apply to QEW_F32 line state only");

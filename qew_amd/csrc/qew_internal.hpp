@@ -170,16 +170,36 @@ inline void ctx_check(qew_ctx ctx) { QEW_REQUIRE(ctx != nullptr, "null context")
 
 inline void use_device(qew_ctx ctx) { QEW_HIP(hipSetDevice(ctx->device)); }
 
-// 2-byte (BF16) tables are read only by the FP32 batch evaluation and
-// dynamics; every entry point that reads tables as FP64 or FP32 values
-// rejects them before launching anything.
+// BF16 tables and midpoint-contracted FP32 tables are read only by the FP32
+// batch evaluation and dynamics; every entry point that reads tables as
+// per-cell FP64 or FP32 coefficients rejects them before launching anything.
 inline void require_wide_table(const qew_dev::NoiseDesc &nd) {
-    QEW_REQUIRE(nd.kind != qew_dev::TABLE || nd.dtype != QEW_BF16,
-                "BF16 coefficient tables are read only by qew_batch_eval and qew_batch_langevin "
-                "with QEW_F32 line state");
+    QEW_REQUIRE(nd.kind != qew_dev::TABLE || (nd.dtype != QEW_BF16 && nd.dtype != QEW_F32_MID),
+                "BF16 and QEW_F32_MID tables are read only by qew_batch_eval and "
+                "qew_batch_langevin with QEW_F32 line state");
 }
 inline void require_wide_tables(const std::vector<qew_dev::NoiseDesc> &v) {
     for (const auto &nd : v) require_wide_table(nd);
+}
+
+// The tables an FP32-state batch call reads: FP32, BF16 or QEW_F32_MID, the
+// last only for lines whose edge midpoints all lie at x-fraction 1/2 (site
+// spacing L / n equal to the table's dx, so the FP32 locate's x scale is
+// exactly 1/2, and n even, so the wrap edge's midpoint (n - 1) / 2 is a
+// half-integer too).
+inline void require_f32_tables(const std::vector<qew_dev::LineDesc> &lines,
+                               const std::vector<qew_dev::NoiseDesc> &noises, const char *what) {
+    for (const auto &nd : noises)
+        QEW_REQUIRE(nd.kind == qew_dev::TABLE &&
+                        (nd.dtype == QEW_F32 || nd.dtype == QEW_BF16 || nd.dtype == QEW_F32_MID),
+                    std::string(what) + " needs FP32, BF16 or QEW_F32_MID coefficient tables");
+    for (const auto &l : lines) {
+        const qew_dev::NoiseDesc &nd = noises[l.noise];
+        if (nd.dtype == QEW_F32_MID)
+            QEW_REQUIRE(l.n % 2 == 0 && l.L / (double)l.n == nd.dx,
+                        "a QEW_F32_MID table needs lines of even length whose site spacing "
+                        "L / n equals the table's x spacing");
+    }
 }
 
 // Makes sure the context's scratch covers the energy partials of `ntiles`

@@ -641,9 +641,7 @@ qew_status qew_batch_langevin(qew_batch b, int dtype, void *h, const double *anc
         use_device(ctx);
         const size_t es = dtype == QEW_F32 ? sizeof(float) : sizeof(double);
         if (dtype == QEW_F32) {
-            for (const auto &nd : b->noises)
-                QEW_REQUIRE(nd.kind == TABLE && (nd.dtype == QEW_F32 || nd.dtype == QEW_BF16),
-                            "QEW_F32 dynamics needs FP32 or BF16 coefficient tables");
+            require_f32_tables(b->lines, b->noises, "QEW_F32 dynamics");
         } else {
             QEW_REQUIRE(!anchor, "anchors apply to QEW_F32 line state only");
             require_wide_tables(b->noises);

@@ -242,6 +242,38 @@ __global__ void bf16_table(const S *src, int s_pitch, unsigned short *dst, int d
     dst[(int64_t)iy * d_pitch + j] = bf16_bits((float)src[(int64_t)iy * s_pitch + cell + 1]);
 }
 
+// Midpoint-contracted FP32 table (pitch >= nx, no halo): element ix of row
+// iy holds sum_a w_a coef(ix - 1 + a mod nx, iy), w = the cubic B-spline
+// weights at fraction 1/2, contracted in FP64 (a = 0..3 in order) and
+// rounded to FP32 once; the padding elements are 0. The source rows have the
+// standard halo (element j = coef(j - 1 mod nx)), so the four neighbours of
+// ix are source elements ix .. ix + 3.
+template <typename S>
+__global__ void mid_table(const S *src, int s_pitch, float *dst, int d_pitch, int nx, int ny) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (int64_t)d_pitch * ny) return;
+    const int ix = (int)(k % d_pitch);
+    const int iy = (int)(k / d_pitch);
+    if (ix >= nx) {
+        dst[k] = 0.0f;
+        return;
+    }
+    double w[4];
+    qew_dev::bspline_w(0.5, w);
+    const S *r = src + (int64_t)iy * s_pitch + ix;
+    double acc = 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) acc += w[a] * (double)r[a];
+    dst[k] = (float)acc;
+}
+
+__global__ void mid_table_to_rowmajor(const float *table, double *dst, int nx, int ny, int pitch) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (int64_t)nx * ny) return;
+    const int i = (int)(k / ny), j = (int)(k % ny);
+    dst[k] = (double)table[(int64_t)j * pitch + i];
+}
+
 __global__ void bf16_table_to_rowmajor(const unsigned short *table, double *dst, int nx, int ny,
                                        int pitch) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -583,17 +615,46 @@ qew_status qew_noise_generate_batch(qew_ctx ctx, int count, const int *nx, const
 qew_status qew_noise_convert(qew_noise noise, int dtype, qew_noise *out) {
     return guard([&] {
         QEW_REQUIRE(noise && out, "null pointer");
-        QEW_REQUIRE(dtype == QEW_F64 || dtype == QEW_F32 || dtype == QEW_BF16,
-                    "dtype must be QEW_F64, QEW_F32 or QEW_BF16");
-        QEW_REQUIRE(noise->desc.kind != TABLE || noise->desc.dtype != QEW_BF16,
-                    "BF16 tables are not converted further: convert the FP64 or FP32 table");
+        QEW_REQUIRE(dtype == QEW_F64 || dtype == QEW_F32 || dtype == QEW_BF16 ||
+                        dtype == QEW_F32_MID,
+                    "dtype must be QEW_F64, QEW_F32, QEW_BF16 or QEW_F32_MID");
+        QEW_REQUIRE(noise->desc.kind != TABLE ||
+                        (noise->desc.dtype != QEW_BF16 && noise->desc.dtype != QEW_F32_MID),
+                    "BF16 and QEW_F32_MID tables are not converted further: convert the FP64 or "
+                    "FP32 table");
         qew_ctx ctx = noise->ctx;
         use_device(ctx);
         qew_noise nz = new_noise(ctx, noise->desc.kind, dtype, noise->desc.nx, noise->desc.ny,
                                  noise->desc.dx, noise->desc.dy);
         for (int k = 0; k < 4; ++k) nz->desc.p[k] = noise->desc.p[k];
         nz->field = noise->field;
-        if (noise->desc.kind == TABLE && dtype == QEW_BF16) {
+        if (noise->desc.kind == TABLE && dtype == QEW_F32_MID) {
+            const int nx = noise->desc.nx, ny = noise->desc.ny;
+            const int pitch = ((nx + 31) / 32) * 32;
+            hipError_t e = hipMalloc(&nz->table, sizeof(float) * (size_t)pitch * ny);
+            if (e != hipSuccess) {
+                delete nz;
+                QEW_HIP(e);
+            }
+            nz->desc.pitch = pitch;
+            nz->desc.coef = nz->table;
+            const dim3 g = grid1((int64_t)pitch * ny);
+            if (noise->desc.dtype == 1)
+                hipLaunchKernelGGL(mid_table<float>, g, dim3(256), 0, ctx->stream,
+                                   static_cast<const float *>(noise->table), noise->desc.pitch,
+                                   static_cast<float *>(nz->table), pitch, nx, ny);
+            else
+                hipLaunchKernelGGL(mid_table<double>, g, dim3(256), 0, ctx->stream,
+                                   static_cast<const double *>(noise->table), noise->desc.pitch,
+                                   static_cast<float *>(nz->table), pitch, nx, ny);
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) {
+                (void)hipFree(nz->table);
+                delete nz;
+                QEW_HIP(e);
+            }
+        } else if (noise->desc.kind == TABLE && dtype == QEW_BF16) {
             const int nx = noise->desc.nx, ny = noise->desc.ny;
             const int pitch = ((nx + 7 + 7) / 8) * 8;
             hipError_t e = hipMalloc(&nz->table, sizeof(unsigned short) * (size_t)pitch * ny);
@@ -708,7 +769,11 @@ qew_status qew_noise_coefficients(qew_noise noise, double *coef_host) {
         double *d = nullptr;
         QEW_HIP(hipMalloc(&d, sizeof(double) * (size_t)nx * ny));
         const dim3 g((nx + 31) / 32, (ny + 31) / 32), b(32, 8);
-        if (noise->desc.dtype == 2)
+        if (noise->desc.dtype == QEW_F32_MID)
+            hipLaunchKernelGGL(mid_table_to_rowmajor, grid1((int64_t)nx * ny), dim3(256), 0,
+                               ctx->stream, static_cast<const float *>(noise->table), d, nx, ny,
+                               noise->desc.pitch);
+        else if (noise->desc.dtype == 2)
             hipLaunchKernelGGL(bf16_table_to_rowmajor, grid1((int64_t)nx * ny), dim3(256), 0,
                                ctx->stream, static_cast<const unsigned short *>(noise->table), d,
                                nx, ny, noise->desc.pitch);
