@@ -129,6 +129,9 @@ def bench_lbfgs(log, nlines=512, plain_sample=8):
              for k, (L_, xi) in enumerate(zip(lengths, xis))]
     n64 = q.FilteredNoise.generate_batch(specs, dtype=q.F64)
     n32 = [nz.convert(q.F32) for nz in n64]
+    # every line has n = L sites on an L-wide table of spacing 1: its edge
+    # midpoints lie at x-fraction 1/2, so midpoint-contracted tables apply
+    nmid = [nz.convert(q.F32_MID) for nz in n64]
     torch.cuda.synchronize()
     log(f"lbfgs: {nlines} tables generated in {time.perf_counter() - t0:.1f} s "
         f"(sum L = {sum(lengths)})")
@@ -143,7 +146,8 @@ def bench_lbfgs(log, nlines=512, plain_sample=8):
     # |E| ~ 1e4), where the reference algorithm converges in ~100-300 iterations
     # FP32: the line offsets' resolution floors the FP32 gradient near 1e-4, so
     # FP32 stops there and is compared by its energy against the FP64 minimum
-    for name, noises, dtype, ftol in (("f64", n64, q.F64, 1e-5), ("f32", n32, q.F32, 1e-4)):
+    for name, noises, dtype, ftol in (("f64", n64, q.F64, 1e-5), ("f32", n32, q.F32, 1e-4),
+                                      ("f32_mid", nmid, q.F32, 1e-4)):
         b = q.Batch(lengths, noises, params)
         b.set_precond_shifts(shifts)
         lp = q.LbfgsParams(m=10, ftol=ftol, max_iter=10000, precondition=True)
@@ -171,6 +175,10 @@ def bench_lbfgs(log, nlines=512, plain_sample=8):
     out["f32_vs_f64_energy_rel_err"] = {"max": float(rel_err.max()),
                                         "median": float(np.median(rel_err)),
                                         "lines_within_1e-6": int((rel_err <= 1e-6).sum())}
+    rel_mid = np.abs(finals["f32_mid"] - finals["f64"]) / np.abs(finals["f64"])
+    out["f32_mid_vs_f64_energy_rel_err"] = {"max": float(rel_mid.max()),
+                                            "median": float(np.median(rel_mid)),
+                                            "lines_within_1e-6": int((rel_mid <= 1e-6).sum())}
     # max |grad| per line at each precision's minima, from the FP64 gradient
     out["max_grad_fp64_at_minima"] = {k: {"median": float(np.median(v)), "max": float(v.max())}
                                       for k, v in gmax.items()}

@@ -726,13 +726,56 @@ __device__ __forceinline__ void edge32_mid(const LineGeom &G, int e, int ep, flo
     dvdy = dd * G.inv_dy;
 }
 
+// Column of a midpoint-contracted table that edge (e, ep) reads (see
+// edge32_mid).
+__device__ __forceinline__ int mid_column(const NoiseDesc &nd, int e, int ep) {
+    int ix = (e + ep) >> 1;
+    if (ix >= nd.nx) ix -= (ix / nd.nx) * nd.nx;
+    return ix;
+}
+
+// V at height y of column ix of a midpoint-contracted table, in FP64: the y
+// locate and weights of locate64 / bspline_w, the 4 column values widened.
+__device__ __forceinline__ void mid_locate_y64(const NoiseDesc &nd, double y, int iyn[4],
+                                               double &fy) {
+    const int ny = nd.ny;
+    double gy = (nd.dy == 1.0) ? y : y / nd.dy;
+    gy = wrap_cells(gy, ny);
+    int iy = (int)floor(gy);
+    if (iy >= ny) { iy -= ny; gy -= (double)ny; }
+    fy = gy - (double)iy;
+    iyn[0] = (iy == 0) ? ny - 1 : iy - 1;
+    iyn[1] = iy;
+    iyn[2] = (iy + 1 == ny) ? 0 : iy + 1;
+    iyn[3] = (iyn[2] + 1 == ny) ? 0 : iyn[2] + 1;
+}
+__device__ __forceinline__ void mid_column4(const NoiseDesc &nd, int ix, const int iyn[4],
+                                            float c[4]) {
+    const float *t = static_cast<const float *>(nd.coef);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) c[b] = *(const QEW_GLOBAL float *)(t + (size_t)iyn[b] * nd.pitch + ix);
+}
+__device__ __forceinline__ double mid_v_from(const float c[4], double fy) {
+    double wy[4];
+    bspline_w(fy, wy);
+    double v = 0.0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) v += (double)c[b] * wy[b];
+    return v;
+}
+__device__ __forceinline__ double mid_v64(const NoiseDesc &nd, int ix, double y) {
+    int iyn[4];
+    double fy;
+    mid_locate_y64(nd, y, iyn, fy);
+    float c[4];
+    mid_column4(nd, ix, iyn, c);
+    return mid_v_from(c, fy);
+}
+
+// The 4 x 4 window path of an FP32 or BF16 table.
 template <bool WANT_V, bool WANT_DY>
-__device__ __forceinline__ void edge32(const LineGeom &G, int e, int ep, float he, float hep,
-                                       float &v, float &dvdy) {
-    if (G.nd.dtype == 3) {
-        edge32_mid<WANT_V, WANT_DY>(G, e, ep, he, hep, v, dvdy);
-        return;
-    }
+__device__ __forceinline__ void edge32_grid(const LineGeom &G, int e, int ep, float he, float hep,
+                                            float &v, float &dvdy) {
     const Edge32Loc l = edge32_locate(G, e, ep, he, hep);
     const int ny = G.nd.ny;
     const int y2 = (l.iy + 1 == ny) ? 0 : l.iy + 1;
@@ -759,6 +802,14 @@ __device__ __forceinline__ void edge32(const LineGeom &G, int e, int ep, float h
         }
     }
     edge32_combine<WANT_V, WANT_DY>(G, l, cf, v, dvdy);
+}
+
+// FP32 edge (e, ep) on any FP32-state table: FP32, BF16 or midpoint-contracted.
+template <bool WANT_V, bool WANT_DY>
+__device__ __forceinline__ void edge32(const LineGeom &G, int e, int ep, float he, float hep,
+                                       float &v, float &dvdy) {
+    if (G.nd.dtype == 3) edge32_mid<WANT_V, WANT_DY>(G, e, ep, he, hep, v, dvdy);
+    else edge32_grid<WANT_V, WANT_DY>(G, e, ep, he, hep, v, dvdy);
 }
 
 }  // namespace qew_dev

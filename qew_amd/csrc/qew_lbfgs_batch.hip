@@ -210,7 +210,7 @@ __device__ __forceinline__ double wg_maxabs(const T *a, int n, Red r) {
 template <typename T>
 constexpr int kUG = sizeof(T) == 8 ? 2 : 4;
 
-template <typename T, typename C>
+template <typename T, typename C, bool MID = false>
 __device__ double wg_objective(const LineGeom &G, const T *x, const T *d, double alpha, Red r) {
     const int n = G.n;
     double acc = 0;
@@ -251,8 +251,11 @@ __device__ double wg_objective(const LineGeom &G, const T *x, const T *d, double
                    const double s2 = dh * dh;
                    const double le =
                        (G.model == 0) ? G.lt * s2 * 0.5 : G.lt * (s2 / (sqrt(1.0 + s2) + 1.0));
-                   const double xc = (double)(i + ip) * 0.5 * G.dx;
-                   acc += G.dx * (le + sample_v64<float>(G.nd, xc, (hip + hi) * 0.5) - G.f * hi);
+                   const double hc = (hip + hi) * 0.5;
+                   double vv;
+                   if constexpr (MID) vv = mid_v64(G.nd, mid_column(G.nd, i, ip), hc);
+                   else vv = sample_v64<float>(G.nd, (double)(i + ip) * 0.5 * G.dx, hc);
+                   acc += G.dx * (le + vv - G.f * hi);
                });
     }
     return block_sum(acc, r.d);
@@ -270,11 +273,80 @@ __device__ double wg_objective(const LineGeom &G, const T *x, const T *d, double
 template <typename T>
 constexpr int kUT = sizeof(T) == 8 ? 1 : 2;
 
-template <typename T, typename C>
+// FP32 state on a midpoint-contracted table (dtype 3, its own kernel
+// instantiation): an edge's V needs the 4 values of one column, so the held
+// block is 4 floats; trials in the held y cell take V from it, others read
+// their own column values. Energies in FP64 as on FP32 tables. Two edges in
+// flight per thread measured fastest on the 512-line batch (5650 minima/s
+// against 5480 with four, whose spills grew from 224 to 400 B per lane).
+constexpr int kUTM = 2;
+
+__device__ __forceinline__ void objective_trials_mid(const LineGeom &G, const float *x,
+                                                     const float *d, const double *alpha,
+                                                     double (&acc)[kLS]) {
+    const int n = G.n;
+    constexpr int K = kLS;
+    struct Item {
+        P4<float> v;
+        int iy;
+        float c[4];
+    };
+    auto offsets = [&](const P4<float> &v, int j, double &oi, double &oip) {
+        const float al = (float)alpha[j];
+        oi = (double)__builtin_fmaf(al, v.b, v.a);
+        oip = (double)__builtin_fmaf(al, v.d, v.c);
+    };
+    stream<kUTM>(
+        n,
+        [&](int i) {
+            const int ip = (i + 1 == n) ? 0 : i + 1;
+            Item it;
+            it.v = P4<float>{x[i], d[i], x[ip], d[ip]};
+            double oi, oip, fy;
+            offsets(it.v, 0, oi, oip);
+            int iyn[4];
+            mid_locate_y64(G.nd, ((G.anchor + oip) + (G.anchor + oi)) * 0.5, iyn, fy);
+            it.iy = iyn[1];
+            mid_column4(G.nd, mid_column(G.nd, i, ip), iyn, it.c);
+            return it;
+        },
+        [&](int i, const Item &it) {
+            const int ip = (i + 1 == n) ? 0 : i + 1;
+            const int ix = mid_column(G.nd, i, ip);
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                double oi, oip;
+                offsets(it.v, j, oi, oip);
+                const double hi = G.anchor + oi, hip = G.anchor + oip;
+                int iyn[4];
+                double fy;
+                mid_locate_y64(G.nd, (hip + hi) * 0.5, iyn, fy);
+                double vv;
+                if (iyn[1] == it.iy) {
+                    vv = mid_v_from(it.c, fy);
+                } else {
+                    float c[4];
+                    mid_column4(G.nd, ix, iyn, c);
+                    vv = mid_v_from(c, fy);
+                }
+                const double dh = div1(oip - oi, G.dx);
+                const double s2 = dh * dh;
+                const double le =
+                    (G.model == 0) ? G.lt * s2 * 0.5 : G.lt * (s2 / (sqrt(1.0 + s2) + 1.0));
+                acc[j] += G.dx * (le + vv - G.f * hi);
+            }
+        });
+}
+
+template <typename T, typename C, bool MID = false>
 __device__ __forceinline__ void objective_trials_table(const LineGeom &G, const T *x, const T *d,
                                                        const double *alpha, double (&acc)[kLS]) {
     const int n = G.n;
     constexpr int K = kLS;
+    if constexpr (MID) {
+        objective_trials_mid(G, x, d, alpha, acc);
+        return;
+    }
     using B = typename std::conditional<sizeof(T) == 8, C, float>::type;  // block storage
     // trial heights of edge (i, ip) for step alpha[j]: FP64 state adds
     // alpha d; FP32 state forms the offset in FP32 and adds the anchor
@@ -364,7 +436,7 @@ __device__ __forceinline__ void objective_trials_table(const LineGeom &G, const 
 // Energies at the kLS trial points x + alpha[j] d in one pass over the line.
 // Each site term, each thread's summation order and the workgroup sum are
 // those of wg_objective, so phi[j] equals wg_objective(G, x, d, alpha[j]).
-template <typename T, typename C>
+template <typename T, typename C, bool MID = false>
 __device__ __forceinline__ void wg_objective_trials(const LineGeom &G, const T *x, const T *d,
                                     const double *alpha, double *phi, Red r) {
     const int n = G.n;
@@ -373,7 +445,7 @@ __device__ __forceinline__ void wg_objective_trials(const LineGeom &G, const T *
 #pragma unroll
     for (int j = 0; j < K; ++j) acc[j] = 0;
     if (G.nd.kind == TABLE) {
-        objective_trials_table<T, C>(G, x, d, alpha, acc);
+        objective_trials_table<T, C, MID>(G, x, d, alpha, acc);
         block_sumk<K>(acc, r.d);
 #pragma unroll
         for (int j = 0; j < K; ++j) phi[j] = acc[j];
@@ -415,7 +487,7 @@ __device__ __forceinline__ void wg_objective_trials(const LineGeom &G, const T *
     for (int j = 0; j < K; ++j) phi[j] = acc[j];
 }
 
-template <typename T, typename C>
+template <typename T, typename C, bool MID = false>
 __device__ __forceinline__ void wg_gradient(const LineGeom &G, const T *x, T *g, T *edy, T *esl) {
     const int n = G.n;
     const bool arc = G.model == 1;
@@ -431,7 +503,8 @@ __device__ __forceinline__ void wg_gradient(const LineGeom &G, const T *x, T *g,
                    if (arc) esl[i] = dh / sqrt(1.0 + dh * dh);
                } else {
                    float vv, dy;
-                   edge32<false, true>(G, i, ip, v.a, v.b, vv, dy);
+                   if constexpr (MID) edge32_mid<false, true>(G, i, ip, v.a, v.b, vv, dy);
+                   else edge32_grid<false, true>(G, i, ip, v.a, v.b, vv, dy);
                    edy[i] = dy;
                    const float dh = (v.b - v.a) * G.inv_dx;
                    if (arc) esl[i] = dh * rsqrtf(1.0f + dh * dh);
@@ -602,7 +675,7 @@ __device__ void wg_precondition(int n, double kdx, double shift, C2<T> *buf, con
     __syncthreads();
 }
 
-template <typename T, typename C>
+template <typename T, typename C, bool MID = false>
 __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, double *rho, double *la,
                            int *slot, int *sbuf, C2<T> *fftbuf) {
     const qew_lbfgs_params &lp = a.lp;
@@ -637,7 +710,7 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
         }
     }
 
-    wg_gradient<T, C>(G, x, g, edy, esl);
+    wg_gradient<T, C, MID>(G, x, g, edy, esl);
     qew_lbfgs_result res{};
     int nslots = 0;  // slot[0..nslots) chronological rows, slot[nslots-1] newest
     double gamma = 1.0;
@@ -761,7 +834,7 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
             dphi0 = -acc.reduce(r);
         }
         if (it == 0) {
-            phi = wg_objective<T, C>(G, x, nullptr, 0.0, r);
+            phi = wg_objective<T, C, MID>(G, x, nullptr, 0.0, r);
             ++res.n_func_evals;
         }
         const double phi0 = phi;
@@ -778,7 +851,7 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
             al[0] = alpha;
 #pragma unroll
             for (int j = 1; j < K; ++j) al[j] = al[j - 1] * lp.ls_shrink;
-            wg_objective_trials<T, C>(G, x, d, al, ph, r);
+            wg_objective_trials<T, C, MID>(G, x, d, al, ph, r);
 #pragma unroll
             for (int j = 0; j < K; ++j) {
                 if (ok || ls >= lp.max_ls) continue;
@@ -819,7 +892,7 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
             st[i] = (sizeof(T) == 8) ? alT * v.b : xn - v.a;
         });
         __syncthreads();
-        wg_gradient<T, C>(G, x, gn, edy, esl);
+        wg_gradient<T, C, MID>(G, x, gn, edy, esl);
         double sy = 0, ss = 0, yy = 0;
         stream(n, [&](int i) { return P3<T>{st[i], gn[i], g[i]}; }, [&](int i, P3<T> v) {
             const T y = v.b - v.c;
@@ -879,7 +952,7 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
     __syncthreads();
 }
 
-template <typename T, typename C>
+template <typename T, typename C, bool MID = false>
 __global__ __launch_bounds__(kLB) void lbfgs_batch_kernel(BatchArgs a) {
     extern __shared__ unsigned char lds[];
     __shared__ double s_red[kLS * (kLB / kWave)];
@@ -894,7 +967,7 @@ __global__ __launch_bounds__(kLB) void lbfgs_batch_kernel(BatchArgs a) {
         const int qi = s_line;
         __syncthreads();
         if (qi >= a.nlines) return;
-        solve_line<T, C>(a, a.order[qi], r, rho, la, slot, sbuf, reinterpret_cast<C2<T> *>(lds));
+        solve_line<T, C, MID>(a, a.order[qi], r, rho, la, slot, sbuf, reinterpret_cast<C2<T> *>(lds));
     }
 }
 
@@ -1034,15 +1107,19 @@ extern "C" qew_status qew_batch_lbfgs(qew_batch b, int dtype, const qew_lbfgs_pa
         QEW_REQUIRE(b && lp && h && results, "null pointer");
         QEW_REQUIRE(dtype == QEW_F64 || dtype == QEW_F32, "dtype must be QEW_F64 or QEW_F32");
         QEW_REQUIRE(lp->m >= 1 && lp->m <= kMaxM, "history length m must be in [1, 32]");
-        require_wide_tables(b->noises);
+        if (dtype == QEW_F32) {
+            // FP32 tables, or midpoint-contracted ones on lines of their geometry
+            for (const auto &nd : b->noises)
+                QEW_REQUIRE(nd.kind == TABLE && (nd.dtype == QEW_F32 || nd.dtype == QEW_F32_MID),
+                            "QEW_F32 minimisation needs FP32 or QEW_F32_MID coefficient tables");
+            require_f32_tables(b->lines, b->noises, "QEW_F32 minimisation");
+        } else {
+            require_wide_tables(b->noises);
+        }
         const int tdt = b->noises[0].kind == TABLE ? b->noises[0].dtype : 0;
-        for (const auto &nd : b->noises) {
+        for (const auto &nd : b->noises)
             QEW_REQUIRE(nd.kind != TABLE || nd.dtype == tdt,
                         "all tables of a batch must share one storage type");
-            if (dtype == QEW_F32)
-                QEW_REQUIRE(nd.kind == TABLE && nd.dtype == 1,
-                            "QEW_F32 minimisation needs FP32 coefficient tables");
-        }
         if (lp->precondition)
             for (const auto &l : b->lines)
                 QEW_REQUIRE(l.n >= 4 && (l.n & (l.n - 1)) == 0 && l.n / 2 <= kMaxFftN,
@@ -1105,7 +1182,8 @@ extern "C" qew_status qew_batch_lbfgs(qew_batch b, int dtype, const qew_lbfgs_pa
             hipLaunchKernelGGL(kernel, dim3(grid), dim3(kLB), lds, ctx->stream, a);
             QEW_HIP(hipGetLastError());
         };
-        if (dtype == QEW_F32) launch(lbfgs_batch_kernel<float, float>);
+        if (dtype == QEW_F32 && tdt == QEW_F32_MID) launch(lbfgs_batch_kernel<float, float, true>);
+        else if (dtype == QEW_F32) launch(lbfgs_batch_kernel<float, float>);
         else if (tdt == 1) launch(lbfgs_batch_kernel<double, float>);
         else launch(lbfgs_batch_kernel<double, double>);
         QEW_HIP(hipMemcpyAsync(results, a.results, sizeof(qew_lbfgs_result) * b->nlines,

@@ -136,6 +136,40 @@ def test_mid_batch_langevin_plans_bitwise_and_close_to_fp64(q):
     assert rel(b.var(), a.var()) < 1e-4
 
 
+def test_mid_batch_lbfgs_energy_error(q):
+    """The batched FP32 minimisation on midpoint tables (the FP32-table test's
+    problem, test_gpu_batch_lbfgs.py): every minimum within 1e-6 relative
+    energy of the FP64 oracle's, evaluated by the oracle."""
+    LT = 4.0 ** 1.5
+    lengths = [1024, 2048, 512, 1024, 4096, 256]
+    xis = [2.0, 4.0, 8.0, 4.0, 8.0, 2.0]
+    models = [0, 1, 0, 1, 1, 0]
+    ny = 256
+    noises, oracle, params, h0 = [], [], [], []
+    for n, xi, model, seed in zip(lengths, xis, models, [5, 6, 7, 8, 9, 10]):
+        field, coef = O.filtered_noise(n, ny, float(n), float(ny), 1.0, xi, xi, seed)
+        oracle.append((O.OracleNoise.table(coef, float(n), float(ny)), O.params(float(n), LT, 0.0, model)))
+        noises.append(q.FilteredNoise.from_coefficients(coef, 1.0, 1.0).convert(q.F32_MID))
+        params.append(q.Params(float(n), LT, 0.0, q.Model(model)))
+        h0.append(np.full(n, ny / 2.0))
+    b = q.Batch(lengths, noises, params)
+    h = dev(np.concatenate(h0))
+    res = b.lbfgs(h, q.LbfgsParams(m=10, ftol=1e-4, max_iter=4000, precondition=True,
+                                   precond_shift=0.5 / 16.0), dtype=q.F32)
+    hh = host(h)
+    for k, n in enumerate(lengths):
+        on, po = oracle[k]
+        h_ref, r_ref = O.lbfgs(h0[k], po, on, O.lbfgs_params(m=10, ftol=1e-6, max_iter=4000,
+                                                             precondition=True,
+                                                             precond_shift=0.5 / 16.0))
+        line = hh[b.offsets[k]:b.offsets[k + 1]]
+        e, e_ref = O.objective(line, po, on), O.objective(h_ref, po, on)
+        assert rel(e, e_ref) < 1e-6, (k, res[k], e, e_ref)
+    # FP64 line state does not read midpoint tables
+    with pytest.raises(q.QewInvalidArgument):
+        b.lbfgs(dev(np.concatenate(h0)), q.LbfgsParams(m=10, precondition=True), dtype=q.F64)
+
+
 def test_mid_tables_rejected_elsewhere_and_for_other_geometries(q):
     n, ny = 64, 32
     field, coef = O.filtered_noise(n, ny, float(n), float(ny), 1.0, 4.0, 4.0, 11)
@@ -166,3 +200,5 @@ def test_mid_tables_rejected_elsewhere_and_for_other_geometries(q):
             bb.eval(o, dtype=q.F32, anchor=anchor)
         with pytest.raises(q.QewInvalidArgument):
             bb.langevin(o, [0.01], [0.0], seed=1, dtype=q.F32, anchor=anchor)
+        with pytest.raises(q.QewInvalidArgument):
+            bb.lbfgs(dev(np.full(L, ny / 2.0)), q.LbfgsParams(m=4, precondition=False), dtype=q.F32)
