@@ -310,13 +310,15 @@ def bench_rk(log, nlines=64, budget_s=20.0):
     specs = [(Lr, ny, float(Lr), float(ny), 1.0, 4.0, 4.0, 1 + k) for k in range(nlines)]
     n64 = q.FilteredNoise.generate_batch(specs, dtype=q.F64)
     n32 = [nz.convert(q.F32) for nz in n64]
+    # lines of n = L sites on L-wide tables of spacing 1: midpoint tables apply
+    nmid = [nz.convert(q.F32_MID) for nz in n64]
     torch.cuda.synchronize()
     log(f"rk: {nlines} tables generated in {time.perf_counter() - t0:.1f} s")
     params = [q.Params(float(Lr), 1.0, 0.0, q.Model.Linear)] * nlines
     rp = q.RkParams(0.5, float(ny), 1e-10, 1e-8, 2.0 * ny, 200000)
     out = {}
     brackets = {}
-    for name, noises in (("f32", n32), ("f64", n64)):
+    for name, noises in (("f32", n32), ("f64", n64), ("f32_mid", nmid)):
         b = q.Batch([Lr] * nlines, noises, params)
         h_lo = torch.full((nlines * Lr,), ny / 2.0, dtype=torch.float64, device="cuda:0")
         lo = np.zeros(nlines)
@@ -362,6 +364,9 @@ def bench_rk(log, nlines=64, budget_s=20.0):
     same = (np.array_equal(brackets["f32"][0], brackets["f64"][0])
             and np.array_equal(brackets["f32"][1], brackets["f64"][1]))
     out["f32_f64_same_brackets"] = bool(same)
+    out["f32_mid_f64_same_brackets"] = bool(
+        np.array_equal(brackets["f32_mid"][0], brackets["f64"][0])
+        and np.array_equal(brackets["f32_mid"][1], brackets["f64"][1]))
     out["workload"] = (f"{nlines} lines x L={Lr}, tables {Lr}x{ny} (period M={ny}), xi=4, "
                        f"linear, f_c ramp + bisection, {budget_s:g} s budget per precision")
     out["velocity_evals_note"] = ("evaluations the reference algorithm makes (sites whose "

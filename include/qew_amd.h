@@ -45,8 +45,9 @@ typedef enum qew_status {
  *    lies at x-fraction 1/2, so its 4 x 4 gather reduces to 4 values of T.
  * Such tables are read by qew_batch_eval and qew_batch_langevin with QEW_F32
  * line state, and QEW_F32_MID tables also by qew_batch_lbfgs with QEW_F32
- * (QEW_F32_MID only for lines with that geometry); every other entry point
- * rejects them with QEW_ERR_INVALID_ARGUMENT. */
+ * and by qew_batch_rk_block (FP64 arithmetic on the FP32 values, as on FP32
+ * tables) (QEW_F32_MID only for lines with that geometry); every other entry
+ * point rejects them with QEW_ERR_INVALID_ARGUMENT. */
 typedef enum qew_dtype { QEW_F64 = 0, QEW_F32 = 1, QEW_BF16 = 2, QEW_F32_MID = 3 } qew_dtype;
 
 /* qew::Model (src/qew_model.h:18). */

@@ -52,7 +52,13 @@ using cplx = std::complex<double>;
 // (reference tests/test_qew_model.cpp:22-42, tests/test_lbfgs.cpp:24-38,
 // tests/test_dynamics.cpp:20-28, tests/test_rosso_krauth.cpp:24-31).
 // ---------------------------------------------------------------------------
-enum Kind { TABLE = 0, SINCOS = 1, CONST = 2, ZERO = 3, HARMONIC = 4, WASHBOARD = 5 };
+// MIDTABLE: coef holds T(ix, iy) = sum_a w_a(1/2) c(ix - 1 + a, iy), the
+// x-sum of the bicubic at x-fraction 1/2 (no reference counterpart: the
+// device's midpoint-contracted FP32 tables, sampled only at points whose
+// x-fraction is 1/2, i.e. at the edge midpoints of lines whose site spacing
+// equals the table's; there V = sum_b T(ix, iy_b) wy_b exactly).
+enum Kind { TABLE = 0, SINCOS = 1, CONST = 2, ZERO = 3, HARMONIC = 4, WASHBOARD = 5,
+            MIDTABLE = 6 };
 
 }  // namespace
 
@@ -158,12 +164,34 @@ void orc_locate(const orc_noise *nz, double x, double y, int *ixn, int *iyn,
     iyn[3] = (iyn[2] + 1 == ny) ? 0 : iyn[2] + 1;
 }
 
+// V and dV/dy from a MIDTABLE at (x, y): the cell of orc_locate, the y
+// weights, the column's 4 values summed b = 0..3 (dV/dx is not available).
+static void orc_mid(const orc_noise *nz, double x, double y, double *v, double *dvdy) {
+    int ixn[4], iyn[4];
+    double fx, fy;
+    orc_locate(nz, x, y, ixn, iyn, &fx, &fy);
+    double wy[4], dwy[4];
+    orc_bspline_weights(fy, wy, dwy);
+    double vv = 0, d = 0;
+    for (int b = 0; b < 4; ++b) {
+        const double t = nz->coef[(size_t)ixn[1] * nz->ny + iyn[b]];
+        vv += t * wy[b];
+        d += t * dwy[b];
+    }
+    *v = vv;
+    *dvdy = d / nz->dy;
+}
+
 // ---- full sample / value / dy: src/filtered_noise.h:103-176 plus the
 //      analytic test functors' sample() ----
 void orc_sample(const orc_noise *nz, double x, double y, double *v, double *dvdx,
                 double *dvdy) {
     const double *p = nz->p;
     switch (nz->kind) {
+    case MIDTABLE:
+        orc_mid(nz, x, y, v, dvdy);
+        *dvdx = std::nan("");
+        return;
     case SINCOS:
         *v = p[0] * std::sin(p[1] * x) * std::cos(p[2] * y);
         *dvdx = p[0] * p[1] * std::cos(p[1] * x) * std::cos(p[2] * y);
@@ -210,6 +238,11 @@ void orc_sample(const orc_noise *nz, double x, double y, double *v, double *dvdx
 }
 
 double orc_sample_value(const orc_noise *nz, double x, double y) {
+    if (nz->kind == MIDTABLE) {
+        double v, d;
+        orc_mid(nz, x, y, &v, &d);
+        return v;
+    }
     if (nz->kind != TABLE) {
         double v, a, b;
         orc_sample(nz, x, y, &v, &a, &b);
@@ -232,6 +265,11 @@ double orc_sample_value(const orc_noise *nz, double x, double y) {
 }
 
 double orc_sample_dy(const orc_noise *nz, double x, double y) {
+    if (nz->kind == MIDTABLE) {
+        double v, d;
+        orc_mid(nz, x, y, &v, &d);
+        return d;
+    }
     if (nz->kind != TABLE) {
         double v, a, b;
         orc_sample(nz, x, y, &v, &a, &b);

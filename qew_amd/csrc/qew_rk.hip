@@ -17,6 +17,7 @@
 // cooperatively (forward_zero_group). The line state, the advance sums, the
 // v <= 0 tests and the bisection are FP64; the tables may be FP64 or FP32.
 #include <algorithm>
+#include <type_traits>
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
@@ -55,6 +56,41 @@ struct RkArgs {
     qew_rk_params rp;
 };
 
+// Coefficient-type tag of midpoint-contracted FP32 tables (dtype 3): one
+// value per cell, the bicubic's x-sum at x-fraction 1/2, which is every edge
+// midpoint's fraction on the lines such tables are accepted for.
+struct MidF32 {};
+
+// Column of the x cell of an edge midpoint (locate64's x part).
+__device__ __forceinline__ int mid_cell_x(const NoiseDesc &nd, double x) {
+    double gx = (nd.dx == 1.0) ? x : x / nd.dx;
+    gx = wrap_cells(gx, nd.nx);
+    int ix = (int)floor(gx);
+    if (ix >= nd.nx) ix -= nd.nx;
+    return ix;
+}
+
+// dV/dy at an edge midpoint: sample_dy64 on FP64/FP32 tables and analytic
+// potentials; on a midpoint-contracted table the y weights of locate64 /
+// bspline_dw applied to the 4 values of the column, summed b = 0..3 in FP64.
+template <typename C>
+__device__ __forceinline__ double edge_dy(const NoiseDesc &nd, double x, double y) {
+    if constexpr (std::is_same<C, MidF32>::value) {
+        int iyn[4];
+        double fy, dwy[4];
+        mid_locate_y64(nd, y, iyn, fy);
+        bspline_dw(fy, dwy);
+        float c[4];
+        mid_column4(nd, mid_cell_x(nd, x), iyn, c);
+        double d = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) d += (double)c[b] * dwy[b];
+        return (nd.dy == 1.0) ? d : d / nd.dy;
+    } else {
+        return sample_dy64<C>(nd, x, y);
+    }
+}
+
 // rk_site_velocity (src/rosso_krauth.h:52-82)
 template <typename C>
 __device__ __forceinline__ double site_velocity(int i, int n, double dx, double lt, double f,
@@ -74,8 +110,8 @@ __device__ __forceinline__ double site_velocity(int i, int n, double dx, double 
     const double hcl = (hl + hi) * 0.5;
     const double xcr = (double)(i + ip) * 0.5 * dx;
     const double hcr = (hi + hr) * 0.5;
-    const double dvl = sample_dy64<C>(nd, xcl, hcl);
-    const double dvr = sample_dy64<C>(nd, xcr, hcr);
+    const double dvl = edge_dy<C>(nd, xcl, hcl);
+    const double dvr = edge_dy<C>(nd, xcr, hcr);
     const double grad = dx * (line_force + 0.5 * (dvl + dvr) - f);
     return -grad;
 }
@@ -116,6 +152,29 @@ struct EdgeDy {
             for (int b = 0; b < 4; ++b) row_dy += (double)c[a][b] * dwy[b];
             d += wx[a] * row_dy;
         }
+        return (nd.dy == 1.0) ? d : d / nd.dy;
+    }
+};
+
+// The same for a midpoint-contracted table: the held block is the column's
+// 4 values at the last y cell (same locate, weights and order as edge_dy).
+template <>
+struct EdgeDy<MidF32> {
+    int iy = -1;
+    float c[4];
+
+    __device__ __forceinline__ double eval(const NoiseDesc &nd, double x, double y) {
+        int iyn[4];
+        double fy, dwy[4];
+        mid_locate_y64(nd, y, iyn, fy);
+        bspline_dw(fy, dwy);
+        if (iyn[1] != iy) {
+            iy = iyn[1];
+            mid_column4(nd, mid_cell_x(nd, x), iyn, c);
+        }
+        double d = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) d += (double)c[b] * dwy[b];
         return (nd.dy == 1.0) ? d : d / nd.dy;
     }
 };
@@ -597,7 +656,12 @@ void run_rk(qew_ctx ctx, const LineDesc *d_lines, const NoiseDesc *d_noises, int
     if (line) {
         const size_t lds = (sizeof(double) + 1) * (size_t)max_n;
         QEW_HIP(hipMemsetAsync(w.state, 0, sizeof(RkState) * (size_t)nlines, s));
-        if (table_dtype == 1) {
+        if (table_dtype == QEW_F32_MID) {
+            QEW_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(rk_line_kernel<MidF32>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL(rk_line_kernel<MidF32>, dim3(nlines), dim3(kRkLineBlock), lds, s,
+                               a, max_n);
+        } else if (table_dtype == 1) {
             QEW_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(rk_line_kernel<float>),
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             hipLaunchKernelGGL(rk_line_kernel<float>, dim3(nlines), dim3(kRkLineBlock), lds, s, a,
@@ -620,7 +684,10 @@ void run_rk(qew_ctx ctx, const LineDesc *d_lines, const NoiseDesc *d_noises, int
             const int todo = std::min(poll, rp.max_sweeps - sweep);
             for (int k = 0; k < todo; ++k) {
                 for (int color = 0; color < 2; ++color) {
-                    if (table_dtype == 1)
+                    if (table_dtype == QEW_F32_MID)
+                        hipLaunchKernelGGL(rk_color_kernel<MidF32>, grid, dim3(kRkBlock), 0, s, a,
+                                           color);
+                    else if (table_dtype == 1)
                         hipLaunchKernelGGL(rk_color_kernel<float>, grid, dim3(kRkBlock), 0, s, a,
                                            color);
                     else
@@ -703,7 +770,12 @@ qew_status qew_batch_rk_block(qew_batch b, const qew_rk_params *rp, double *h,
         for (const auto &nd : b->noises)
             QEW_REQUIRE(nd.kind != TABLE || nd.dtype == b->noises[0].dtype,
                         "all tables of a batch must share one storage type");
-        require_wide_tables(b->noises);
+        // FP64 or FP32 tables (or analytic potentials), or midpoint-contracted
+        // FP32 tables on lines of their geometry
+        if (b->noises[0].kind == TABLE && b->noises[0].dtype == QEW_F32_MID)
+            require_f32_tables(b->lines, b->noises, "rk_block");
+        else
+            require_wide_tables(b->noises);
         qew_ctx ctx = b->ctx;
         use_device(ctx);
         b->d_scratch.ensure(rk_scratch_bytes(b->nlines, b->total, rk_tiles(b->max_n)));

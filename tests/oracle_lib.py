@@ -12,7 +12,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 
-TABLE, SINCOS, CONST, ZERO, HARMONIC, WASHBOARD = range(6)
+TABLE, SINCOS, CONST, ZERO, HARMONIC, WASHBOARD, MIDTABLE = range(7)
 LINEAR, ARCLENGTH = 0, 1
 
 
@@ -135,6 +135,13 @@ class OracleNoise:
     def table(cls, coef, Lx, Ly):
         nx, ny = coef.shape
         return cls(TABLE, coef, nx, ny, Lx / nx, Ly / ny)
+
+    @classmethod
+    def mid_table(cls, values, Lx, Ly):
+        """Midpoint-contracted values T[ix, iy] (row-major) of a table of
+        spacing Lx / nx, Ly / ny: sampled only at x-fraction 1/2."""
+        nx, ny = values.shape
+        return cls(MIDTABLE, values, nx, ny, Lx / nx, Ly / ny)
 
     @property
     def ref(self):

@@ -170,6 +170,38 @@ def test_mid_batch_lbfgs_energy_error(q):
         b.lbfgs(dev(np.concatenate(h0)), q.LbfgsParams(m=10, precondition=True), dtype=q.F64)
 
 
+@pytest.mark.parametrize("path", ["line", "launch"])
+def test_mid_batch_rk_bitwise_vs_oracle(q, monkeypatch, path):
+    """Rosso-Krauth sweeps on a midpoint table (FP64 arithmetic on the FP32
+    values, as on FP32 tables) are bit for bit the oracle's on the same
+    values (its MIDTABLE sampler, pinned against the full bicubic on the CPU):
+    heights, blocked/runaway flags, sweeps and velocity-evaluation counts, on
+    both device paths, for lines that block and a line that runs away."""
+    nx, ny = 64, 64
+    field, coef = O.filtered_noise(nx, ny, 3.2, 1.0, 0.6, 0.2, 0.2, 29)
+    gmid = q.FilteredNoise.from_coefficients(coef, 0.05, 1.0 / ny).convert(q.F32_MID)
+    on = O.OracleNoise.mid_table(gmid.coefficients(), 3.2, 1.0)
+    forces = [0.0, 0.1, 5.0, 20.0]  # the last runs away
+    models = [q.Model.Linear, q.Model.Arclength, q.Model.Linear, q.Model.Arclength]
+    params = [q.Params(3.2, 1.2, f, m) for f, m in zip(forces, models)]
+    rp = q.RkParams(0.5 / 64, 1.0, 1e-11, 1e-9, 3.0)
+    rpo = O.rk_params(dstep=0.5 / 64, max_advance=1.0, root_tol=1e-11, sweep_tol=1e-9, runaway=3.0)
+    b = q.Batch([nx] * len(forces), [gmid] * len(forces), params)
+    hb = dev(np.full(nx * len(forces), 0.5))
+    monkeypatch.setenv("QEW_RK_KERNEL", path)
+    res = b.rk_block(hb, rp)
+    monkeypatch.delenv("QEW_RK_KERNEL", raising=False)
+    hh = host(hb)
+    for k, (f, m) in enumerate(zip(forces, models)):
+        h_ref = np.full(nx, 0.5)
+        r_ref = O.rk_block(h_ref, O.params(3.2, 1.2, f, int(m)), on, rpo)
+        assert (res[k].blocked, res[k].runaway, res[k].sweeps) == \
+            (bool(r_ref.blocked), bool(r_ref.runaway), r_ref.sweeps), k
+        assert res[k].n_velocity_evals == r_ref.n_velocity_evals, k
+        np.testing.assert_array_equal(hh[k * nx:(k + 1) * nx], h_ref, err_msg=str(k))
+    assert any(r.blocked for r in res) and any(r.runaway for r in res)
+
+
 def test_mid_tables_rejected_elsewhere_and_for_other_geometries(q):
     n, ny = 64, 32
     field, coef = O.filtered_noise(n, ny, float(n), float(ny), 1.0, 4.0, 4.0, 11)
@@ -186,7 +218,7 @@ def test_mid_tables_rejected_elsewhere_and_for_other_geometries(q):
             call()
     b = q.Batch([n], [gmid], [p])
     for call in (lambda: b.eval(h), lambda: b.lbfgs(h, q.LbfgsParams(m=4, precondition=False)),
-                 lambda: b.rk_block(h, rk), lambda: b.langevin(h, [0.01], [0.0], seed=1)):
+                 lambda: b.langevin(h, [0.01], [0.0], seed=1)):
         with pytest.raises(q.QewInvalidArgument):
             call()
     off = dev(np.zeros(n, dtype=np.float32))
@@ -202,3 +234,6 @@ def test_mid_tables_rejected_elsewhere_and_for_other_geometries(q):
             bb.langevin(o, [0.01], [0.0], seed=1, dtype=q.F32, anchor=anchor)
         with pytest.raises(q.QewInvalidArgument):
             bb.lbfgs(dev(np.full(L, ny / 2.0)), q.LbfgsParams(m=4, precondition=False), dtype=q.F32)
+        if L % 2 == 0:  # (odd lengths are rejected by the red-black sweep anyway)
+            with pytest.raises(q.QewInvalidArgument):
+                bb.rk_block(dev(np.full(L, ny / 2.0)), rk)

@@ -235,3 +235,38 @@ def test_filtered_noise_amplitude_and_band_limit():
     Fx, Fy = np.meshgrid(fx, fy, indexing="ij")
     masked = (Fx * xx) ** 2 + (Fy * xy) ** 2 > 1
     assert H[masked].max() < 1e-8 * H.max()
+
+
+def _mid_fold(coef):
+    """The x-sum of the bicubic at x-fraction 1/2 per (ix, iy), FP64: the
+    cardinal cubic B-spline's values at offsets -3/2, -1/2, 1/2, 3/2 times
+    coef(ix - 1 .. ix + 2)."""
+    w = beta3(np.array([1.5, 0.5, -0.5, -1.5]))
+    return sum(w[a] * np.roll(coef, 1 - a, axis=0) for a in range(4))
+
+
+@pytest.mark.parametrize("nx,ny,dx,dy", [(48, 40, 1.0, 1.0), (64, 32, 0.05, 0.25)])
+def test_oracle_midpoint_table_equals_full_sampler_at_half_cells(nx, ny, dx, dy):
+    """The oracle's MIDTABLE sampler (T = the x-contracted table) gives the
+    full bicubic's V and dV/dy at every point with x-fraction 1/2, including
+    the cell of the wrap edge ((n - 1) / 2), for any y; the weights w(+-1/2),
+    w(+-3/2) are (23, 1)/48. A column off by one or a transposed table fails."""
+    rng = np.random.default_rng(nx)
+    coef = rng.standard_normal((nx, ny))
+    full = O.OracleNoise.table(coef, nx * dx, ny * dy)
+    mid = O.OracleNoise.mid_table(_mid_fold(coef), nx * dx, ny * dy)
+    np.testing.assert_allclose(beta3(np.array([1.5, 0.5])) * 48, [1, 23], rtol=0, atol=1e-13)
+    i = np.concatenate([np.arange(nx - 1), [nx - 1]])
+    ip = (i + 1) % nx
+    x = (i + ip) * 0.5 * dx
+    y = rng.uniform(-2 * ny * dy, 3 * ny * dy, x.size)
+    v, _, dv = O.sample_points(full, x, y)
+    vm, dxm, dvm = O.sample_points(mid, x, y)
+    scale = np.max(np.abs(v))
+    np.testing.assert_allclose(vm, v, rtol=0, atol=1e-13 * scale)
+    np.testing.assert_allclose(dvm, dv, rtol=0, atol=1e-13 * np.max(np.abs(dv)))
+    assert np.all(np.isnan(dxm))
+    # a planted off-by-one column shift is caught
+    bad = O.OracleNoise.mid_table(np.roll(_mid_fold(coef), 1, axis=0), nx * dx, ny * dy)
+    vb, _, _ = O.sample_points(bad, x, y)
+    assert np.max(np.abs(vb - v)) > 1e-3 * scale
