@@ -186,18 +186,18 @@ __device__ __forceinline__ void block_sumk(double *v, double *red) {
     }
 }
 
-template <typename T>
+template <int U = kU, typename T>
 __device__ __forceinline__ double wg_dot(const T *a, const T *b, int n, Red r) {
     Acc<T> acc;
-    stream(n, [&](int i) { return P2<T>{a[i], b[i]}; },
-           [&](int, P2<T> v) { acc.add(v.a * v.b); });
+    stream<U>(n, [&](int i) { return P2<T>{a[i], b[i]}; },
+              [&](int, P2<T> v) { acc.add(v.a * v.b); });
     return acc.reduce(r);
 }
 
-template <typename T>
+template <int U = kU, typename T>
 __device__ __forceinline__ double wg_maxabs(const T *a, int n, Red r) {
     double m = 0;
-    stream(n, [&](int i) { return a[i]; }, [&](int, T v) {
+    stream<U>(n, [&](int i) { return a[i]; }, [&](int, T v) {
         const double f = fabs((double)v);
         m = f > m ? f : m;
     });
@@ -678,6 +678,10 @@ __device__ void wg_precondition(int n, double kdx, double shift, C2<T> *buf, con
 template <typename T, typename C, bool MID = false>
 __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, double *rho, double *la,
                            int *slot, int *sbuf, C2<T> *fftbuf) {
+    // elements per thread with loads in flight in the vector passes: 16 for
+    // the midpoint-table kernel (its line search frees the registers), 8
+    // elsewhere (wider passes spill); the per-thread order is the same
+    constexpr int UV = MID ? 16 : kU;
     const qew_lbfgs_params &lp = a.lp;
     const LineDesc L = a.lines[line];
     const NoiseDesc nd = a.noises[L.noise];
@@ -731,7 +735,7 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
         auto first_pass = [&](T *qv) {
             const T *S0 = hs > 0 ? Sb(sbuf[slot[hs - 1]]) : nullptr;
             Acc<T> acc;
-            stream(n, [&](int i) { return P2<T>{g[i], S0 ? S0[i] : (T)0}; },
+            stream<UV>(n, [&](int i) { return P2<T>{g[i], S0 ? S0[i] : (T)0}; },
                    [&](int i, P2<T> v) {
                        const double f = fabs((double)v.a);
                        gmax = f > gmax ? f : gmax;
@@ -761,7 +765,7 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
                 const T *Y = Yb(sbuf[row]);
                 const T *Sn = idx > 0 ? Sb(sbuf[slot[idx - 1]]) : nullptr;
                 Acc<T> acc;
-                stream(n, [&](int i) { return P3<T>{qv[i], Y[i], Sn ? Sn[i] : (T)0}; },
+                stream<UV>(n, [&](int i) { return P3<T>{qv[i], Y[i], Sn ? Sn[i] : (T)0}; },
                        [&](int i, P3<T> v) {
                            const T qn = v.a + c * v.b;
                            qv[i] = qn;
@@ -777,7 +781,7 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
         } else {
             first_loop(q);
             const T g0 = (T)(nslots == 0 ? 1.0 : gamma);
-            stream(n, [&](int i) { return q[i]; }, [&](int i, T v) { d[i] = g0 * v; });
+            stream<UV>(n, [&](int i) { return q[i]; }, [&](int i, T v) { d[i] = g0 * v; });
             __syncthreads();
         }
         // second loop, oldest pair first: d += (la - b) S with b = rho Y . d;
@@ -785,7 +789,7 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
         double dphi0 = 0;
         auto second_loop = [&](T *dv) {
             double b_dot = 0;
-            if (hs > 0) b_dot = wg_dot(Yb(sbuf[slot[0]]), dv, n, r);
+            if (hs > 0) b_dot = wg_dot<UV>(Yb(sbuf[slot[0]]), dv, n, r);
             for (int idx = 0; idx < hs; ++idx) {
                 const int row = slot[idx];
                 const double bb = rho[row] * b_dot;
@@ -797,7 +801,7 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
                 // separate call sites for the two destinations keep each
                 // store in its own address space
                 auto pass = [&](T *dout, bool neg) {
-                    stream(n, [&](int i) { return P3<T>{dv[i], S[i], Yn[i]}; },
+                    stream<UV>(n, [&](int i) { return P3<T>{dv[i], S[i], Yn[i]}; },
                            [&](int i, P3<T> v) {
                                T dn = v.a + c * v.b;
                                if (neg) dn = -dn;
@@ -813,7 +817,7 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
             }
             if (hs == 0) {
                 Acc<T> acc;
-                stream(n, [&](int i) { return P2<T>{dv[i], g[i]}; },
+                stream<UV>(n, [&](int i) { return P2<T>{dv[i], g[i]}; },
                        [&](int i, P2<T> v) {
                            const T dn = -v.a;
                            d[i] = dn;
@@ -827,7 +831,7 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
         if (dphi0 >= 0) {
             nslots = 0;
             Acc<T> acc;
-            stream(n, [&](int i) { return g[i]; }, [&](int i, T v) {
+            stream<UV>(n, [&](int i) { return g[i]; }, [&](int i, T v) {
                 d[i] = -v;
                 acc.add(v * v);
             });
@@ -839,7 +843,7 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
         }
         const double phi0 = phi;
         double alpha = 1.0;
-        if (!lp.precondition && nslots == 0) alpha = 1.0 / fmax(1.0, sqrt(wg_dot(g, g, n, r)));
+        if (!lp.precondition && nslots == 0) alpha = 1.0 / fmax(1.0, sqrt(wg_dot<UV>(g, g, n, r)));
         // Armijo backtracking alpha, alpha s, alpha s^2, ... (the sequence and
         // the first accepted step of the reference loop); the energies of K
         // consecutive trials come from one pass, the evaluation count is the
@@ -886,7 +890,7 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
         // near convergence alpha d falls below the offsets' resolution and
         // the intended step no longer matches the realised one.
         T *st = Sb(sbuf[m]), *yt = Yb(sbuf[m]);
-        stream(n, [&](int i) { return P2<T>{x[i], d[i]}; }, [&](int i, P2<T> v) {
+        stream<UV>(n, [&](int i) { return P2<T>{x[i], d[i]}; }, [&](int i, P2<T> v) {
             const T xn = v.a + alT * v.b;
             x[i] = xn;
             st[i] = (sizeof(T) == 8) ? alT * v.b : xn - v.a;
@@ -894,7 +898,7 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
         __syncthreads();
         wg_gradient<T, C, MID>(G, x, gn, edy, esl);
         double sy = 0, ss = 0, yy = 0;
-        stream(n, [&](int i) { return P3<T>{st[i], gn[i], g[i]}; }, [&](int i, P3<T> v) {
+        stream<UV>(n, [&](int i) { return P3<T>{st[i], gn[i], g[i]}; }, [&](int i, P3<T> v) {
             const T y = v.b - v.c;
             yt[i] = y;
             sy += (double)v.a * (double)y;
@@ -907,7 +911,7 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
             // everywhere, so the line can no longer move; stop (not
             // converged) instead of iterating in place until max_iter
             res.iterations = it + 1;
-            res.max_force = wg_maxabs(gn, n, r);
+            res.max_force = wg_maxabs<UV>(gn, n, r);
             res.converged = res.max_force < lp.ftol ? 1 : 0;
             g = gn;
             finished = true;
@@ -944,7 +948,7 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
     }
     if (!finished) {
         res.iterations = lp.max_iter;
-        res.max_force = wg_maxabs(g, n, r);
+        res.max_force = wg_maxabs<UV>(g, n, r);
         res.converged = 0;
     }
     for (int i = tid; i < n; i += blockDim.x) h[i] = anchor + (double)x[i];
