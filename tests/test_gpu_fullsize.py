@@ -62,6 +62,17 @@ def test_full_size_eval_matches_oracle(q):
     e32, g32 = b32.eval(off, dtype=q.F32, anchor=dev(np.array([NY / 2])))
     assert np.max(np.abs(host(g32) - g_ref)) < 2e-4 * np.max(np.abs(g_ref))
     assert rel(host(e32)[0], e_ref) < 1e-5
+    del n32, b32
+
+    # the midpoint-contracted FP32 table (every edge of this line lies at
+    # x-fraction 1/2): the same stated FP32 tolerance, and no larger error
+    # than the FP32 table's beyond FP32 rounding
+    nmid = noise.convert(q.F32_MID)
+    em, gm = q.Batch([L], [nmid], [p]).eval(off, dtype=q.F32, anchor=dev(np.array([NY / 2])))
+    err_mid = np.max(np.abs(host(gm) - g_ref))
+    assert err_mid < 2e-4 * np.max(np.abs(g_ref))
+    assert err_mid <= 2 * np.max(np.abs(host(g32) - g_ref)) + 1e-7 * np.max(np.abs(g_ref))
+    assert rel(host(em)[0], e_ref) < 1e-5
 
 
 def test_full_size_langevin_matches_oracle(q):
@@ -102,8 +113,10 @@ def test_full_size_rk_matches_oracle(q):
     16384 x 1024 table, period M = 1024 in h, xi = 4, seed 1, linear, line
     tension 1, the depinning tool's knobs): relaxation at f = 0 and the first
     ramp probe f = 0.25 from the flat line at M/2, on the FP64 table and on
-    its FP32 copy (the oracle then runs on the FP32-rounded coefficients):
-    heights, sweep counts and evaluation counts identical to the oracle."""
+    its FP32 copy (the oracle then runs on the FP32-rounded coefficients) and
+    on its midpoint-contracted FP32 table (the oracle's MIDTABLE sampler on
+    the same values): heights, sweep counts and evaluation counts identical
+    to the oracle."""
     n, ny = 16384, 1024
     n64 = q.FilteredNoise(n, ny, float(n), float(ny), 1.0, 4.0, 4.0, 1, dtype=q.F64)
     coef = n64.coefficients()
@@ -123,3 +136,16 @@ def test_full_size_rk_matches_oracle(q):
                                                          r_ref.sweeps), f
             assert r.blocked and r.n_velocity_evals == r_ref.n_velocity_evals
             np.testing.assert_array_equal(host(hd), h_ref)
+    # the midpoint-contracted table through the batch entry point, against
+    # the oracle's MIDTABLE sampler on the same values
+    nmid = n64.convert(q.F32_MID)
+    on = O.OracleNoise.mid_table(nmid.coefficients(), float(n), float(ny))
+    h_ref = np.full(n, ny / 2.0)
+    hd = dev(h_ref)
+    for f in (0.0, 0.25):
+        r_ref = O.rk_block(h_ref, O.params(float(n), 1.0, f, O.LINEAR), on, rpo, omp=True)
+        (r,) = q.Batch([n], [nmid], [q.Params(float(n), 1.0, f, q.Model.Linear)]).rk_block(hd, rp)
+        assert (r.blocked, r.runaway, r.sweeps) == (bool(r_ref.blocked), bool(r_ref.runaway),
+                                                     r_ref.sweeps), f
+        assert r.blocked and r.n_velocity_evals == r_ref.n_velocity_evals
+        np.testing.assert_array_equal(host(hd), h_ref)
