@@ -216,6 +216,7 @@ def bench_lbfgs(log, nlines=512, plain_sample=8):
                            f"{[lengths[k] for k in idx]}), {tp:.1f} s")
     out["speedup_f32_batched_vs_plain_f64"] = out["f32"]["minima_per_s"] / plain
     out["speedup_f64_batched_vs_plain_f64"] = out["f64"]["minima_per_s"] / plain
+    out["speedup_f32_mid_batched_vs_plain_f64"] = out["f32_mid"]["minima_per_s"] / plain
     out["workload"] = (f"{nlines} lines, L from {{1024..16384}} (sum {sum(lengths)}), "
                        f"tables L x {ny}, xi in {{2,4,8}}, m=10, FFT preconditioner")
     log(f"lbfgs: {json.dumps(out)}")

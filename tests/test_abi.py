@@ -57,3 +57,15 @@ def test_errors_without_device_are_reported_not_raised_across_abi():
     assert (f.f_inc, f.f_dec, f.alpha_start, f.f_alpha) == (1.1, 0.5, 0.1, 0.99)
     st = lib.qew_fire_minimize(None, None, None, None, 4, None, None)
     assert st == _lib.QEW_ERR_INVALID_ARGUMENT and b"null context" in lib.qew_last_error()
+
+
+def test_dtype_values_match_the_header():
+    """The Python layer's table/state types carry the header's enum values
+    (QEW_F64, QEW_F32, QEW_BF16, QEW_F32_MID)."""
+    import re
+    import qew_amd._lib as L
+    src = open(os.path.join(os.path.dirname(LIB), "..", "include", "qew_amd.h")).read()
+    m = re.search(r"typedef enum qew_dtype \{([^}]*)\}", src)
+    assert m, "qew_dtype enum not found"
+    vals = dict((k.strip(), int(v)) for k, v in (e.split("=") for e in m.group(1).split(",")))
+    assert vals == {"QEW_F64": L.F64, "QEW_F32": L.F32, "QEW_BF16": L.BF16, "QEW_F32_MID": L.F32_MID}
