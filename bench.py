@@ -132,6 +132,7 @@ def bench_lbfgs(log, nlines=512, plain_sample=8):
     # every line has n = L sites on an L-wide table of spacing 1: its edge
     # midpoints lie at x-fraction 1/2, so midpoint-contracted tables apply
     nmid = [nz.convert(q.F32_MID) for nz in n64]
+    n64mid = [nz.convert(q.F64_MID) for nz in n64]
     torch.cuda.synchronize()
     log(f"lbfgs: {nlines} tables generated in {time.perf_counter() - t0:.1f} s "
         f"(sum L = {sum(lengths)})")
@@ -147,7 +148,8 @@ def bench_lbfgs(log, nlines=512, plain_sample=8):
     # FP32: the line offsets' resolution floors the FP32 gradient near 1e-4, so
     # FP32 stops there and is compared by its energy against the FP64 minimum
     for name, noises, dtype, ftol in (("f64", n64, q.F64, 1e-5), ("f32", n32, q.F32, 1e-4),
-                                      ("f32_mid", nmid, q.F32, 1e-4)):
+                                      ("f32_mid", nmid, q.F32, 1e-4),
+                                      ("f64_mid", n64mid, q.F64, 1e-5)):
         b = q.Batch(lengths, noises, params)
         b.set_precond_shifts(shifts)
         lp = q.LbfgsParams(m=10, ftol=ftol, max_iter=10000, precondition=True)
@@ -175,6 +177,10 @@ def bench_lbfgs(log, nlines=512, plain_sample=8):
     out["f32_vs_f64_energy_rel_err"] = {"max": float(rel_err.max()),
                                         "median": float(np.median(rel_err)),
                                         "lines_within_1e-6": int((rel_err <= 1e-6).sum())}
+    rel_m64 = np.abs(finals["f64_mid"] - finals["f64"]) / np.abs(finals["f64"])
+    out["f64_mid_vs_f64_energy_rel_err"] = {"max": float(rel_m64.max()),
+                                            "median": float(np.median(rel_m64)),
+                                            "lines_within_1e-6": int((rel_m64 <= 1e-6).sum())}
     rel_mid = np.abs(finals["f32_mid"] - finals["f64"]) / np.abs(finals["f64"])
     out["f32_mid_vs_f64_energy_rel_err"] = {"max": float(rel_mid.max()),
                                             "median": float(np.median(rel_mid)),
@@ -217,6 +223,7 @@ def bench_lbfgs(log, nlines=512, plain_sample=8):
     out["speedup_f32_batched_vs_plain_f64"] = out["f32"]["minima_per_s"] / plain
     out["speedup_f64_batched_vs_plain_f64"] = out["f64"]["minima_per_s"] / plain
     out["speedup_f32_mid_batched_vs_plain_f64"] = out["f32_mid"]["minima_per_s"] / plain
+    out["speedup_f64_mid_batched_vs_plain_f64"] = out["f64_mid"]["minima_per_s"] / plain
     out["workload"] = (f"{nlines} lines, L from {{1024..16384}} (sum {sum(lengths)}), "
                        f"tables L x {ny}, xi in {{2,4,8}}, m=10, FFT preconditioner")
     log(f"lbfgs: {json.dumps(out)}")

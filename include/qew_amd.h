@@ -46,9 +46,18 @@ typedef enum qew_status {
  * Such tables are read by qew_batch_eval and qew_batch_langevin with QEW_F32
  * line state, and QEW_F32_MID tables also by qew_batch_lbfgs with QEW_F32
  * and by qew_batch_rk_block (FP64 arithmetic on the FP32 values, as on FP32
- * tables) (QEW_F32_MID only for lines with that geometry); every other entry
- * point rejects them with QEW_ERR_INVALID_ARGUMENT. */
-typedef enum qew_dtype { QEW_F64 = 0, QEW_F32 = 1, QEW_BF16 = 2, QEW_F32_MID = 3 } qew_dtype;
+ * tables) (QEW_F32_MID only for lines with that geometry). QEW_F64_MID is the
+ * same contraction kept in FP64, read only by qew_batch_lbfgs with QEW_F64
+ * line state (the reference's arithmetic up to the order of the x and y
+ * sums). Every other entry point rejects these tables with
+ * QEW_ERR_INVALID_ARGUMENT. */
+typedef enum qew_dtype {
+    QEW_F64 = 0,
+    QEW_F32 = 1,
+    QEW_BF16 = 2,
+    QEW_F32_MID = 3,
+    QEW_F64_MID = 4
+} qew_dtype;
 
 /* qew::Model (src/qew_model.h:18). */
 typedef enum qew_model { QEW_LINEAR = 0, QEW_ARCLENGTH = 1 } qew_model;

@@ -28,7 +28,7 @@ from enum import IntEnum
 import numpy as np
 
 from . import _lib
-from ._lib import (ARCLENGTH, BF16, F32, F32_MID, F64, LINEAR, QewError, QewInvalidArgument, check,
+from ._lib import (ARCLENGTH, BF16, F32, F32_MID, F64, F64_MID, LINEAR, QewError, QewInvalidArgument, check,
                    load)
 
 __all__ = [
@@ -37,7 +37,7 @@ __all__ = [
     "Context", "context", "FilteredNoise", "AnalyticNoise", "ZeroNoise", "ConstNoise",
     "HarmonicWell", "SinusoidNoise", "objective", "objective_trial", "gradient",
     "evaluate", "lbfgs_minimize", "LangevinDynamics", "rk_block", "rk_site_velocity",
-    "Batch", "F32", "F32_MID", "F64", "BF16", "QewError", "QewInvalidArgument",
+    "Batch", "F32", "F32_MID", "F64", "F64_MID", "BF16", "QewError", "QewInvalidArgument",
 ]
 
 

@@ -14,6 +14,8 @@ F64, F32, BF16 = 0, 1, 2  # BF16: table storage only (read with F32 line state)
 # table storage only: FP32 values contracted with the B-spline x-weights at
 # fraction 1/2 (read with F32 line state, lines with spacing = table dx, even n)
 F32_MID = 3
+# the same contraction kept in FP64 (read by the batched L-BFGS with F64 state)
+F64_MID = 4
 LINEAR, ARCLENGTH = 0, 1
 TABLE, SINCOS, CONST, ZERO, HARMONIC, WASHBOARD = range(6)
 

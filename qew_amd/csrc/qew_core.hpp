@@ -749,13 +749,15 @@ __device__ __forceinline__ void mid_locate_y64(const NoiseDesc &nd, double y, in
     iyn[2] = (iy + 1 == ny) ? 0 : iy + 1;
     iyn[3] = (iyn[2] + 1 == ny) ? 0 : iyn[2] + 1;
 }
+template <typename V>
 __device__ __forceinline__ void mid_column4(const NoiseDesc &nd, int ix, const int iyn[4],
-                                            float c[4]) {
-    const float *t = static_cast<const float *>(nd.coef);
+                                            V c[4]) {
+    const V *t = static_cast<const V *>(nd.coef);
 #pragma unroll
-    for (int b = 0; b < 4; ++b) c[b] = *(const QEW_GLOBAL float *)(t + (size_t)iyn[b] * nd.pitch + ix);
+    for (int b = 0; b < 4; ++b) c[b] = *(const QEW_GLOBAL V *)(t + (size_t)iyn[b] * nd.pitch + ix);
 }
-__device__ __forceinline__ double mid_v_from(const float c[4], double fy) {
+template <typename V>
+__device__ __forceinline__ double mid_v_from(const V c[4], double fy) {
     double wy[4];
     bspline_w(fy, wy);
     double v = 0.0;
@@ -763,13 +765,29 @@ __device__ __forceinline__ double mid_v_from(const float c[4], double fy) {
     for (int b = 0; b < 4; ++b) v += (double)c[b] * wy[b];
     return v;
 }
+template <typename V = float>
 __device__ __forceinline__ double mid_v64(const NoiseDesc &nd, int ix, double y) {
     int iyn[4];
     double fy;
     mid_locate_y64(nd, y, iyn, fy);
-    float c[4];
-    mid_column4(nd, ix, iyn, c);
-    return mid_v_from(c, fy);
+    V c[4];
+    mid_column4<V>(nd, ix, iyn, c);
+    return mid_v_from<V>(c, fy);
+}
+
+// dV/dy at height y of column ix, in FP64 (stored values of type V).
+template <typename V>
+__device__ __forceinline__ double mid_dy64(const NoiseDesc &nd, int ix, double y) {
+    int iyn[4];
+    double fy, dwy[4];
+    mid_locate_y64(nd, y, iyn, fy);
+    bspline_dw(fy, dwy);
+    V c[4];
+    mid_column4<V>(nd, ix, iyn, c);
+    double d = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) d += (double)c[b] * dwy[b];
+    return (nd.dy == 1.0) ? d : d / nd.dy;
 }
 
 // The 4 x 4 window path of an FP32 or BF16 table.

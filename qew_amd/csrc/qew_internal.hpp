@@ -174,9 +174,10 @@ inline void use_device(qew_ctx ctx) { QEW_HIP(hipSetDevice(ctx->device)); }
 // batch evaluation and dynamics; every entry point that reads tables as
 // per-cell FP64 or FP32 coefficients rejects them before launching anything.
 inline void require_wide_table(const qew_dev::NoiseDesc &nd) {
-    QEW_REQUIRE(nd.kind != qew_dev::TABLE || (nd.dtype != QEW_BF16 && nd.dtype != QEW_F32_MID),
-                "BF16 and QEW_F32_MID tables are read only by qew_batch_eval and "
-                "qew_batch_langevin with QEW_F32 line state");
+    QEW_REQUIRE(nd.kind != qew_dev::TABLE ||
+                    (nd.dtype != QEW_BF16 && nd.dtype != QEW_F32_MID && nd.dtype != QEW_F64_MID),
+                "BF16 and midpoint-contracted tables are read only by the batch entry points "
+                "that document them");
 }
 inline void require_wide_tables(const std::vector<qew_dev::NoiseDesc> &v) {
     for (const auto &nd : v) require_wide_table(nd);

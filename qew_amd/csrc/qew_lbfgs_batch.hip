@@ -229,7 +229,10 @@ __device__ double wg_objective(const LineGeom &G, const T *x, const T *d, double
                        (G.model == 0) ? G.lt * dh * dh * 0.5 : G.lt * (sqrt(1.0 + dh * dh) - 1.0);
                    const double xc = (double)(i + ip) * 0.5 * G.dx;
                    const double hc = (hip + hi) * 0.5;
-                   acc += G.dx * (le + sample_v64<C>(G.nd, xc, hc) - G.f * hi);
+                   double vv;
+                   if constexpr (MID) vv = mid_v64<double>(G.nd, mid_column(G.nd, i, ip), hc);
+                   else vv = sample_v64<C>(G.nd, xc, hc);
+                   acc += G.dx * (le + vv - G.f * hi);
                });
     } else {
         // The sufficient-decrease test compares energies whose difference
@@ -281,31 +284,39 @@ constexpr int kUT = sizeof(T) == 8 ? 1 : 2;
 // against 5480 with four, whose spills grew from 224 to 400 B per lane).
 constexpr int kUTM = 2;
 
-__device__ __forceinline__ void objective_trials_mid(const LineGeom &G, const float *x,
-                                                     const float *d, const double *alpha,
-                                                     double (&acc)[kLS]) {
+template <typename T>
+__device__ __forceinline__ void objective_trials_mid(const LineGeom &G, const T *x, const T *d,
+                                                     const double *alpha, double (&acc)[kLS]) {
     const int n = G.n;
     constexpr int K = kLS;
+    // FP32 state: FP32 offsets from the anchor on an FP32 midpoint table;
+    // FP64 state: heights (anchor 0) on an FP64 midpoint table
     struct Item {
-        P4<float> v;
+        P4<T> v;
         int iy;
-        float c[4];
+        T c[4];
     };
-    auto offsets = [&](const P4<float> &v, int j, double &oi, double &oip) {
-        const float al = (float)alpha[j];
-        oi = (double)__builtin_fmaf(al, v.b, v.a);
-        oip = (double)__builtin_fmaf(al, v.d, v.c);
+    auto offsets = [&](const P4<T> &v, int j, double &oi, double &oip) {
+        if constexpr (sizeof(T) == 8) {
+            oi = v.a + alpha[j] * v.b;
+            oip = v.c + alpha[j] * v.d;
+        } else {
+            const float al = (float)alpha[j];
+            oi = (double)__builtin_fmaf(al, v.b, v.a);
+            oip = (double)__builtin_fmaf(al, v.d, v.c);
+        }
     };
     stream<kUTM>(
         n,
         [&](int i) {
             const int ip = (i + 1 == n) ? 0 : i + 1;
             Item it;
-            it.v = P4<float>{x[i], d[i], x[ip], d[ip]};
+            it.v = P4<T>{x[i], d[i], x[ip], d[ip]};
             double oi, oip, fy;
             offsets(it.v, 0, oi, oip);
             int iyn[4];
-            mid_locate_y64(G.nd, ((G.anchor + oip) + (G.anchor + oi)) * 0.5, iyn, fy);
+            const double a0 = (sizeof(T) == 8) ? 0.0 : G.anchor;
+            mid_locate_y64(G.nd, ((a0 + oip) + (a0 + oi)) * 0.5, iyn, fy);
             it.iy = iyn[1];
             mid_column4(G.nd, mid_column(G.nd, i, ip), iyn, it.c);
             return it;
@@ -317,7 +328,8 @@ __device__ __forceinline__ void objective_trials_mid(const LineGeom &G, const fl
             for (int j = 0; j < K; ++j) {
                 double oi, oip;
                 offsets(it.v, j, oi, oip);
-                const double hi = G.anchor + oi, hip = G.anchor + oip;
+                const double a0 = (sizeof(T) == 8) ? 0.0 : G.anchor;
+                const double hi = a0 + oi, hip = a0 + oip;
                 int iyn[4];
                 double fy;
                 mid_locate_y64(G.nd, (hip + hi) * 0.5, iyn, fy);
@@ -325,14 +337,17 @@ __device__ __forceinline__ void objective_trials_mid(const LineGeom &G, const fl
                 if (iyn[1] == it.iy) {
                     vv = mid_v_from(it.c, fy);
                 } else {
-                    float c[4];
+                    T c[4];
                     mid_column4(G.nd, ix, iyn, c);
                     vv = mid_v_from(c, fy);
                 }
                 const double dh = div1(oip - oi, G.dx);
                 const double s2 = dh * dh;
-                const double le =
-                    (G.model == 0) ? G.lt * s2 * 0.5 : G.lt * (s2 / (sqrt(1.0 + s2) + 1.0));
+                double le;
+                if constexpr (sizeof(T) == 8)  // src/qew_model.h:53-99
+                    le = (G.model == 0) ? G.lt * dh * dh * 0.5 : G.lt * (sqrt(1.0 + dh * dh) - 1.0);
+                else
+                    le = (G.model == 0) ? G.lt * s2 * 0.5 : G.lt * (s2 / (sqrt(1.0 + s2) + 1.0));
                 acc[j] += G.dx * (le + vv - G.f * hi);
             }
         });
@@ -344,7 +359,7 @@ __device__ __forceinline__ void objective_trials_table(const LineGeom &G, const 
     const int n = G.n;
     constexpr int K = kLS;
     if constexpr (MID) {
-        objective_trials_mid(G, x, d, alpha, acc);
+        objective_trials_mid<T>(G, x, d, alpha, acc);
         return;
     }
     using B = typename std::conditional<sizeof(T) == 8, C, float>::type;  // block storage
@@ -499,7 +514,10 @@ __device__ __forceinline__ void wg_gradient(const LineGeom &G, const T *x, T *g,
                const int ip = (i + 1 == n) ? 0 : i + 1;
                if constexpr (sizeof(T) == 8) {
                    const double dh = div1(v.b - v.a, G.dx);
-                   edy[i] = sample_dy64<C>(G.nd, (double)(i + ip) * 0.5 * G.dx, (v.b + v.a) * 0.5);
+                   if constexpr (MID)
+                       edy[i] = mid_dy64<double>(G.nd, mid_column(G.nd, i, ip), (v.b + v.a) * 0.5);
+                   else
+                       edy[i] = sample_dy64<C>(G.nd, (double)(i + ip) * 0.5 * G.dx, (v.b + v.a) * 0.5);
                    if (arc) esl[i] = dh / sqrt(1.0 + dh * dh);
                } else {
                    float vv, dy;
@@ -679,9 +697,9 @@ template <typename T, typename C, bool MID = false>
 __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, double *rho, double *la,
                            int *slot, int *sbuf, C2<T> *fftbuf) {
     // elements per thread with loads in flight in the vector passes: 16 for
-    // the midpoint-table kernel (its line search frees the registers), 8
+    // the FP32 midpoint-table kernel (its line search frees the registers), 8
     // elsewhere (wider passes spill); the per-thread order is the same
-    constexpr int UV = MID ? 16 : kU;
+    constexpr int UV = (MID && sizeof(T) == 4) ? 16 : kU;
     const qew_lbfgs_params &lp = a.lp;
     const LineDesc L = a.lines[line];
     const NoiseDesc nd = a.noises[L.noise];
@@ -1117,6 +1135,16 @@ extern "C" qew_status qew_batch_lbfgs(qew_batch b, int dtype, const qew_lbfgs_pa
                 QEW_REQUIRE(nd.kind == TABLE && (nd.dtype == QEW_F32 || nd.dtype == QEW_F32_MID),
                             "QEW_F32 minimisation needs FP32 or QEW_F32_MID coefficient tables");
             require_f32_tables(b->lines, b->noises, "QEW_F32 minimisation");
+        } else if (b->noises[0].kind == TABLE && b->noises[0].dtype == QEW_F64_MID) {
+            // FP64 midpoint-contracted tables on lines of their geometry
+            for (const auto &l : b->lines) {
+                const auto &nd = b->noises[l.noise];
+                QEW_REQUIRE(nd.kind == TABLE && nd.dtype == QEW_F64_MID,
+                            "all tables of a batch must share one storage type");
+                QEW_REQUIRE(l.n % 2 == 0 && l.L / (double)l.n == nd.dx,
+                            "a QEW_F64_MID table needs lines of even length whose site spacing "
+                            "L / n equals the table's x spacing");
+            }
         } else {
             require_wide_tables(b->noises);
         }
@@ -1186,7 +1214,8 @@ extern "C" qew_status qew_batch_lbfgs(qew_batch b, int dtype, const qew_lbfgs_pa
             hipLaunchKernelGGL(kernel, dim3(grid), dim3(kLB), lds, ctx->stream, a);
             QEW_HIP(hipGetLastError());
         };
-        if (dtype == QEW_F32 && tdt == QEW_F32_MID) launch(lbfgs_batch_kernel<float, float, true>);
+        if (dtype == QEW_F64 && tdt == QEW_F64_MID) launch(lbfgs_batch_kernel<double, double, true>);
+        else if (dtype == QEW_F32 && tdt == QEW_F32_MID) launch(lbfgs_batch_kernel<float, float, true>);
         else if (dtype == QEW_F32) launch(lbfgs_batch_kernel<float, float>);
         else if (tdt == 1) launch(lbfgs_batch_kernel<double, float>);
         else launch(lbfgs_batch_kernel<double, double>);

@@ -248,14 +248,14 @@ __global__ void bf16_table(const S *src, int s_pitch, unsigned short *dst, int d
 // rounded to FP32 once; the padding elements are 0. The source rows have the
 // standard halo (element j = coef(j - 1 mod nx)), so the four neighbours of
 // ix are source elements ix .. ix + 3.
-template <typename S>
-__global__ void mid_table(const S *src, int s_pitch, float *dst, int d_pitch, int nx, int ny) {
+template <typename S, typename D>
+__global__ void mid_table(const S *src, int s_pitch, D *dst, int d_pitch, int nx, int ny) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (int64_t)d_pitch * ny) return;
     const int ix = (int)(k % d_pitch);
     const int iy = (int)(k / d_pitch);
     if (ix >= nx) {
-        dst[k] = 0.0f;
+        dst[k] = (D)0;
         return;
     }
     double w[4];
@@ -264,10 +264,11 @@ __global__ void mid_table(const S *src, int s_pitch, float *dst, int d_pitch, in
     double acc = 0.0;
 #pragma unroll
     for (int a = 0; a < 4; ++a) acc += w[a] * (double)r[a];
-    dst[k] = (float)acc;
+    dst[k] = (D)acc;
 }
 
-__global__ void mid_table_to_rowmajor(const float *table, double *dst, int nx, int ny, int pitch) {
+template <typename D>
+__global__ void mid_table_to_rowmajor(const D *table, double *dst, int nx, int ny, int pitch) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (int64_t)nx * ny) return;
     const int i = (int)(k / ny), j = (int)(k % ny);
@@ -616,22 +617,24 @@ qew_status qew_noise_convert(qew_noise noise, int dtype, qew_noise *out) {
     return guard([&] {
         QEW_REQUIRE(noise && out, "null pointer");
         QEW_REQUIRE(dtype == QEW_F64 || dtype == QEW_F32 || dtype == QEW_BF16 ||
-                        dtype == QEW_F32_MID,
-                    "dtype must be QEW_F64, QEW_F32, QEW_BF16 or QEW_F32_MID");
-        QEW_REQUIRE(noise->desc.kind != TABLE ||
-                        (noise->desc.dtype != QEW_BF16 && noise->desc.dtype != QEW_F32_MID),
-                    "BF16 and QEW_F32_MID tables are not converted further: convert the FP64 or "
-                    "FP32 table");
+                        dtype == QEW_F32_MID || dtype == QEW_F64_MID,
+                    "dtype must be QEW_F64, QEW_F32, QEW_BF16, QEW_F32_MID or QEW_F64_MID");
+        QEW_REQUIRE(noise->desc.kind != TABLE || noise->desc.dtype == QEW_F64 ||
+                        noise->desc.dtype == QEW_F32,
+                    "BF16 and midpoint-contracted tables are not converted further: convert the "
+                    "FP64 or FP32 table");
         qew_ctx ctx = noise->ctx;
         use_device(ctx);
         qew_noise nz = new_noise(ctx, noise->desc.kind, dtype, noise->desc.nx, noise->desc.ny,
                                  noise->desc.dx, noise->desc.dy);
         for (int k = 0; k < 4; ++k) nz->desc.p[k] = noise->desc.p[k];
         nz->field = noise->field;
-        if (noise->desc.kind == TABLE && dtype == QEW_F32_MID) {
+        if (noise->desc.kind == TABLE && (dtype == QEW_F32_MID || dtype == QEW_F64_MID)) {
             const int nx = noise->desc.nx, ny = noise->desc.ny;
             const int pitch = ((nx + 31) / 32) * 32;
-            hipError_t e = hipMalloc(&nz->table, sizeof(float) * (size_t)pitch * ny);
+            const bool wide = dtype == QEW_F64_MID;
+            hipError_t e = hipMalloc(&nz->table, (wide ? sizeof(double) : sizeof(float)) *
+                                                     (size_t)pitch * ny);
             if (e != hipSuccess) {
                 delete nz;
                 QEW_HIP(e);
@@ -639,12 +642,20 @@ qew_status qew_noise_convert(qew_noise noise, int dtype, qew_noise *out) {
             nz->desc.pitch = pitch;
             nz->desc.coef = nz->table;
             const dim3 g = grid1((int64_t)pitch * ny);
-            if (noise->desc.dtype == 1)
-                hipLaunchKernelGGL(mid_table<float>, g, dim3(256), 0, ctx->stream,
+            if (noise->desc.dtype == 1 && wide)
+                hipLaunchKernelGGL((mid_table<float, double>), g, dim3(256), 0, ctx->stream,
+                                   static_cast<const float *>(noise->table), noise->desc.pitch,
+                                   static_cast<double *>(nz->table), pitch, nx, ny);
+            else if (noise->desc.dtype == 1)
+                hipLaunchKernelGGL((mid_table<float, float>), g, dim3(256), 0, ctx->stream,
                                    static_cast<const float *>(noise->table), noise->desc.pitch,
                                    static_cast<float *>(nz->table), pitch, nx, ny);
+            else if (wide)
+                hipLaunchKernelGGL((mid_table<double, double>), g, dim3(256), 0, ctx->stream,
+                                   static_cast<const double *>(noise->table), noise->desc.pitch,
+                                   static_cast<double *>(nz->table), pitch, nx, ny);
             else
-                hipLaunchKernelGGL(mid_table<double>, g, dim3(256), 0, ctx->stream,
+                hipLaunchKernelGGL((mid_table<double, float>), g, dim3(256), 0, ctx->stream,
                                    static_cast<const double *>(noise->table), noise->desc.pitch,
                                    static_cast<float *>(nz->table), pitch, nx, ny);
             e = hipGetLastError();
@@ -770,9 +781,13 @@ qew_status qew_noise_coefficients(qew_noise noise, double *coef_host) {
         QEW_HIP(hipMalloc(&d, sizeof(double) * (size_t)nx * ny));
         const dim3 g((nx + 31) / 32, (ny + 31) / 32), b(32, 8);
         if (noise->desc.dtype == QEW_F32_MID)
-            hipLaunchKernelGGL(mid_table_to_rowmajor, grid1((int64_t)nx * ny), dim3(256), 0,
+            hipLaunchKernelGGL(mid_table_to_rowmajor<float>, grid1((int64_t)nx * ny), dim3(256), 0,
                                ctx->stream, static_cast<const float *>(noise->table), d, nx, ny,
                                noise->desc.pitch);
+        else if (noise->desc.dtype == QEW_F64_MID)
+            hipLaunchKernelGGL(mid_table_to_rowmajor<double>, grid1((int64_t)nx * ny), dim3(256),
+                               0, ctx->stream, static_cast<const double *>(noise->table), d, nx,
+                               ny, noise->desc.pitch);
         else if (noise->desc.dtype == 2)
             hipLaunchKernelGGL(bf16_table_to_rowmajor, grid1((int64_t)nx * ny), dim3(256), 0,
                                ctx->stream, static_cast<const unsigned short *>(noise->table), d,

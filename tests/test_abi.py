@@ -61,11 +61,12 @@ def test_errors_without_device_are_reported_not_raised_across_abi():
 
 def test_dtype_values_match_the_header():
     """The Python layer's table/state types carry the header's enum values
-    (QEW_F64, QEW_F32, QEW_BF16, QEW_F32_MID)."""
+    (QEW_F64, QEW_F32, QEW_BF16, QEW_F32_MID, QEW_F64_MID)."""
     import re
     import qew_amd._lib as L
     src = open(os.path.join(os.path.dirname(LIB), "..", "include", "qew_amd.h")).read()
     m = re.search(r"typedef enum qew_dtype \{([^}]*)\}", src)
     assert m, "qew_dtype enum not found"
     vals = dict((k.strip(), int(v)) for k, v in (e.split("=") for e in m.group(1).split(",")))
-    assert vals == {"QEW_F64": L.F64, "QEW_F32": L.F32, "QEW_BF16": L.BF16, "QEW_F32_MID": L.F32_MID}
+    assert vals == {"QEW_F64": L.F64, "QEW_F32": L.F32, "QEW_BF16": L.BF16, "QEW_F32_MID": L.F32_MID,
+                    "QEW_F64_MID": L.F64_MID}

@@ -170,6 +170,48 @@ def test_mid_batch_lbfgs_energy_error(q):
         b.lbfgs(dev(np.concatenate(h0)), q.LbfgsParams(m=10, precondition=True), dtype=q.F64)
 
 
+@pytest.mark.parametrize("precondition", [True, False])
+def test_f64_mid_batch_lbfgs_matches_oracle(q, precondition):
+    """FP64 line state on FP64 midpoint tables against the oracle's L-BFGS on
+    its MIDTABLE sampler with the same values: minima within 1e-11 relative
+    energy (the FP64 batch's bar against the oracle: tree against sequential
+    sums), all converged."""
+    LT = 4.0 ** 1.5
+    lengths = [256, 64, 512, 128]
+    xis = [2.0, 4.0, 8.0, 4.0]
+    models = [0, 1, 0, 1]
+    ny = 128
+    noises, oracle, params, h0 = [], [], [], []
+    for n, xi, model, seed in zip(lengths, xis, models, [1, 2, 3, 4]):
+        field, coef = O.filtered_noise(n, ny, float(n), float(ny), 1.0, xi, xi, seed)
+        m64 = q.FilteredNoise.from_coefficients(coef, 1.0, 1.0).convert(q.F64_MID)
+        assert m64.info()["dtype"] == q.F64_MID
+        np.testing.assert_allclose(m64.coefficients(), mid_fold(coef), rtol=0,
+                                   atol=1e-14 * np.max(np.abs(coef)))
+        oracle.append((O.OracleNoise.mid_table(m64.coefficients(), float(n), float(ny)),
+                       O.params(float(n), LT, 0.0, model)))
+        noises.append(m64)
+        params.append(q.Params(float(n), LT, 0.0, q.Model(model)))
+        h0.append(np.full(n, ny / 2.0))
+    lp_kw = dict(m=10, ftol=1e-6, max_iter=4000, precondition=precondition)
+    b = q.Batch(lengths, noises, params)
+    h = dev(np.concatenate(h0))
+    res = b.lbfgs(h, q.LbfgsParams(**lp_kw, precond_shift=0.5 / 16.0), dtype=q.F64)
+    hh = host(h)
+    for k, n in enumerate(lengths):
+        on, po = oracle[k]
+        h_ref, r_ref = O.lbfgs(h0[k], po, on, O.lbfgs_params(**lp_kw, precond_shift=0.5 / 16.0))
+        assert res[k].converged and r_ref.converged, (k, res[k], r_ref.iterations)
+        line = hh[b.offsets[k]:b.offsets[k + 1]]
+        e, e_ref = O.objective(line, po, on), O.objective(h_ref, po, on)
+        assert rel(e, e_ref) < 1e-11, (k, e, e_ref)
+    # read by no other entry point
+    with pytest.raises(q.QewInvalidArgument):
+        b.eval(dev(np.concatenate(h0)))
+    with pytest.raises(q.QewInvalidArgument):
+        b.lbfgs(dev(np.concatenate(h0)), q.LbfgsParams(m=4), dtype=q.F32)
+
+
 @pytest.mark.parametrize("path", ["line", "launch"])
 def test_mid_batch_rk_bitwise_vs_oracle(q, monkeypatch, path):
     """Rosso-Krauth sweeps on a midpoint table (FP64 arithmetic on the FP32
