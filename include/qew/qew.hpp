@@ -420,12 +420,15 @@ struct RkResult {
     int sweeps = 0;
     real_t total_advance = 0;
     real_t mean_h = 0;
-    long n_velocity_evals = 0;  // site-velocity evaluations (no reference field)
+    // site-velocity evaluations the reference algorithm makes (no reference field)
+    long n_velocity_evals = 0;
+    // evaluations performed on the device (unchanged sites skipped)
+    long n_velocity_evals_done = 0;
 };
 
 inline RkResult from_c(const qew_rk_result &r) {
     return RkResult{r.blocked != 0, r.runaway != 0, r.sweeps, r.total_advance, r.mean_h,
-                    r.n_velocity_evals};
+                    r.n_velocity_evals, r.n_velocity_evals_done};
 }
 
 template <class Noise>

@@ -155,7 +155,13 @@ typedef struct qew_rk_result {
     int sweeps;
     double total_advance;
     double mean_h;
-    long n_velocity_evals; /* rk_site_velocity calls made (no reference field) */
+    /* rk_site_velocity calls the reference algorithm makes (no reference
+     * field; sites the device skips because their height and both neighbours
+     * are unchanged since they were last found blocked are counted) */
+    long n_velocity_evals;
+    /* site-velocity evaluations the device performed (skipped sites not
+     * counted; every lane of a cooperative march or bisection counted) */
+    long n_velocity_evals_done;
 } qew_rk_result;
 
 /* ---- errors, context, memory ------------------------------------------- */

@@ -148,12 +148,15 @@ class RkResult:
     sweeps: int = 0
     total_advance: float = 0.0
     mean_h: float = 0.0
-    n_velocity_evals: int = 0  # site-velocity evaluations (no reference field)
+    # site-velocity evaluations the reference algorithm makes (no reference field)
+    n_velocity_evals: int = 0
+    # evaluations performed on the device (unchanged sites skipped)
+    n_velocity_evals_done: int = 0
 
     @classmethod
     def from_c(cls, r):
         return cls(bool(r.blocked), bool(r.runaway), r.sweeps, r.total_advance, r.mean_h,
-                   r.n_velocity_evals)
+                   r.n_velocity_evals, r.n_velocity_evals_done)
 
 
 def _torch():

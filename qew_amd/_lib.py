@@ -57,7 +57,7 @@ class RkParams(C.Structure):
 class RkResult(C.Structure):
     _fields_ = [("blocked", C.c_int), ("runaway", C.c_int), ("sweeps", C.c_int),
                 ("total_advance", C.c_double), ("mean_h", C.c_double),
-                ("n_velocity_evals", C.c_long)]
+                ("n_velocity_evals", C.c_long), ("n_velocity_evals_done", C.c_long)]
 
 
 VP = C.c_void_p

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <exception>
 #include <string>
+#include <vector>
 
 #include "qew/cli.hpp"
 #include "qew/qew.hpp"
@@ -18,11 +19,18 @@
 using namespace qew;
 
 namespace {
-const char *kUsage =
-    "qew_benchmark -- hot-loop kernel timings (gradient, objective, Langevin step)\n"
-    "on a 262144-site line, then one preconditioned L-BFGS relaxation.\n\n"
-    "  --reps    repetitions per kernel   (default 200)\n"
-    "  --device  HIP device ordinal       (default 0)\n";
+
+const char *kTool = "qew_benchmark";
+const char *kSummary =
+    "hot-loop timings (gradient, objective, Langevin step) on a 262144-site line,\n"
+    "then one preconditioned L-BFGS relaxation";
+
+std::vector<cli::OptionSpec> option_table() {
+    return {
+        {"reps", cli::Kind::Integer, "200", "repetitions per timed kernel", ""},
+        {"device", cli::Kind::Integer, "0", "HIP device ordinal", ""},
+    };
+}
 
 class Timer {
 public:
@@ -35,16 +43,10 @@ private:
     std::chrono::steady_clock::time_point t0_ = std::chrono::steady_clock::now();
 };
 
-int run(int argc, char **argv) {
-    cli::Args args(argc, argv);
-    if (args.help) {
-        std::fputs(kUsage, stdout);
-        return 0;
-    }
-    const int reps = args.integer("reps", 200);
-    const int device = args.integer("device", 0);
-    args.require_all_used();
-    if (reps < 1) throw std::runtime_error("--reps must be at least 1");
+int run(const cli::Options &opt) {
+    const int reps = opt.integer("reps");
+    const int device = opt.integer("device");
+    if (reps < 1) throw std::runtime_error("--reps is at least 1");
     initialize(device);
 
     const int nx = 262144;  // line length (the parallel dimension)
@@ -120,14 +122,17 @@ int run(int argc, char **argv) {
 }  // namespace
 
 int main(int argc, char *argv[]) {
-    int rc = 0;
+    const std::vector<cli::OptionSpec> table = option_table();
+    int status = 0;
     try {
-        rc = run(argc, argv);
+        const cli::Options opt(table, argc, argv);
+        if (opt.help()) std::fputs(opt.usage(kTool, kSummary).c_str(), stdout);
+        else status = run(opt);
     } catch (const std::exception &e) {
-        std::fprintf(stderr, "qew_benchmark: %s\n\n", e.what());
-        std::fputs(kUsage, stderr);
-        rc = 2;
+        std::fprintf(stderr, "%s: %s\n", kTool, e.what());
+        std::fprintf(stderr, "(%s --help lists the options)\n", kTool);
+        status = 2;
     }
     finalize();
-    return rc;
+    return status;
 }

@@ -1,101 +1,117 @@
-// qew_depinning: Rosso-Krauth force ramp toward f_c (the reference's
-// executables/qew_depinning/main.cpp:31-127: same options, RK knob defaults
-// derived from Ly/ny, same summary lines; exit code 2 on a bad option).
-// Extra option: --fc_rel_tol > 0 bisects the bracket the ramp found down to
-// that relative width, warm-starting every probe from the highest blocked
-// configuration (no reference counterpart; 0 keeps the reference's ramp only).
+// qew_depinning: Rosso-Krauth force ramp toward the depinning threshold f_c.
+// Drop-in for the reference's executables/qew_depinning/main.cpp:31-127: the
+// same option names, the same solver defaults derived from Ly / ny, the same
+// summary lines and exit code 2 on a bad option. Added option: --fc_rel_tol
+// > 0 bisects the ramp's bracket to that relative width, each probe started
+// from the highest blocked configuration (0 keeps the ramp alone).
 #include <cstdint>
 #include <cstdio>
 #include <exception>
 #include <string>
+#include <vector>
 
 #include "qew/cli.hpp"
 #include "qew/qew_io.hpp"
 #include "qew/qew_runs.hpp"
 
 using namespace qew;
+using cli::Kind;
 
 namespace {
-const char *kUsage =
-    "qew_depinning -- Rosso-Krauth force ramp toward the threshold f_c.\n"
-    "All options take a value; defaults match qew_depinning.py.\n\n"
-    "  --model           linear|arclength      (default arclength)\n"
-    "  --amplitude       disorder amplitude U0  (default 1)\n"
-    "  --xi              correlation length     (default 0.1)\n"
-    "  --pinning_length  lambda_p/xi (stiff)    (default 5)\n"
-    "  --Lx --Ly         system size            (default 8, 1)\n"
-    "  --nx --ny         resolution             (default 1024, 256)\n"
-    "  --f_step          force ramp increment   (default 1)\n"
-    "  --f_max           force ramp cap         (default 40)\n"
-    "  --fc_rel_tol      bisect f_c to this relative width; 0 = ramp only (default 0)\n"
-    "  --seed            noise RNG seed          (default 1)\n"
-    "  --output          output file            (default depinning_<model>.h5; .kv: text)\n"
-    "  RK solver knobs (defaults derived from Ly/ny):\n"
-    "  --dstep --max_advance --runaway --root_tol --sweep_tol --max_sweeps\n";
 
-int run(int argc, char **argv) {
-    cli::Args args(argc, argv);
-    if (args.help) {
-        std::fputs(kUsage, stdout);
-        return 0;
-    }
-    DepinningConfig cfg;
-    const std::string model_name = args.str("model", "arclength");
-    if (model_name == "linear") cfg.model = Model::Linear;
-    else if (model_name == "arclength") cfg.model = Model::Arclength;
-    else throw std::runtime_error("--model must be 'linear' or 'arclength', got '" + model_name + "'");
-    cfg.amplitude = args.dbl("amplitude", 1.0);
-    cfg.xi = args.dbl("xi", 0.1);
-    cfg.pinning_length_over_xi = args.dbl("pinning_length", 5.0);
-    cfg.Lx = args.dbl("Lx", 8.0);
-    cfg.Ly = args.dbl("Ly", 1.0);
-    cfg.nx = args.integer("nx", 1024);
-    cfg.ny = args.integer("ny", 256);
-    cfg.f_step = args.dbl("f_step", 1.0);
-    cfg.f_max = args.dbl("f_max", 40.0);
-    cfg.fc_rel_tol = args.dbl("fc_rel_tol", 0.0);
-    cfg.rp.dstep = args.dbl("dstep", 0.5 * (cfg.Ly / cfg.ny));
-    cfg.rp.max_advance = args.dbl("max_advance", cfg.Ly);
-    cfg.rp.runaway = args.dbl("runaway", 2.0 * cfg.Ly);
-    cfg.rp.root_tol = args.dbl("root_tol", 1e-10);
-    cfg.rp.sweep_tol = args.dbl("sweep_tol", 1e-8);
-    cfg.rp.max_sweeps = args.integer("max_sweeps", 200000);
-    const std::uint64_t seed = args.u64("seed", 1);
-    const std::string fname = args.str("output", std::string("depinning_") + model_name + ".h5");
-    args.require_all_used();
+const char *kTool = "qew_depinning";
+const char *kSummary = "Rosso-Krauth force ramp toward the depinning threshold f_c";
 
+std::vector<cli::OptionSpec> option_table() {
+    return {
+        {"model", Kind::Word, "arclength", "elastic energy", "linear|arclength"},
+        {"amplitude", Kind::Real, "1", "disorder amplitude", ""},
+        {"xi", Kind::Real, "0.1", "disorder correlation length", ""},
+        {"pinning_length", Kind::Real, "5", "pinning length over xi", ""},
+        {"Lx", Kind::Real, "8", "box length along the line", ""},
+        {"Ly", Kind::Real, "1", "box height (disorder period)", ""},
+        {"nx", Kind::Integer, "1024", "sites of the line = noise grid columns", ""},
+        {"ny", Kind::Integer, "256", "noise grid rows", ""},
+        {"f_step", Kind::Real, "1", "force increment of the ramp", ""},
+        {"f_max", Kind::Real, "40", "largest force of the ramp", ""},
+        {"fc_rel_tol", Kind::Real, "0", "bisect f_c to this relative width; 0: ramp only", ""},
+        {"seed", Kind::Seed, "1", "disorder seed", ""},
+        {"output", Kind::Word, "", "output file (.kv: text archive) [depinning_<model>.h5]", ""},
+        {"dstep", Kind::Real, "", "march step of a site [Ly / (2 ny)]", ""},
+        {"max_advance", Kind::Real, "", "largest advance of a site per sweep [Ly]", ""},
+        {"runaway", Kind::Real, "", "mean advance that counts as depinned [2 Ly]", ""},
+        {"root_tol", Kind::Real, "1e-10", "bisection width of a site's forward zero", ""},
+        {"sweep_tol", Kind::Real, "1e-8", "total advance of a blocked sweep", ""},
+        {"max_sweeps", Kind::Integer, "200000", "sweep cap per force", ""},
+    };
+}
+
+DepinningConfig config_from(const cli::Options &o) {
+    DepinningConfig c;
+    c.model = o.word("model") == "linear" ? Model::Linear : Model::Arclength;
+    c.amplitude = o.real("amplitude");
+    c.xi = o.real("xi");
+    c.pinning_length_over_xi = o.real("pinning_length");
+    c.Lx = o.real("Lx");
+    c.Ly = o.real("Ly");
+    c.nx = o.integer("nx");
+    c.ny = o.integer("ny");
+    c.f_step = o.real("f_step");
+    c.f_max = o.real("f_max");
+    c.fc_rel_tol = o.real("fc_rel_tol");
+    // knobs whose defaults scale with the disorder cell
+    const double cell = c.Ly / c.ny;
+    c.rp.dstep = o.given("dstep") ? o.real("dstep") : 0.5 * cell;
+    c.rp.max_advance = o.given("max_advance") ? o.real("max_advance") : c.Ly;
+    c.rp.runaway = o.given("runaway") ? o.real("runaway") : 2.0 * c.Ly;
+    c.rp.root_tol = o.real("root_tol");
+    c.rp.sweep_tol = o.real("sweep_tol");
+    c.rp.max_sweeps = o.integer("max_sweeps");
+    return c;
+}
+
+const char *verdict(const RkResult &r) {
+    if (r.blocked) return "blocked";
+    return r.runaway ? "DEPINNED" : "timeout";
+}
+
+int run(const cli::Options &opt) {
+    const DepinningConfig cfg = config_from(opt);
+    const std::string model = opt.word("model");
+    const std::string out = opt.given("output") ? opt.word("output") : "depinning_" + model + ".h5";
     initialize();
-    FilteredNoise noise(cfg.nx, cfg.ny, cfg.Lx, cfg.Ly, cfg.amplitude, cfg.xi, cfg.xi, seed);
-    const DepinningResult res = depinning_ramp(cfg, noise.device_noise());
-    std::printf("model=%s  lambda_p/xi=%.3g  line_tension=%.4g\n", model_name.c_str(),
-                static_cast<double>(cfg.pinning_length_over_xi),
-                static_cast<double>(res.line_tension));
-    for (const DepinningStep &st : res.steps)
-        std::printf("f=%7.3f  sweeps=%-7d mean_h=%8.4f  %s\n", static_cast<double>(st.f),
-                    st.result.sweeps, static_cast<double>(st.result.mean_h),
-                    st.result.blocked ? "blocked" : (st.result.runaway ? "DEPINNED" : "timeout"));
+    FilteredNoise disorder(cfg.nx, cfg.ny, cfg.Lx, cfg.Ly, cfg.amplitude, cfg.xi, cfg.xi,
+                           opt.seed("seed"));
+    const DepinningResult res = depinning_ramp(cfg, disorder.device_noise());
+    std::printf("model=%s  lambda_p/xi=%.3g  line_tension=%.4g\n", model.c_str(),
+                (double)cfg.pinning_length_over_xi, (double)res.line_tension);
+    for (const DepinningStep &s : res.steps)
+        std::printf("f=%7.3f  sweeps=%-7d mean_h=%8.4f  %s\n", (double)s.f, s.result.sweeps,
+                    (double)s.result.mean_h, verdict(s.result));
     if (res.f_c_upper_bound >= 0)
-        std::printf("Line depinned near f=%.3f; stopping ramp.\n",
-                    static_cast<double>(res.f_c_upper_bound));
+        std::printf("Line depinned near f=%.3f; stopping ramp.\n", (double)res.f_c_upper_bound);
     if (res.f_c >= 0)
-        std::printf("f_c = %.12g  in [%.12g, %.12g] after %d bisection probes\n",
-                    static_cast<double>(res.f_c), static_cast<double>(res.f_c_lower),
-                    static_cast<double>(res.f_c_upper), res.bisection_probes);
-    write_output(fname, [&](auto &w) { write_depinning(w, cfg, res); });
-    std::printf("wrote %s\n", fname.c_str());
+        std::printf("f_c = %.12g  in [%.12g, %.12g] after %d bisection probes\n", (double)res.f_c,
+                    (double)res.f_c_lower, (double)res.f_c_upper, res.bisection_probes);
+    write_output(out, [&](auto &w) { write_depinning(w, cfg, res); });
+    std::printf("wrote %s\n", out.c_str());
     return 0;
 }
+
 }  // namespace
 
 int main(int argc, char *argv[]) {
-    int rc = 0;
+    const std::vector<cli::OptionSpec> table = option_table();
+    int status = 0;
     try {
-        rc = run(argc, argv);
+        const cli::Options opt(table, argc, argv);
+        if (opt.help()) std::fputs(opt.usage(kTool, kSummary).c_str(), stdout);
+        else status = run(opt);
     } catch (const std::exception &e) {
-        std::fprintf(stderr, "qew_depinning: %s\n\n", e.what());
-        std::fputs(kUsage, stderr);
-        rc = 2;
+        std::fprintf(stderr, "%s: %s\n", kTool, e.what());
+        std::fprintf(stderr, "(%s --help lists the options)\n", kTool);
+        status = 2;
     }
     finalize();
-    return rc;
+    return status;
 }

@@ -43,6 +43,9 @@ struct RkState {
     // site-velocity evaluations the reference algorithm makes in this call
     // (skipped sites included), added to by every wave
     unsigned long long vel_evals;
+    // site-velocity evaluations this call performed (skipped sites excluded,
+    // every lane of a group counted), added to by every wave
+    unsigned long long vel_done;
 };
 
 struct RkArgs {
@@ -219,12 +222,13 @@ __device__ __forceinline__ unsigned long long group_bits(unsigned long long m, i
 // and checks the same loop condition, so the path walked afterwards is the
 // sequential path. Every lane of the group must call this with the same
 // site; all return the same height. `evals` (every lane) grows by the number
-// of evaluations the sequential loop makes, march and bisection.
+// of evaluations the sequential loop makes, march and bisection; `done` (per
+// lane) by the evaluations this lane performs.
 template <typename C>
 __device__ double forward_zero_group(int i, int n, double dx, double lt, double f, int model,
                                      double hi, double hl, double hr, const NoiseDesc &nd,
                                      const qew_rk_params &rp, int g, int li, int base,
-                                     unsigned &evals) {
+                                     unsigned &evals, unsigned &done) {
     // one evaluation site for both phases keeps a single inlined copy of the
     // velocity (two gathers) in the kernel
     double a = hi, advanced = 0;  // march state
@@ -266,7 +270,10 @@ __device__ double forward_zero_group(int i, int n, double dx, double lt, double 
             trial = 0.5 * (l + h);
         }
         double v = 0;
-        if (want) v = site_velocity_held<C>(i, n, dx, lt, f, model, trial, hl, hr, nd, el, er);
+        if (want) {
+            v = site_velocity_held<C>(i, n, dx, lt, f, model, trial, hl, hr, nd, el, er);
+            ++done;
+        }
         if (marching) {
             const unsigned long long hm = group_bits(__ballot(want && v <= 0), base, g);
             const unsigned long long rm = group_bits(__ballot(want), base, g);
@@ -381,6 +388,7 @@ __global__ __launch_bounds__(kRkBlock, 3) void rk_color_kernel(RkArgs a, int col
     if (tid == 0) s_count = 0;
     __syncthreads();
     unsigned evals = valid ? 1u : 0u;
+    unsigned done = dirty ? 1u : 0u;
     double hi = 0;
     if (dirty) {
         hi = h[i];
@@ -400,7 +408,7 @@ __global__ __launch_bounds__(kRkBlock, 3) void rk_color_kernel(RkArgs a, int col
         const int sp = (s + 1 == n) ? 0 : s + 1;
         unsigned e = 0;
         const double hnew = forward_zero_group<C>(s, n, dx, L.lt, L.f, L.model, h[s], h[sm], h[sp],
-                                                  nd, a.rp, g, li, base, e);
+                                                  nd, a.rp, g, li, base, e, done);
         if (li == 0) {
             evals += e;
             s_new[t] = hnew;
@@ -420,6 +428,8 @@ __global__ __launch_bounds__(kRkBlock, 3) void rk_color_kernel(RkArgs a, int col
     // every lane of the wave reaches the reduction (no early return above)
     const unsigned ev = wave_sum(evals);
     if ((tid & (kWave - 1)) == 0 && ev) atomicAdd(&a.state[line].vel_evals, (unsigned long long)ev);
+    const unsigned dn = wave_sum(done);
+    if ((tid & (kWave - 1)) == 0 && dn) atomicAdd(&a.state[line].vel_done, (unsigned long long)dn);
 }
 
 // One workgroup per line: the sweep's advance (per-workgroup sums of both
@@ -501,7 +511,7 @@ __global__ __launch_bounds__(kRkLineBlock) void rk_line_kernel(RkArgs a, int lds
     }
     RkState st{};
     st.mean_h0 = block_sum(acc, red) / (double)n;
-    unsigned long long evals = 0;
+    unsigned long long evals = 0, done = 0;
     bool go = a.rp.max_sweeps > 0;
     while (go) {
         double my_adv = 0;
@@ -525,6 +535,7 @@ __global__ __launch_bounds__(kRkLineBlock) void rk_line_kernel(RkArgs a, int lds
                 __syncthreads();
                 double hi = 0;
                 if (dirty) {
+                    ++done;
                     hi = s_h[i];
                     const double v0 =
                         site_velocity<C>(i, n, dx, L.lt, L.f, L.model, hi, s_h[im], s_h[ip], nd);
@@ -541,10 +552,11 @@ __global__ __launch_bounds__(kRkLineBlock) void rk_line_kernel(RkArgs a, int lds
                     const int s = 2 * (c0 + t) + color;
                     const int sm = (s == 0) ? n - 1 : s - 1;
                     const int sp = (s + 1 == n) ? 0 : s + 1;
-                    unsigned e = 0;
+                    unsigned e = 0, dn = 0;
                     const double hnew = forward_zero_group<C>(s, n, dx, L.lt, L.f, L.model, s_h[s],
                                                               s_h[sm], s_h[sp], nd, a.rp, g, li,
-                                                              base, e);
+                                                              base, e, dn);
+                    done += dn;
                     if (li == 0) {
                         evals += e;
                         s_new[t] = hnew;
@@ -569,6 +581,8 @@ __global__ __launch_bounds__(kRkLineBlock) void rk_line_kernel(RkArgs a, int lds
     for (int i = tid; i < n; i += kRkLineBlock) hg[i] = s_h[i];
     const unsigned long long ev = wave_sum(evals);
     if ((tid & (kWave - 1)) == 0 && ev) atomicAdd(&a.state[line].vel_evals, ev);
+    const unsigned long long dn = wave_sum(done);
+    if ((tid & (kWave - 1)) == 0 && dn) atomicAdd(&a.state[line].vel_done, dn);
     if (tid == 0) {
         // the counter is added to by every wave: leave it out of the store
         a.state[line].active = 0;
@@ -720,6 +734,7 @@ qew_rk_result to_result(const RkState &st) {
     r.total_advance = st.total_adv;
     r.mean_h = st.sweeps > 0 ? st.mean_h : 0.0;
     r.n_velocity_evals = (long)st.vel_evals;
+    r.n_velocity_evals_done = (long)st.vel_done;
     return r;
 }
 

@@ -279,3 +279,51 @@ def test_mid_tables_rejected_elsewhere_and_for_other_geometries(q):
         if L % 2 == 0:  # (odd lengths are rejected by the red-black sweep anyway)
             with pytest.raises(q.QewInvalidArgument):
                 bb.rk_block(dev(np.full(L, ny / 2.0)), rk)
+
+
+def test_f64_mid_tables_rejected_elsewhere_and_for_other_geometries(q):
+    """QEW_F64_MID tables are read only by the batched FP64 L-BFGS, and only
+    for lines of their geometry (even n, site spacing L / n equal to the
+    table's dx): a wrong geometry would otherwise read the contracted column
+    of the wrong cell without any fault, so every other use is rejected."""
+    n, ny = 64, 32
+    field, coef = O.filtered_noise(n, ny, float(n), float(ny), 1.0, 4.0, 4.0, 12)
+    g64 = q.FilteredNoise.from_coefficients(coef, 1.0, 1.0)
+    gm = g64.convert(q.F64_MID)
+    p = q.Params(float(n), 1.0, 0.0, q.Model.Linear)
+    h = dev(np.full(n, ny / 2.0))
+    rk = q.RkParams(0.5, 1.0, 1e-10, 1e-9, 2.0)
+    fp = q.FireParams(dt_max=0.1)
+    for call in (lambda: q.gradient(h, p, gm), lambda: q.objective(h, p, gm),
+                 lambda: q.evaluate(h, p, gm), lambda: q.lbfgs_minimize(h, p, gm),
+                 lambda: q.rk_block(h, p, gm, rk), lambda: q.fire_minimize(h, p, gm, fp),
+                 lambda: q.LangevinDynamics(n, 1).step(h, p, gm, 0.01, 0.0),
+                 lambda: gm.sample(dev(np.array([1.0])), dev(np.array([1.0]))),
+                 lambda: gm.convert(q.F32), lambda: gm.convert(q.F64_MID)):
+        with pytest.raises(q.QewInvalidArgument):
+            call()
+    b = q.Batch([n], [gm], [p])
+    off = dev(np.zeros(n, dtype=np.float32))
+    anchor = dev(np.array([ny / 2.0]))
+    for call in (lambda: b.eval(h), lambda: b.eval(off, dtype=q.F32, anchor=anchor),
+                 lambda: b.langevin(h, [0.01], [0.0], seed=1),
+                 lambda: b.langevin(off, [0.01], [0.0], seed=1, dtype=q.F32, anchor=anchor),
+                 lambda: b.lbfgs(h, q.LbfgsParams(m=4, precondition=False), dtype=q.F32),
+                 lambda: b.rk_block(h, rk), lambda: b.fire(h, fp)):
+        with pytest.raises(q.QewInvalidArgument):
+            call()
+    # the accepted geometry
+    (r,) = b.lbfgs(dev(np.full(n, ny / 2.0)), q.LbfgsParams(m=4, ftol=1e-6), dtype=q.F64)
+    assert r.converged
+    # site spacing != table dx, and an odd length without the preconditioner
+    for L, phys in [(n, 2.0 * n), (n - 1, float(n - 1))]:
+        bb = q.Batch([L], [gm], [q.Params(phys, 1.0, 0.0, q.Model.Linear)])
+        with pytest.raises(q.QewInvalidArgument):
+            bb.lbfgs(dev(np.full(L, ny / 2.0)), q.LbfgsParams(m=4, precondition=False),
+                     dtype=q.F64)
+    # mixed FP64 / FP64_MID batches, in both orders
+    for noises in ([g64, gm], [gm, g64]):
+        bb = q.Batch([n, n], noises, [p, p])
+        with pytest.raises(q.QewInvalidArgument):
+            bb.lbfgs(dev(np.full(2 * n, ny / 2.0)), q.LbfgsParams(m=4, precondition=False),
+                     dtype=q.F64)
