@@ -249,6 +249,21 @@ qew_status qew_gradient(qew_ctx ctx, qew_noise noise, const qew_params *p, int n
 qew_status qew_eval(qew_ctx ctx, qew_noise noise, const qew_params *p, int n,
                     const double *h, double *grad, double *energy);
 
+/* qew_eval without any allocation: no reference counterpart. For callers that
+ * cannot allocate device or pinned host memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Gradient and energy
+ * equal qew_eval's bit for bit (the same kernels); the energy is written to
+ * the DEVICE pointer `energy` (FP64[1], or NULL for the gradient only) and the
+ * call does not synchronise. The per-tile energy partials live in a
+ * file-scope device array of 16384 doubles (128 KiB, held by every device
+ * the library runs on until the process ends; its address is resolved per
+ * device by qew_ctx_create), so n is limited to 16384 * 255 = 4177920 sites;
+ * longer lines are rejected with QEW_ERR_INVALID_ARGUMENT. All calls on one
+ * device share that array: calls that overlap on one device (other streams,
+ * other threads) race and must be serialised by the caller. */
+qew_status qew_eval_noalloc(qew_ctx ctx, qew_noise noise, const qew_params *p, int n,
+                            const double *h, double *grad, double *energy);
+
 /* ---- batches of independent lines --------------------------------------- */
 
 /* A ragged batch: line l has lengths[l] sites, potential noises[l] and model

@@ -91,6 +91,7 @@ SIGNATURES = [
     ("qew_objective_trial", [VP, VP, C.POINTER(Params), I, VP, VP, D, DP]),
     ("qew_gradient", [VP, VP, C.POINTER(Params), I, VP, VP]),
     ("qew_eval", [VP, VP, C.POINTER(Params), I, VP, VP, DP]),
+    ("qew_eval_noalloc", [VP, VP, C.POINTER(Params), I, VP, VP, VP]),
     ("qew_batch_create", [VP, I, C.POINTER(I), C.POINTER(VP), C.POINTER(Params), C.POINTER(VP)]),
     ("qew_batch_destroy", [VP]),
     ("qew_batch_offsets", [VP, C.POINTER(C.c_int64)]),

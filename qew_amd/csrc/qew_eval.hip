@@ -10,7 +10,17 @@
 // multi-XCD part where a device-scope release writes back the L2.
 #include "qew_internal.hpp"
 
+#include <mutex>
+
 using namespace qew_dev;
+
+// Scratch of qew_eval_noalloc: the per-tile energy partials of one line, in a
+// file-scope device array instead of a context buffer, so that the entry point
+// allocates nothing. Every device the library runs on holds its own copy until
+// the process ends. On the host this name is not a device address: host code
+// uses only the per-device address hipGetSymbolAddress returns for it.
+constexpr int kNoAllocMaxTiles = 16384;  // lines of up to 16384 * 255 sites
+static __device__ __align__(256) double scratch[kNoAllocMaxTiles];
 
 namespace {
 
@@ -279,6 +289,29 @@ LineDesc single_line(int n, const qew_params *p) {
     return l;
 }
 
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+void noalloc_scratch_init(int device) {
+    QEW_REQUIRE(device >= 0 && device < kMaxDevices, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void *ptr = nullptr;
+    QEW_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    QEW_REQUIRE(ptr != nullptr, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[device] = ptr;
+}
+
+// The scratch address set up for `device`, or null.
+void *noalloc_scratch_ptr(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
 }  // namespace qew_host
 
 using namespace qew_host;
@@ -354,6 +387,32 @@ qew_status qew_eval(qew_ctx ctx, qew_noise noise, const qew_params *p, int n,
         QEW_REQUIRE(h && grad, "null pointer");
         const double e = single_eval(ctx, noise, p, n, h, nullptr, 0.0, grad, energy != nullptr);
         if (energy) *energy = e;
+    });
+}
+
+qew_status qew_eval_noalloc(qew_ctx ctx, qew_noise noise, const qew_params *p, int n,
+                            const double *h, double *grad, double *energy) {
+    return guard([&] {
+        check_line(ctx, noise, p, n);
+        QEW_REQUIRE(h && grad, "null pointer");
+        const LineDesc l = single_line(n, p);
+        QEW_REQUIRE(l.ntiles <= kNoAllocMaxTiles,
+                    "qew_eval_noalloc: a line of " + std::to_string(n) + " sites needs " +
+                        std::to_string(l.ntiles) + " tiles of scratch; the maximum is " +
+                        std::to_string(kNoAllocMaxTiles) + " tiles (" +
+                        std::to_string(kNoAllocMaxTiles * kTileSites) + " sites)");
+        void *scr = noalloc_scratch_ptr(ctx->device);
+        QEW_REQUIRE(scr != nullptr, "the device's scratch address was not set up by qew_ctx_create");
+        use_device(ctx);
+        // every tile writes its partial before line_energy64 (the next launch
+        // on the same stream) reads it: nothing needs clearing
+        EvalArgs a{};
+        a.ntiles = l.ntiles;
+        a.h = h;
+        a.grad = grad;
+        a.energy = energy;
+        a.partial = scr;
+        launch_eval_f64(ctx->stream, a, &l, &noise->desc);
     });
 }
 

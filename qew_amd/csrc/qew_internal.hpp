@@ -207,6 +207,10 @@ inline void require_f32_tables(const std::vector<qew_dev::LineDesc> &lines,
 // tiles and the results of `nlines` lines.
 void ctx_reserve_eval(qew_ctx ctx, int ntiles, int nlines);
 
+// Resolves, once per device and with that device current, the address of the
+// scratch array qew_eval_noalloc uses (called by qew_ctx_create).
+void noalloc_scratch_init(int device);
+
 // Single-line descriptor helper.
 qew_dev::LineDesc single_line(int n, const qew_params *p);
 

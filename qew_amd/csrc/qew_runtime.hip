@@ -391,6 +391,7 @@ qew_status qew_ctx_create(int device, void *hip_stream, qew_ctx *out) {
         QEW_HIP(hipGetDeviceCount(&count));
         QEW_REQUIRE(device >= 0 && device < count, "no such device");
         QEW_HIP(hipSetDevice(device));
+        noalloc_scratch_init(device);
         qew_ctx c = new qew_ctx_s();
         c->device = device;
         // NULL selects the device's default (null) stream, which orders with
