@@ -41,6 +41,40 @@ FC_EST_LANGEVIN = 0.512
 ROOT = os.path.dirname(os.path.abspath(__file__))
 
 
+_ALL_CPUS = None
+
+
+def omp_setup():
+    """Thread placement of the OpenMP CPU baselines, set before torch or the
+    oracle loads the OpenMP runtime: one thread per core this process may run
+    on (an OMP_NUM_THREADS already set is kept), spread over the cores. An
+    OpenMP runtime that reads these binds the thread that loads it to one
+    core; unpin() gives that thread its whole CPU set back, so that the host
+    threads the library starts (table generation and conversion) are not
+    confined to one core."""
+    global _ALL_CPUS
+    _ALL_CPUS = os.sched_getaffinity(0)
+    os.environ.setdefault("OMP_PROC_BIND", "spread")
+    os.environ.setdefault("OMP_PLACES", "threads")
+    nproc = len(_ALL_CPUS)
+    os.environ.setdefault("OMP_NUM_THREADS", str(nproc))
+    return {"nproc": nproc, "OMP_NUM_THREADS": os.environ["OMP_NUM_THREADS"],
+            "OMP_PROC_BIND": os.environ["OMP_PROC_BIND"], "OMP_PLACES": os.environ["OMP_PLACES"]}
+
+
+def unpin():
+    if _ALL_CPUS:
+        os.sched_setaffinity(0, _ALL_CPUS)
+
+
+def oracle():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    O.load(omp=True)
+    unpin()
+    return O
+
+
 def walk(seed=2):
     rng = np.random.default_rng(seed)
     return NY / 2 + np.cumsum(rng.standard_normal(L))
@@ -88,8 +122,7 @@ def cpu_baseline(coef_rowmajor, h0, budget_s=15.0):
     """The oracle's OpenMP build (restatement of the reference's two-pass
     gradient + separate objective reduce) on this host, on a bounded sample of
     the same evaluations."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as O
+    O = oracle()
     lib = O.load(omp=True)
     nz = O.OracleNoise.table(coef_rowmajor, float(L), float(NY))
     p = O.params(float(L), 1.0, 0.0, O.ARCLENGTH)
@@ -109,7 +142,7 @@ def cpu_baseline(coef_rowmajor, h0, budget_s=15.0):
                       f"OpenMP restatement of the reference two-pass gradient) in {dt:.1f} s"}
 
 
-def bench_lbfgs(log, nlines=512, plain_sample=8):
+def bench_lbfgs(log, nlines=512, plain_sample=8, with_cpu=True, omp_info=None):
     """Batched static minimisation: nlines independent realizations with line
     lengths drawn from {1024, ..., 16384}, each on its own L x 2048 table with
     xi from {2, 4, 8}, linear or arclength elasticity (seeded draw), flat start
@@ -128,10 +161,12 @@ def bench_lbfgs(log, nlines=512, plain_sample=8):
     specs = [(L_, ny, float(L_), float(ny), 1.0, xi, xi, 1 + k)
              for k, (L_, xi) in enumerate(zip(lengths, xis))]
     n64 = q.FilteredNoise.generate_batch(specs, dtype=q.F64)
+    log(f"lbfgs: {nlines} FP64 tables generated in {time.perf_counter() - t0:.1f} s")
     n32 = [nz.convert(q.F32) for nz in n64]
     # every line has n = L sites on an L-wide table of spacing 1: its edge
     # midpoints lie at x-fraction 1/2, so midpoint-contracted tables apply
     nmid = [nz.convert(q.F32_MID) for nz in n64]
+    log(f"lbfgs: FP32 and midpoint tables converted at {time.perf_counter() - t0:.1f} s")
     n64mid = [nz.convert(q.F64_MID) for nz in n64]
     torch.cuda.synchronize()
     log(f"lbfgs: {nlines} tables generated in {time.perf_counter() - t0:.1f} s "
@@ -141,8 +176,9 @@ def bench_lbfgs(log, nlines=512, plain_sample=8):
     shifts = [0.5 / xi ** 2 for xi in xis]
     h0 = torch.full((sum(lengths),), ny / 2.0, dtype=torch.float64, device="cuda:0")
     out = {}
-    finals, gmax, minima = {}, {}, {}
+    finals, gmax, minima, results = {}, {}, {}, {}
     offs = np.concatenate([[0], np.cumsum(lengths)])
+    m_hist = 10
     # ftol 1e-5: above the Armijo roundoff floor of the longest lines (L=16384,
     # |E| ~ 1e4), where the reference algorithm converges in ~100-300 iterations
     # FP32: the line offsets' resolution floors the FP32 gradient near 1e-4, so
@@ -155,9 +191,15 @@ def bench_lbfgs(log, nlines=512, plain_sample=8):
         lp = q.LbfgsParams(m=10, ftol=ftol, max_iter=10000, precondition=True)
         h = h0.clone()
         torch.cuda.synchronize()
+        log(f"lbfgs {name}: launching")
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         t1 = time.perf_counter()
+        ev0.record()
         res = b.lbfgs(h, lp, dtype=dtype)
+        ev1.record()
         dt = time.perf_counter() - t1
+        torch.cuda.synchronize()
+        span = ev0.elapsed_time(ev1) * 1e-3
         # energies and gradients of the minima, evaluated in FP64 on the FP64 tables
         e, g = q.Batch(lengths, n64, params).eval(h)
         finals[name] = e.cpu().numpy()
@@ -165,7 +207,23 @@ def bench_lbfgs(log, nlines=512, plain_sample=8):
         ga = g.abs().cpu().numpy()
         gmax[name] = np.array([ga[o0:o1].max() for o0, o1 in zip(offs[:-1], offs[1:])])
         its = [r.iterations for r in res]
+        results[name] = (res, span)
+        # the critical path: the line with the most site-iterations, solved
+        # alone (one workgroup), against the whole launch
+        crit = int(np.argmax(np.array(its) * np.array(lengths)))
+        b1 = q.Batch([lengths[crit]], [noises[crit]], [params[crit]])
+        b1.set_precond_shifts([shifts[crit]])
+        h1 = torch.full((lengths[crit],), ny / 2.0, dtype=torch.float64, device="cuda:0")
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        (r1,) = b1.lbfgs(h1, lp, dtype=dtype)
+        alone = time.perf_counter() - t2
+        del b1, h1
         out[name] = {"minima_per_s": nlines / dt, "seconds": dt, "ftol": ftol,
+                     "launch_s": span,
+                     "critical_line": {"line": crit, "L": int(lengths[crit]),
+                                       "iterations": int(its[crit]), "alone_s": alone,
+                                       "alone_over_launch": alone / span},
                      "converged": int(sum(r.converged for r in res)),
                      "curvature_skips": int(sum(r.n_curvature_skips for r in res)),
                      "ls_restarts": int(sum(r.n_ls_restarts for r in res)),
@@ -173,6 +231,30 @@ def bench_lbfgs(log, nlines=512, plain_sample=8):
                      "iterations_mean": float(np.mean(its)), "iterations_max": int(max(its))}
         log(f"lbfgs {name}: {json.dumps(out[name])}")
         del b, h
+    # algorithmic bytes of each minimisation: per iteration the vector algebra
+    # of the two-loop recursion and the update, (2m + 8) n s (history read
+    # once); per function evaluation (line-search trials, n_func_evals) and per
+    # gradient (iterations + 1) the distinct coefficients the line's edges
+    # touch, s_c sum_r U_r (taken at the FP64 minimum), plus 2 n s of line
+    # vectors; divided by the launch span
+    u_line = np.array([distinct_coefficients(minima["f64"][o0:o1], o1 - o0, o1 - o0, ny)
+                       for o0, o1 in zip(offs[:-1], offs[1:])], dtype=np.float64)
+    u_mid = np.array([distinct_mid_values(minima["f64"][o0:o1], o1 - o0, o1 - o0, ny)
+                      for o0, o1 in zip(offs[:-1], offs[1:])], dtype=np.float64)
+    n_arr = np.array(lengths, dtype=np.float64)
+    for name, (res, span) in results.items():
+        s_state = 8 if name.startswith("f64") else 4
+        s_c = 8 if name.startswith("f64") else 4
+        u = u_mid if name.endswith("mid") else u_line
+        its = np.array([r.iterations for r in res], dtype=np.float64)
+        nfe = np.array([r.n_func_evals for r in res], dtype=np.float64)
+        per_eval = s_c * u + 2 * n_arr * s_state
+        bytes_total = float((its * (2 * m_hist + 8) * n_arr * s_state
+                             + (nfe + its + 1) * per_eval).sum())
+        ach = bytes_total / span / 1e9
+        out[name]["roofline"] = {"bound": "latency (one workgroup per line)", "achieved": ach,
+                                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": ach / HBM_PEAK_GBS, "bytes": bytes_total}
     rel_err = np.abs(finals["f32"] - finals["f64"]) / np.abs(finals["f64"])
     out["f32_vs_f64_energy_rel_err"] = {"max": float(rel_err.max()),
                                         "median": float(np.median(rel_err)),
@@ -226,11 +308,57 @@ def bench_lbfgs(log, nlines=512, plain_sample=8):
     out["speedup_f64_mid_batched_vs_plain_f64"] = out["f64_mid"]["minima_per_s"] / plain
     out["workload"] = (f"{nlines} lines, L from {{1024..16384}} (sum {sum(lengths)}), "
                        f"tables L x {ny}, xi in {{2,4,8}}, m=10, FFT preconditioner")
+    if with_cpu:
+        out["cpu_baseline"], out["f64_vs_oracle"] = lbfgs_oracle_sample(
+            log, n64, lengths, xis, models, lt, shifts, ny, minima["f64"], offs,
+            results["f64"][0], omp_info)
     log(f"lbfgs: {json.dumps(out)}")
     return out
 
 
-def bench_langevin(log, nsteps=100000):
+def lbfgs_oracle_sample(log, n64, lengths, xis, models, lt, shifts, ny, h64, offs, res64,
+                        omp_info, stride=2):
+    """The oracle's L-BFGS (OpenMP over lines, each line sequential) on every
+    stride-th line of the batch: the CPU baseline in minima/s, and the GPU FP64
+    batch's minima against the oracle's (energies from the sequential oracle
+    objective on the same table)."""
+    O = oracle()
+    idx = list(range(0, len(lengths), stride))
+    t0 = time.perf_counter()
+    tables = [O.OracleNoise.table(n64[k].coefficients(), float(lengths[k]), float(ny))
+              for k in idx]
+    log(f"lbfgs cpu: {len(idx)} tables copied in {time.perf_counter() - t0:.1f} s")
+    po = [O.params(float(lengths[k]), lt, 0.0, models[k]) for k in idx]
+    lp = O.lbfgs_params(m=10, ftol=1e-5, max_iter=10000, precondition=True)
+    lib = O.load(omp=True)
+    t1 = time.perf_counter()
+    hs, rs = O.lbfgs_batch([np.full(lengths[k], ny / 2.0) for k in idx], po, tables, lp,
+                           shifts=[shifts[k] for k in idx])
+    sec = time.perf_counter() - t1
+    e_err, it_diff, conv = [], [], 0
+    for j, k in enumerate(idx):
+        e_ref = O.objective(hs[j], po[j], tables[j])
+        e_gpu = O.objective(h64[offs[k]:offs[k + 1]], po[j], tables[j])
+        e_err.append(abs(e_gpu - e_ref) / abs(e_ref))
+        it_diff.append(abs(res64[k].iterations - rs[j].iterations))
+        conv += int(rs[j].converged)
+    e_err = np.array(e_err)
+    cpu = {"value": len(idx) / sec, "unit": "minima/s", "cores": int(lib.orc_num_threads()),
+           "kind": "port", **omp_info,
+           "sample": f"{len(idx)} lines of the batch (every {stride}th; lengths "
+                     f"{sorted(set(lengths[k] for k in idx))}), oracle L-BFGS m=10 ftol 1e-5 "
+                     f"preconditioned, OpenMP over lines, {sec:.1f} s"}
+    parity = {"lines": len(idx), "oracle_converged": conv,
+              "energy_rel_err_max": float(e_err.max()),
+              "energy_rel_err_median": float(np.median(e_err)),
+              "lines_within_1e-12": int((e_err <= 1e-12).sum()),
+              "lines_within_1e-11": int((e_err <= 1e-11).sum()),
+              "iterations_abs_diff_max": int(max(it_diff))}
+    log(f"lbfgs cpu: {json.dumps(cpu)} parity {json.dumps(parity)}")
+    return cpu, parity
+
+
+def bench_langevin(log, nsteps=100000, with_oracle=True, omp_info=None):
     """Finite-temperature anneal: one line of L = 65536 sites on a 65536 x 4096
     FilteredNoise table (xi = 4, seed 1), linear elasticity, line tension 1,
     flat start at Ny/2, f = 0.5 f_c(est). Euler-Maruyama with dt = 0.01 for
@@ -239,7 +367,13 @@ def bench_langevin(log, nsteps=100000):
     with the drivers' one-step lag, src/qew_runs.h:154-170), Philox normals
     keyed by (seed 2, step, site). FP64, FP32 (anchored offsets, FP32 table)
     and FP32 on the midpoint-contracted table run the same noise; the mean
-    velocity, W^2 and the roughness exponent of the final line are compared."""
+    velocity, W^2 and the roughness exponent of the final line are compared.
+
+    With `with_oracle`, the FP64 oracle runs the same anneal (all 10^5 steps,
+    OpenMP over sites, one core fewer than the process may use) on a
+    background thread while the caller goes on with other GPU work; the
+    returned `finish()` joins it and adds the comparison with the oracle and
+    the CPU baseline to the result."""
     import torch
     import qew_amd as q
     Lc, ny = 65536, 4096
@@ -258,6 +392,7 @@ def bench_langevin(log, nsteps=100000):
     diff = temp * np.sqrt(2 * dt)
     p = q.Params(float(Lc), 1.0, f, q.Model.Linear)
     out = {}
+    lines = {}
     for name, nz, dtype in (("f64", n64, q.F64), ("f32", n32, q.F32), ("f32_mid", nmid, q.F32)):
         b = q.Batch([Lc], [nz], [p])
         if dtype == q.F64:
@@ -276,6 +411,7 @@ def bench_langevin(log, nsteps=100000):
         torch.cuda.synchronize()
         sec = time.perf_counter() - t1
         hh = h.double().cpu().numpy() + (ny / 2.0 if dtype == q.F32 else 0.0)
+        lines[name] = hh
         # roughness exponent of the final line from its height-difference
         # ACF over 16 <= ell <= 4096 grid units
         from qew_amd import analysis
@@ -297,90 +433,165 @@ def bench_langevin(log, nsteps=100000):
     out["workload"] = (f"L={Lc}, table {Lc}x{ny}, xi=4, linear, f=0.5*f_c(est)={f:g}, "
                        f"dt={dt}, {nsteps} steps, T 0.5 -> 0.01")
     log(f"langevin: {json.dumps(out)}")
-    return out
+    if not with_oracle:
+        return out, lambda: None
+
+    import threading
+    O = oracle()
+    coef = n64.coefficients()
+    del n32, nmid, n64
+    box = {}
+
+    def anneal():
+        lib = O.load(omp=True)
+        lib.orc_set_num_threads(max(1, int(omp_info["OMP_NUM_THREADS"]) - 1))
+        box["threads"] = int(lib.orc_num_threads())
+        on = O.OracleNoise.table(coef, float(Lc), float(ny))
+        po = O.params(float(Lc), 1.0, f, O.LINEAR)
+        h_ref = np.full(Lc, ny / 2.0)
+        t = time.perf_counter()
+        O.langevin_run(h_ref, po, on, drift, diff, 2, 0, omp=True)
+        box["seconds"] = time.perf_counter() - t
+        box["h"] = h_ref
+
+    th = threading.Thread(target=anneal, daemon=True)
+    th.start()
+    log("langevin: FP64 oracle anneal started on a background thread")
+
+    def finish():
+        th.join()
+        h_ref = box["h"]
+        v_ref = (h_ref.mean() - ny / 2.0) / total
+        w2_ref = h_ref.var()
+        out["vs_oracle"] = {
+            name: {"mean_velocity_rel_err": abs(out[name]["mean_velocity"] - v_ref) / abs(v_ref),
+                   "W2_rel_err": abs(out[name]["W2"] - w2_ref) / w2_ref,
+                   "h_max_abs_err": float(np.max(np.abs(lines[name] - h_ref)))}
+            for name in lines}
+        out["vs_oracle"]["oracle"] = {"mean_velocity": float(v_ref), "W2": float(w2_ref)}
+        out["cpu_baseline"] = {
+            "value": nsteps * Lc / box["seconds"], "unit": "site-steps/s",
+            "cores": box["threads"], "kind": "port", **omp_info,
+            "sample": f"the whole anneal ({nsteps} steps of {Lc} sites), FP64 oracle with "
+                      f"Philox normals, OpenMP over sites, {box['seconds']:.1f} s, run while the "
+                      "GPU did the Rosso-Krauth workload"}
+        log(f"langevin vs oracle: {json.dumps(out['vs_oracle'])} cpu {json.dumps(out['cpu_baseline'])}")
+
+    return out, finish
 
 
-def bench_rk(log, nlines=64, budget_s=20.0):
+def bench_rk(log, nlines=64, budget_s=60.0, precisions=("f64", "f32_mid"), k_way=4,
+             with_cpu=True, omp_info=None, cpu_lines=2):
     """Batched Rosso-Krauth depinning: nlines realizations of L = 16384 sites,
     each on its own 16384 x 1024 table (period M = 1024 in h, xi = 4, seeds
     1..nlines), linear elasticity, line tension 1, the depinning tool's knobs
     (dstep 0.5, max_advance Ly, runaway 2 Ly, root_tol 1e-10, sweep_tol 1e-8,
-    max_sweeps 200000). Relax at f = 0, ramp f = 0.125 * 2^k per line until it
-    runs away, then bisect every line's bracket, each probe warm-started from
-    the line's highest blocked configuration (no-passing). Probes are issued
-    while `budget_s` lasts; near f_c a probe takes 10^4..2*10^5 sweeps, so the
-    bracket after the budget is reported, not f_c to 1e-10. FP32 tables (FP64
-    state, comparisons and bisection) and FP64 tables run the same probes."""
+    max_sweeps 200000), f_c of every line to 1e-10 relative by
+    qew_amd.depinning.fc_search (k_way forces per line per round) while
+    `budget_s` lasts per precision:
+    FP64 tables, and FP32 or midpoint-contracted FP32 tables (FP64 state,
+    comparisons and bisection). Throughput: sweeps of the batch and of single
+    lines per second, and the site-velocity evaluations actually performed.
+    The CPU baseline runs the oracle (OpenMP over sites) through the same first
+    two probes (relaxation at f = 0, ramp probe f = 0.125) on `cpu_lines`
+    lines, whose heights must equal the GPU's bit for bit."""
     import torch
     import qew_amd as q
+    from qew_amd.depinning import fc_search
     Lr, ny = 16384, 1024
     t0 = time.perf_counter()
     specs = [(Lr, ny, float(Lr), float(ny), 1.0, 4.0, 4.0, 1 + k) for k in range(nlines)]
     n64 = q.FilteredNoise.generate_batch(specs, dtype=q.F64)
-    n32 = [nz.convert(q.F32) for nz in n64]
-    # lines of n = L sites on L-wide tables of spacing 1: midpoint tables apply
-    nmid = [nz.convert(q.F32_MID) for nz in n64]
+    tables = {"f64": n64}
+    if "f32" in precisions:
+        tables["f32"] = [nz.convert(q.F32) for nz in n64]
+    if "f32_mid" in precisions:
+        # lines of n = L sites on L-wide tables of spacing 1: midpoint tables apply
+        tables["f32_mid"] = [nz.convert(q.F32_MID) for nz in n64]
     torch.cuda.synchronize()
     log(f"rk: {nlines} tables generated in {time.perf_counter() - t0:.1f} s")
-    params = [q.Params(float(Lr), 1.0, 0.0, q.Model.Linear)] * nlines
     rp = q.RkParams(0.5, float(ny), 1e-10, 1e-8, 2.0 * ny, 200000)
     out = {}
     brackets = {}
-    for name, noises in (("f32", n32), ("f64", n64), ("f32_mid", nmid)):
-        b = q.Batch([Lr] * nlines, noises, params)
-        h_lo = torch.full((nlines * Lr,), ny / 2.0, dtype=torch.float64, device="cuda:0")
-        lo = np.zeros(nlines)
-        hi = np.full(nlines, np.inf)
-        evals = 0
-        sweeps = 0
-        probes = 0
-        f_ramp = 0.125
-        t1 = time.perf_counter()
-        forces = np.zeros(nlines)
-        while True:
-            b.set_forces(forces)
-            h = h_lo.clone()
-            res = b.rk_block(h, rp)
-            probes += 1
-            evals += sum(r.n_velocity_evals for r in res)
-            sweeps += max(r.sweeps for r in res)
-            blocked = np.array([r.blocked for r in res])
-            if probes == 1:
-                assert blocked.all(), "a line does not block at f = 0"
-                h_lo = h
-            else:
-                m = torch.tensor(np.repeat(blocked & todo, Lr), device="cuda:0")
-                h_lo = torch.where(m, h, h_lo)
-                lo = np.where(blocked & todo, forces, lo)
-                hi = np.where(~blocked & todo, forces, hi)
-            todo = np.isinf(hi) | (hi - lo > 1e-10 * np.maximum(np.abs(lo), np.abs(hi)))
-            if not todo.any() or time.perf_counter() - t1 > budget_s:
-                break
-            forces = np.where(np.isinf(hi), np.maximum(2 * lo, f_ramp), 0.5 * (lo + hi))
-            forces = np.where(todo, forces, lo)  # finished lines: one blocked sweep
-            f_ramp *= 2
-        sec = time.perf_counter() - t1
-        brackets[name] = (lo.copy(), hi.copy())
-        width = (hi - lo) / np.maximum(hi, 1e-300)
-        out[name] = {"velocity_evals_per_s": evals / sec, "sweeps_per_s": sweeps / sec,
-                     "seconds": sec, "probes": probes, "sweeps": sweeps,
-                     "velocity_evals": int(evals),
-                     "f_c_bracket_median": [float(np.median(lo)), float(np.median(hi))],
-                     "bracket_rel_width_max": float(width.max())}
-        log(f"rk {name}: {json.dumps(out[name])}")
-        del b, h, h_lo
-    same = (np.array_equal(brackets["f32"][0], brackets["f64"][0])
-            and np.array_equal(brackets["f32"][1], brackets["f64"][1]))
-    out["f32_f64_same_brackets"] = bool(same)
-    out["f32_mid_f64_same_brackets"] = bool(
-        np.array_equal(brackets["f32_mid"][0], brackets["f64"][0])
-        and np.array_equal(brackets["f32_mid"][1], brackets["f64"][1]))
+    first = {}
+
+    def record(k, res, h, sec):
+        first[k] = ([r.sweeps for r in res[:cpu_lines]], h[:cpu_lines].cpu().numpy(), sec,
+                    sum(r.sweeps for r in res))
+
+    for name in precisions:
+        log(f"rk {name}:")
+        lo, hi, h_lo, st = fc_search(tables[name], Lr, ny / 2.0, rp, k_way=k_way,
+                                     budget_s=budget_s, log=log,
+                                     record=record if name == "f64" else None)
+        brackets[name] = (lo, hi)
+        width = (hi - lo) / np.maximum(np.abs(hi), 1e-300)
+        sec = st.pop("seconds")
+        out[name] = {"sweeps_per_s": st["batch_sweeps"] / sec,
+                     "line_sweeps_per_s": st["line_sweeps"] / sec,
+                     "velocity_evals_done_per_s": st["evals_done"] / sec,
+                     "reference_equivalent_velocity_evals_per_s": st["evals_reference"] / sec,
+                     "seconds": sec, **st,
+                     "f_c_median": float(np.median(0.5 * (lo + hi))),
+                     "f_c_min_max": [float(lo.min()), float(hi.max())],
+                     "bracket_rel_width_max": float(width.max()),
+                     "lines_at_1e-10": int((width <= 1e-10).sum())}
+        log(f"rk {name}: {json.dumps({k: v for k, v in out[name].items() if k != 'round_log'})}")
+    for name in precisions:
+        if name == "f64":
+            continue
+        lo, hi = brackets[name]
+        lo64, hi64 = brackets["f64"]
+        fc, fc64 = 0.5 * (lo + hi), 0.5 * (lo64 + hi64)
+        out[f"{name}_vs_f64"] = {
+            "f_c_rel_err_max": float(np.max(np.abs(fc - fc64) / fc64)),
+            "identical_brackets": int(np.sum((lo == lo64) & (hi == hi64)))}
     out["workload"] = (f"{nlines} lines x L={Lr}, tables {Lr}x{ny} (period M={ny}), xi=4, "
-                       f"linear, f_c ramp + bisection, {budget_s:g} s budget per precision")
-    out["velocity_evals_note"] = ("evaluations the reference algorithm makes (sites whose "
-                                  "inputs did not change are skipped on the GPU but counted)")
+                       f"linear, ramp + {k_way}-way bracketing of f_c to 1e-10, "
+                       f"{budget_s:g} s budget per precision")
     log(f"rk: {json.dumps(out)}")
+    if 0 in first and 1 in first:
+        if with_cpu:
+            out["cpu_baseline"], out["first_probes_vs_oracle"] = rk_oracle_sample(
+                log, n64[:cpu_lines], Lr, ny, first, omp_info, nlines)
+        else:
+            # for the caller to run the oracle sample later (not serialised)
+            out["first_probes"] = {"noises": n64[:cpu_lines], "Lr": Lr, "ny": ny,
+                                   "first": first, "nlines": nlines}
     return out
+
+
+def rk_oracle_sample(log, noises, Lr, ny, first, omp_info, nlines):
+    """The oracle's rk_block (OpenMP over the sites of a colour) through the
+    relaxation at f = 0 and the ramp probe f = 0.125 on the first lines of the
+    batch: line sweeps per second against the GPU's for the same two probes on
+    all lines, and the heights and sweep counts of those lines bit for bit."""
+    O = oracle()
+    lib = O.load(omp=True)
+    rpo = O.rk_params(dstep=0.5, max_advance=float(ny), root_tol=1e-10, sweep_tol=1e-8,
+                      runaway=2.0 * ny, max_sweeps=200000)
+    sweeps, same, sec = 0, True, 0.0
+    for j, nz in enumerate(noises):
+        on = O.OracleNoise.table(nz.coefficients(), float(Lr), float(ny))
+        h = np.full(Lr, ny / 2.0)
+        for k, f in ((0, 0.0), (1, 0.125)):
+            t = time.perf_counter()
+            r = O.rk_block(h, O.params(float(Lr), 1.0, f, O.LINEAR), on, rpo, omp=True)
+            sec += time.perf_counter() - t
+            sweeps += r.sweeps
+            same = same and r.sweeps == first[k][0][j] and np.array_equal(h, first[k][1][j])
+    gpu_sec = first[0][2] + first[1][2]
+    gpu_line_sweeps = first[0][3] + first[1][3]
+    cpu = {"value": sweeps / sec, "unit": "line sweeps/s (16384 sites)",
+           "cores": int(lib.orc_num_threads()), "kind": "port", **omp_info,
+           "sample": f"relaxation at f=0 and the ramp probe f=0.125 on {len(noises)} lines "
+                     f"({sweeps} line sweeps, {sec:.1f} s)",
+           "gpu_same_probes": {"value": gpu_line_sweeps / gpu_sec,
+                               "sample": f"the same two probes on all {nlines} lines, "
+                                         f"{gpu_sec:.2f} s"}}
+    parity = {"lines": len(noises), "heights_and_sweeps_bit_identical": bool(same)}
+    log(f"rk cpu: {json.dumps(cpu)} {json.dumps(parity)}")
+    return cpu, parity
 
 
 def main():
@@ -396,7 +607,14 @@ def main():
                     help="anneal length (the workload is 10^5 steps; fewer only for profiling)")
     ap.add_argument("--no-rk", action="store_true", help="skip the batched Rosso-Krauth workload")
     ap.add_argument("--rk-budget", type=float, default=20.0,
-                    help="seconds of f_c bisection per precision in the Rosso-Krauth workload")
+                    help="seconds of f_c search per precision in the Rosso-Krauth workload "
+                         "(checked between rounds; f_c to 1e-10 takes about 15 rounds)")
+    ap.add_argument("--rk-only", action="store_true",
+                    help="run only the Rosso-Krauth workload and print its JSON object")
+    ap.add_argument("--rk-precisions", default="f64,f32_mid",
+                    help="table types of the Rosso-Krauth workload (f64, f32, f32_mid)")
+    ap.add_argument("--rk-kway", type=int, default=4,
+                    help="forces probed per line per round of the f_c search")
     ap.add_argument("--lbfgs-lines", type=int, default=512)
     ap.add_argument("--only", choices=["both", "f32", "f64"], default="both")
     ap.add_argument("--no-bf16", action="store_true", help="skip the BF16-table evaluation leg")
@@ -406,6 +624,7 @@ def main():
                     help="only run a 1 GiB device copy (reference point for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE scaling) and exit")
     args = ap.parse_args()
+    omp_info = omp_setup()
     if args.copy_calibration:
         import torch
         x = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
@@ -422,9 +641,16 @@ def main():
 
     import torch
     import qew_amd as q
+    unpin()
 
     dev = torch.device("cuda:0")
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    if args.rk_only:
+        out = bench_rk(log, budget_s=args.rk_budget,
+                       precisions=tuple(args.rk_precisions.split(",")), k_way=args.rk_kway,
+                       with_cpu=not args.no_cpu, omp_info=omp_info)
+        print(json.dumps(out), flush=True)
+        return
     h0 = walk()
     B = args.evals
     shifts = torch.tensor((np.arange(B) % NSHIFT) * SHIFT, dtype=torch.float64, device=dev)
@@ -525,6 +751,7 @@ def main():
         coef = results["table64"].coefficients()
         del results["table64"]
         cpu = cpu_baseline(coef, h0)
+        cpu.update(omp_info)
         log(f"cpu: {json.dumps(cpu)}")
 
     head = r32 if r32 else r64
@@ -543,6 +770,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": head["ms_per_step"],
+        "kernel_ms": head["kernel_ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -558,6 +786,7 @@ def main():
     }
     if r64:
         line["f64"] = {"value": r64["evals_per_s"], "ms_per_step": r64["ms_per_step"],
+                       "kernel_ms": r64["kernel_ms"],
                        "roofline_achieved_GBs": r64["achieved_GBs"],
                        "roofline_frac": r64["achieved_GBs"] / HBM_PEAK_GBS,
                        "bytes_per_eval": r64["bytes_per_eval"]}
@@ -568,6 +797,7 @@ def main():
     if r16:
         # BF16 coefficient storage, FP32 line state and arithmetic (optional mode)
         line["bf16_table"] = {"value": r16["evals_per_s"], "ms_per_step": r16["ms_per_step"],
+                              "kernel_ms": r16["kernel_ms"],
                               "bytes_per_eval": r16["bytes_per_eval"],
                               "roofline_achieved_GBs": r16["achieved_GBs"],
                               "roofline_frac": r16["achieved_GBs"] / HBM_PEAK_GBS,
@@ -576,19 +806,35 @@ def main():
         # midpoint-contracted FP32 table: 4 values per edge instead of the
         # 4 x 4 window (every edge of this line lies at x-fraction 1/2)
         line["f32_mid_table"] = {"value": rmid["evals_per_s"], "ms_per_step": rmid["ms_per_step"],
+                                 "kernel_ms": rmid["kernel_ms"],
                                  "bytes_per_eval": rmid["bytes_per_eval"],
                                  "roofline_achieved_GBs": rmid["achieved_GBs"],
                                  "roofline_frac": rmid["achieved_GBs"] / HBM_PEAK_GBS,
                                  "error_vs_f64": error_vs_f64(firstmid) if r64 else None}
+    with_cpu = not args.no_cpu
     if not args.no_lbfgs:
         torch.cuda.empty_cache()
-        line["batched_lbfgs"] = bench_lbfgs(log, nlines=args.lbfgs_lines)
+        line["batched_lbfgs"] = bench_lbfgs(log, nlines=args.lbfgs_lines, with_cpu=with_cpu,
+                                            omp_info=omp_info)
+    finish_langevin = lambda: None
     if not args.no_langevin:
         torch.cuda.empty_cache()
-        line["langevin"] = bench_langevin(log, nsteps=args.langevin_steps)
+        # the FP64 oracle anneal runs on host threads during the next leg
+        line["langevin"], finish_langevin = bench_langevin(
+            log, nsteps=args.langevin_steps, with_oracle=with_cpu, omp_info=omp_info)
     if not args.no_rk:
         torch.cuda.empty_cache()
-        line["rosso_krauth"] = bench_rk(log, budget_s=args.rk_budget)
+        line["rosso_krauth"] = bench_rk(log, budget_s=args.rk_budget,
+                                        precisions=tuple(args.rk_precisions.split(",")),
+                                        k_way=args.rk_kway, with_cpu=False, omp_info=omp_info)
+    finish_langevin()
+    if not args.no_rk and with_cpu:
+        # the oracle's Rosso-Krauth sample after the anneal has freed the cores
+        rk = line["rosso_krauth"]
+        if "first_probes" in rk:
+            fp = rk.pop("first_probes")
+            rk["cpu_baseline"], rk["first_probes_vs_oracle"] = rk_oracle_sample(
+                log, fp["noises"], fp["Lr"], fp["ny"], fp["first"], omp_info, fp["nlines"])
     print(json.dumps(line), flush=True)
 
 

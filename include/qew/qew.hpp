@@ -12,7 +12,10 @@
 //   lbfgs_minimize                     qew::lbfgs_minimize
 //   fire_minimize                      qew::fire_minimize
 //   LangevinDynamics                   qew::LangevinDynamics
-//   rk_site_velocity / rk_block        qew::rk_block
+//   NoiseSample (src/qew_types.h:26)   qew::NoiseSample
+//   rk_site_velocity, rk_forward_zero,
+//   rk_block                           qew::rk_site_velocity, rk_forward_zero,
+//                                      rk_block
 //   static_sweep / dynamic_anneal /
 //   depinning_ramp                     qew::static_sweep / dynamic_anneal /
 //                                      depinning_ramp (qew_runs.h)
@@ -23,6 +26,7 @@
 #ifndef QEW_FACADE_HPP
 #define QEW_FACADE_HPP
 
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -135,6 +139,24 @@ private:
     std::string label_;
 };
 
+// A device array of ints (site indices of the per-site Rosso-Krauth calls).
+class DeviceInts {
+public:
+    explicit DeviceInts(const std::vector<int> &v) : n_(v.size()) {
+        void *p = nullptr;
+        check(qew_device_alloc(context(), sizeof(int) * (n_ ? n_ : 1), &p));
+        qew_ctx c = context();
+        ptr_ = std::shared_ptr<int>(static_cast<int *>(p), [c](int *q) { qew_device_free(c, q); });
+        if (n_) check(qew_copy_to_device(context(), ptr_.get(), v.data(), sizeof(int) * n_));
+    }
+    const int *data() const { return ptr_.get(); }
+    size_t size() const { return n_; }
+
+private:
+    std::shared_ptr<int> ptr_;
+    size_t n_;
+};
+
 inline void deep_copy(const View1D &dst, double value) { const_cast<View1D &>(dst).fill(value); }
 inline void deep_copy(const View1D &dst, const std::vector<double> &src) {
     const_cast<View1D &>(dst).copy_from_host(src.data());
@@ -142,8 +164,18 @@ inline void deep_copy(const View1D &dst, const std::vector<double> &src) {
 inline std::vector<double> to_host_vec(const View1D &h) { return h.to_host(); }
 
 // ---- potentials ----------------------------------------------------------
+// Value and partial derivatives of the pinning potential at a point.
+struct NoiseSample {
+    real_t v;
+    real_t dv_dx;
+    real_t dv_dy;
+};
+
 // A potential on the device (the reference's Noise concept,
 // src/qew_model.h:31-48). Shared handle: copies are cheap, like DeviceNoise.
+// The reference's sample / sample_value / sample_dy are device functions
+// called inside its kernels; here they evaluate device views of points in
+// one launch, or host points through one launch per call.
 class DeviceNoise {
 public:
     DeviceNoise() = default;
@@ -151,6 +183,42 @@ public:
     qew_noise handle() const { return h_.get(); }
     int nx() const { return info().nx; }
     int ny() const { return info().ny; }
+
+    // src/filtered_noise.h:103-134: value and both derivatives at (x[k], y[k])
+    void sample(const View1D &x, const View1D &y, const View1D &v, const View1D &dv_dx,
+                const View1D &dv_dy) const {
+        const size_t m = same_extent(x, y, v);
+        if (dv_dx.extent(0) != m || dv_dy.extent(0) != m)
+            throw std::invalid_argument("sample: views of different extents");
+        check(qew_noise_sample(context(), handle(), (int)m, x.data(), y.data(), v.data(),
+                               dv_dx.data(), dv_dy.data()));
+    }
+    // src/filtered_noise.h:138-155: the value-only sampler (objective)
+    void sample_value(const View1D &x, const View1D &y, const View1D &v) const {
+        const size_t m = same_extent(x, y, v);
+        check(qew_noise_sample_value(context(), handle(), (int)m, x.data(), y.data(), v.data()));
+    }
+    // src/filtered_noise.h:159-176: the dy-only sampler (gradient, site velocity)
+    void sample_dy(const View1D &x, const View1D &y, const View1D &dv_dy) const {
+        const size_t m = same_extent(x, y, dv_dy);
+        check(qew_noise_sample_dy(context(), handle(), (int)m, x.data(), y.data(), dv_dy.data()));
+    }
+    std::vector<NoiseSample> sample(const std::vector<real_t> &x,
+                                    const std::vector<real_t> &y) const {
+        if (x.size() != y.size()) throw std::invalid_argument("sample: x and y differ in length");
+        const size_t m = x.size();
+        View1D X(m), Y(m), V(m), DX(m), DY(m);
+        X.copy_from_host(x.data());
+        Y.copy_from_host(y.data());
+        sample(X, Y, V, DX, DY);
+        const std::vector<double> v = V.to_host(), dx = DX.to_host(), dy = DY.to_host();
+        std::vector<NoiseSample> out(m);
+        for (size_t k = 0; k < m; ++k) out[k] = NoiseSample{v[k], dx[k], dy[k]};
+        return out;
+    }
+    NoiseSample sample(real_t x, real_t y) const { return sample(std::vector<real_t>{x}, {y})[0]; }
+    real_t sample_value(real_t x, real_t y) const { return point(x, y, false); }
+    real_t sample_dy(real_t x, real_t y) const { return point(x, y, true); }
 
 private:
     struct Info {
@@ -161,6 +229,19 @@ private:
         Info i{};
         check(qew_noise_info(handle(), &i.kind, &i.dtype, &i.nx, &i.ny, &i.dx, &i.dy));
         return i;
+    }
+    static size_t same_extent(const View1D &a, const View1D &b, const View1D &c) {
+        if (a.extent(0) != b.extent(0) || a.extent(0) != c.extent(0))
+            throw std::invalid_argument("sample: views of different extents");
+        return a.extent(0);
+    }
+    real_t point(real_t x, real_t y, bool dy) const {
+        View1D X(1), Y(1), R(1);
+        X.copy_from_host(&x);
+        Y.copy_from_host(&y);
+        if (dy) sample_dy(X, Y, R);
+        else sample_value(X, Y, R);
+        return R.to_host()[0];
     }
     std::shared_ptr<qew_noise_s> h_;
 };
@@ -267,9 +348,18 @@ real_t objective(const View1D &x0, const View1D &d, real_t alpha, const Params &
     return e;
 }
 
-// The fused kernel needs no per-edge scratch; kept for source compatibility.
+// The per-edge quantities of gradient() (src/qew_model.h:126-147): the
+// fused kernel evaluates them in registers and, when a workspace is passed,
+// also writes them here, so a caller can read them after the call.
 struct GradientWorkspace {
-    void ensure(int, Model) {}
+    View1D edge_dvdy;   // dV/dy at the midpoint of edge (i, i+1 mod n)
+    View1D edge_slope;  // arclength only: s / sqrt(1 + s^2) on that edge
+
+    void ensure(int n, Model model) {
+        if ((int)edge_dvdy.extent(0) != n) edge_dvdy = View1D("edge_dvdy", (size_t)n);
+        if (model == Model::Arclength && (int)edge_slope.extent(0) != n)
+            edge_slope = View1D("edge_slope", (size_t)n);
+    }
 };
 
 template <class Noise>
@@ -282,8 +372,14 @@ void gradient(const View1D &h, const Params &p, const Noise &noise, const View1D
 
 template <class Noise>
 void gradient(const View1D &h, const Params &p, const Noise &noise, const View1D &grad,
-              GradientWorkspace &) {
-    gradient(h, p, noise, grad);
+              GradientWorkspace &ws) {
+    if (grad.extent(0) != h.extent(0)) throw std::invalid_argument("grad extent differs from h");
+    const int n = (int)h.extent(0);
+    ws.ensure(n, p.model);
+    const qew_params c = p.c();
+    check(qew_gradient_edges(context(), as_device(noise).handle(), &c, n, h.data(), grad.data(),
+                             ws.edge_dvdy.data(),
+                             p.model == Model::Arclength ? ws.edge_slope.data() : nullptr));
 }
 
 // ---- L-BFGS (src/lbfgs.h) ---------------------------------------------------
@@ -429,6 +525,85 @@ struct RkResult {
 inline RkResult from_c(const qew_rk_result &r) {
     return RkResult{r.blocked != 0, r.runaway != 0, r.sweeps, r.total_advance, r.mean_h,
                     r.n_velocity_evals, r.n_velocity_evals_done};
+}
+
+namespace detail {
+// qew_params whose physical_size / n reproduces dx exactly (the kernels
+// derive the spacing as L / n).
+inline qew_params params_for_spacing(int n, real_t dx, real_t lt, real_t f, Model model) {
+    double L = dx * (double)n;
+    for (int k = 0; k < 8 && L / (double)n != dx; ++k)
+        L = std::nextafter(L, (L / (double)n < dx) ? HUGE_VAL : -HUGE_VAL);
+    if (L / (double)n != dx) throw std::invalid_argument("no box length L with L / n == dx");
+    return qew_params{L, lt, f, static_cast<int>(model)};
+}
+}  // namespace detail
+
+// rk_site_velocity (src/rosso_krauth.h:52-82) at sites i[k] of a line of n
+// sites of spacing dx, trial heights hi[k], neighbours hl[k], hr[k] fixed;
+// v[k] out. One launch for all sites.
+template <class Noise>
+void rk_site_velocity(const std::vector<int> &i, int n, real_t dx, real_t lt, real_t f,
+                      Model model, const View1D &hi, const View1D &hl, const View1D &hr,
+                      const Noise &noise, const View1D &v) {
+    const size_t m = i.size();
+    if (hi.extent(0) != m || hl.extent(0) != m || hr.extent(0) != m || v.extent(0) != m)
+        throw std::invalid_argument("rk_site_velocity: views of different extents");
+    const qew_params c = detail::params_for_spacing(n, dx, lt, f, model);
+    const DeviceInts idx(i);
+    check(qew_rk_site_velocity(context(), as_device(noise).handle(), &c, n, (int)m, idx.data(),
+                               hi.data(), hl.data(), hr.data(), v.data()));
+}
+
+// The same for one site, with the reference's argument list (one launch).
+template <class Noise>
+real_t rk_site_velocity(int i, int n, real_t dx, real_t lt, real_t f, Model model, real_t hi,
+                        real_t hl, real_t hr, const Noise &noise) {
+    View1D H(1), Hl(1), Hr(1), V(1);
+    H.copy_from_host(&hi);
+    Hl.copy_from_host(&hl);
+    Hr.copy_from_host(&hr);
+    rk_site_velocity(std::vector<int>{i}, n, dx, lt, f, model, H, Hl, Hr, noise, V);
+    return V.to_host()[0];
+}
+
+// rk_forward_zero (src/rosso_krauth.h:88-124) at sites i[k]: out[k] the
+// nearest forward zero of the site's velocity (hi[k] itself when it is
+// <= 0), found[k] false when the march reached rp.max_advance without one.
+template <class Noise>
+void rk_forward_zero(const std::vector<int> &i, int n, real_t dx, real_t lt, real_t f,
+                     Model model, const View1D &hi, const View1D &hl, const View1D &hr,
+                     const Noise &noise, const RkParams &rp, const View1D &out,
+                     std::vector<bool> &found) {
+    const size_t m = i.size();
+    if (hi.extent(0) != m || hl.extent(0) != m || hr.extent(0) != m || out.extent(0) != m)
+        throw std::invalid_argument("rk_forward_zero: views of different extents");
+    const qew_params c = detail::params_for_spacing(n, dx, lt, f, model);
+    const qew_rk_params r = rp.c();
+    const DeviceInts idx(i);
+    const DeviceInts flags(std::vector<int>(m, 0));
+    check(qew_rk_forward_zero(context(), as_device(noise).handle(), &c, &r, n, (int)m, idx.data(),
+                              hi.data(), hl.data(), hr.data(), out.data(),
+                              const_cast<int *>(flags.data()), nullptr));
+    std::vector<int> fh(m);
+    if (m) check(qew_copy_to_host(context(), fh.data(), flags.data(), sizeof(int) * m));
+    found.assign(m, false);
+    for (size_t k = 0; k < m; ++k) found[k] = fh[k] != 0;
+}
+
+// The same for one site, with the reference's argument list.
+template <class Noise>
+real_t rk_forward_zero(int i, int n, real_t dx, real_t lt, real_t f, Model model, real_t hi,
+                       real_t hl, real_t hr, const Noise &noise, const RkParams &rp,
+                       bool &found) {
+    View1D H(1), Hl(1), Hr(1), O(1);
+    H.copy_from_host(&hi);
+    Hl.copy_from_host(&hl);
+    Hr.copy_from_host(&hr);
+    std::vector<bool> fd;
+    rk_forward_zero(std::vector<int>{i}, n, dx, lt, f, model, H, Hl, Hr, noise, rp, O, fd);
+    found = fd[0];
+    return O.to_host()[0];
 }
 
 template <class Noise>

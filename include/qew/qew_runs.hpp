@@ -222,6 +222,11 @@ struct DepinningConfig {
     // blocked configuration found so far (no-passing keeps it below the
     // critical line). No reference counterpart.
     real_t fc_rel_tol = 0;
+    // forces probed at once per round of that search: 1 bisects; K > 1 probes
+    // lo + j (hi - lo) / (K + 1), j = 1..K, as one batch of K copies of the
+    // line and keeps the bracket between the highest force below which every
+    // probe blocked and the next probe (K + 1 times narrower per round).
+    int fc_probes = 1;
 };
 
 struct DepinningStep {
@@ -273,7 +278,47 @@ DepinningResult depinning_ramp(const DepinningConfig &c, const Noise &noise) {
         res.steps.push_back(std::move(st));
         if (!r.blocked) break;
     }
-    if (c.fc_rel_tol > 0 && res.f_c_upper_bound >= 0 && f_lo >= 0) {
+    if (c.fc_rel_tol > 0 && res.f_c_upper_bound >= 0 && f_lo >= 0 && c.fc_probes > 1) {
+        real_t lo = f_lo, hi = res.f_c_upper_bound;
+        std::vector<double> h_lo = last_blocked;
+        const int K = c.fc_probes;
+        const size_t n = (size_t)c.nx;
+        std::vector<int> lens(K, c.nx);
+        std::vector<qew_noise> nzs(K, dn.handle());
+        std::vector<qew_params> ps(K, p.c());
+        qew_batch b = nullptr;
+        check(qew_batch_create(context(), K, lens.data(), nzs.data(), ps.data(), &b));
+        std::unique_ptr<qew_batch_s, qew_status (*)(qew_batch)> owner(b, qew_batch_destroy);
+        View1D hb("h_probes", K * n);
+        std::vector<double> hh(K * n), forces(K);
+        std::vector<qew_rk_result> rr(K);
+        const qew_rk_params rpc = c.rp.c();
+        while (hi - lo > c.fc_rel_tol * std::max(std::abs(hi), std::abs(lo))) {
+            bool distinct = true;
+            for (int j = 0; j < K; ++j) {
+                forces[j] = lo + (j + 1) * (hi - lo) / (K + 1);
+                distinct = distinct && forces[j] > (j ? forces[j - 1] : lo) && forces[j] < hi;
+                std::copy(h_lo.begin(), h_lo.end(), hh.begin() + j * n);
+            }
+            if (!distinct) break;  // bracket at FP64 resolution
+            hb.copy_from_host(hh.data());
+            check(qew_batch_set_forces(b, forces.data()));
+            check(qew_batch_rk_block(b, &rpc, hb.data(), rr.data()));
+            res.bisection_probes += K;
+            int jb = 0;  // probes 1..jb all blocked
+            while (jb < K && rr[jb].blocked) ++jb;
+            if (jb > 0) {
+                lo = forces[jb - 1];
+                hb.copy_to_host(hh.data());
+                h_lo.assign(hh.begin() + (jb - 1) * n, hh.begin() + jb * n);
+            }
+            if (jb < K) hi = forces[jb];
+        }
+        res.f_c_lower = lo;
+        res.f_c_upper = hi;
+        res.f_c = 0.5 * (lo + hi);
+        res.h_critical = h_lo;
+    } else if (c.fc_rel_tol > 0 && res.f_c_upper_bound >= 0 && f_lo >= 0) {
         real_t lo = f_lo, hi = res.f_c_upper_bound;
         std::vector<double> h_lo = last_blocked;
         while (hi - lo > c.fc_rel_tol * std::max(std::abs(hi), std::abs(lo))) {

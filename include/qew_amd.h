@@ -232,6 +232,14 @@ qew_status qew_noise_coefficients(qew_noise noise, double *coef_host);
 qew_status qew_noise_sample(qew_ctx ctx, qew_noise noise, int m, const double *x,
                             const double *y, double *v, double *dvdx, double *dvdy);
 
+/* DeviceNoise::sample_value and DeviceNoise::sample_dy at m device points
+ * (src/filtered_noise.h:138-176): the value-only and dy-only samplers, whose
+ * arithmetic the objective and the gradient use. */
+qew_status qew_noise_sample_value(qew_ctx ctx, qew_noise noise, int m, const double *x,
+                                  const double *y, double *v);
+qew_status qew_noise_sample_dy(qew_ctx ctx, qew_noise noise, int m, const double *x,
+                               const double *y, double *dvdy);
+
 /* ---- one line, FP64 (src/qew_model.h) ----------------------------------- */
 
 /* objective(h, p, noise) (src/qew_model.h:101-106). h: device FP64[n]. */
@@ -244,6 +252,14 @@ qew_status qew_objective_trial(qew_ctx ctx, qew_noise noise, const qew_params *p
 /* gradient(h, p, noise, grad) (src/qew_model.h:149-201). grad: device FP64[n]. */
 qew_status qew_gradient(qew_ctx ctx, qew_noise noise, const qew_params *p, int n,
                         const double *h, double *grad);
+/* gradient(h, p, noise, grad, ws) (src/qew_model.h:126-201) with the
+ * GradientWorkspace's per-edge quantities written out: edge_dvdy[i] = dV/dy at
+ * the midpoint of edge (i, i+1 mod n), and for the arclength model
+ * edge_slope[i] = s / sqrt(1 + s^2), s = (h[i+1] - h[i]) / dx (edge_slope may
+ * be NULL; it is not written for the linear model). Device FP64[n] arrays. */
+qew_status qew_gradient_edges(qew_ctx ctx, qew_noise noise, const qew_params *p, int n,
+                              const double *h, double *grad, double *edge_dvdy,
+                              double *edge_slope);
 /* gradient and objective fused in one pass (no reference counterpart; both
  * results equal the two calls above). energy may be NULL. */
 qew_status qew_eval(qew_ctx ctx, qew_noise noise, const qew_params *p, int n,
@@ -353,6 +369,18 @@ qew_status qew_batch_rk_block(qew_batch batch, const qew_rk_params *rp, double *
 /* rk_site_velocity at m device sites (src/rosso_krauth.h:52-82): v[k] for
  * site i[k] of a line of length n with trial height hi[k] and neighbours
  * hl[k], hr[k]; device arrays. */
+/* rk_forward_zero at m device sites (src/rosso_krauth.h:88-124): out[k] is
+ * the nearest forward zero of the velocity of site i[k] of a line of length n
+ * with neighbours hl[k], hr[k] fixed, starting from hi[k] (hi[k] itself when
+ * its velocity is <= 0); found[k] is 0 when the march reached
+ * rp->max_advance without a zero (out[k] is then the marched-to height, the
+ * reference's `found = false`); evals[k] counts the site-velocity
+ * evaluations. found and evals may be NULL. Device arrays. March lengths
+ * max_advance / dstep beyond 2^26 steps are rejected. */
+qew_status qew_rk_forward_zero(qew_ctx ctx, qew_noise noise, const qew_params *p,
+                               const qew_rk_params *rp, int n, int m, const int *i,
+                               const double *hi, const double *hl, const double *hr,
+                               double *out, int *found, int64_t *evals);
 qew_status qew_rk_site_velocity(qew_ctx ctx, qew_noise noise, const qew_params *p,
                                 int n, int m, const int *i, const double *hi,
                                 const double *hl, const double *hr, double *v);

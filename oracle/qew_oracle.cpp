@@ -335,14 +335,15 @@ double orc_objective_trial(int n, const double *x0, const double *d, double alph
     return objective_eval(n, p, nz, [=](int i) { return x0[i] + alpha * d[i]; });
 }
 
-// ---- two-pass gradient: src/qew_model.h:149-194 ----
-void orc_gradient(int n, const double *h, const orc_params *p, const orc_noise *nz,
-                  double *grad) {
+// ---- two-pass gradient: src/qew_model.h:149-194; edvdy / eslope are the
+//      GradientWorkspace's edge_dvdy / edge_slope (src/qew_model.h:126-147),
+//      eslope written for the arclength model only ----
+void orc_gradient_edges(int n, const double *h, const orc_params *p, const orc_noise *nz,
+                        double *grad, double *edvdy, double *eslope) {
     const double dx = p->physical_size / (double)n;
     const double lt = p->line_tension;
     const double f = p->driving_force;
     const bool arc = (p->model == 1);
-    std::vector<double> edvdy(n), eslope(arc ? n : 0);
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static)
 #endif
@@ -371,6 +372,12 @@ void orc_gradient(int n, const double *h, const orc_params *p, const orc_noise *
         }
         grad[i] = dx * (line_force + 0.5 * (edvdy[im] + edvdy[i]) - f);
     }
+}
+
+void orc_gradient(int n, const double *h, const orc_params *p, const orc_noise *nz,
+                  double *grad) {
+    std::vector<double> edvdy(n), eslope(p->model == 1 ? n : 0);
+    orc_gradient_edges(n, h, p, nz, grad, edvdy.data(), eslope.data());
 }
 
 // Batched noise query used by the sampler tests (reference
@@ -943,6 +950,16 @@ int orc_rk_block(int n, double *h, const orc_params *p, const orc_noise *nz,
     return 0;
 }
 
+// Sets the OpenMP thread count of later calls (no effect on the sequential
+// oracle).
+void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 // Number of OpenMP threads this library uses (1 for the sequential oracle).
 int orc_num_threads(void) {
 #ifdef _OPENMP
@@ -950,6 +967,57 @@ int orc_num_threads(void) {
 #else
     return 1;
 #endif
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Drivers over many steps / many lines (the reference's drivers loop over
+// these calls one by one: src/qew_runs.h:152-175 for the anneal, :70-105 for
+// the static sweep). Same arithmetic as the single calls above; in the
+// OpenMP build the work is spread over threads without changing any result
+// (the kicks are keyed by (seed, step, site), the gradient is reduction-free,
+// and each line's L-BFGS runs on one thread).
+// ===========================================================================
+extern "C" {
+
+// `nsteps` Langevin steps with per-step coefficients a[k], b[k] and
+// integrator step numbers step0 + k (src/dynamics.h:34-64 per step).
+void orc_langevin_run(int n, double *h, const orc_params *p, const orc_noise *nz, int nsteps,
+                      const double *a, const double *b, uint64_t seed, uint64_t step0) {
+    std::vector<double> g(n);
+    for (int k = 0; k < nsteps; ++k) {
+        orc_gradient(n, h, p, nz, g.data());
+        const double ak = a[k], bk = b[k];
+        const uint64_t step = step0 + (uint64_t)k;
+        if (bk > 0) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+            for (int i = 0; i < n; ++i) h[i] += -ak * g[i] + bk * orc_normal(seed, step, (uint64_t)i);
+        } else {
+            for (int i = 0; i < n; ++i) h[i] += -ak * g[i];
+        }
+    }
+}
+
+// orc_lbfgs on every line of a ragged batch: line l is h[off[l] .. off[l+1])
+// with parameters p[l], potential nz[l] and preconditioner shift shift[l]
+// (> 0 replaces lp->precond_shift). Lines are spread over OpenMP threads
+// (dynamic schedule, longest first as listed by `order`, or in index order
+// when order is null); each line runs sequentially on one thread.
+void orc_lbfgs_batch(int nlines, const int64_t *off, double *h, const orc_params *p,
+                     const orc_noise *nz, const orc_lbfgs_params *lp, const double *shift,
+                     const int *order, orc_lbfgs_result *res) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int k = 0; k < nlines; ++k) {
+        const int l = order ? order[k] : k;
+        orc_lbfgs_params lpl = *lp;
+        if (shift && shift[l] > 0) lpl.precond_shift = shift[l];
+        orc_lbfgs((int)(off[l + 1] - off[l]), h + off[l], &p[l], &nz[l], &lpl, &res[l]);
+    }
 }
 
 }  // extern "C"

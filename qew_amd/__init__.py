@@ -37,6 +37,7 @@ __all__ = [
     "Context", "context", "FilteredNoise", "AnalyticNoise", "ZeroNoise", "ConstNoise",
     "HarmonicWell", "SinusoidNoise", "objective", "objective_trial", "gradient",
     "evaluate", "lbfgs_minimize", "LangevinDynamics", "rk_block", "rk_site_velocity",
+    "rk_forward_zero", "gradient_edges",
     "Batch", "F32", "F32_MID", "F64", "F64_MID", "BF16", "QewError", "QewInvalidArgument",
 ]
 
@@ -244,6 +245,23 @@ class _Noise:
                                            _ptr(y), _ptr(v), _ptr(dvx), _ptr(dvy)))
         return v, dvx, dvy
 
+    def _part(self, name, x, y):
+        torch = _torch()
+        x = x.contiguous().double()
+        y = y.contiguous().double()
+        out = torch.empty_like(x)
+        check(getattr(_lib.load(), name)(self.ctx.handle, self.handle, x.numel(), _ptr(x),
+                                         _ptr(y), _ptr(out)))
+        return out
+
+    def sample_value(self, x, y):
+        """DeviceNoise::sample_value at device points (the objective's sampler)."""
+        return self._part("qew_noise_sample_value", x, y)
+
+    def sample_dy(self, x, y):
+        """DeviceNoise::sample_dy at device points (the gradient's sampler)."""
+        return self._part("qew_noise_sample_dy", x, y)
+
 
 class FilteredNoise(_Noise):
     """FilteredNoise (src/filtered_noise.h:183-191). `dtype` is the device
@@ -385,6 +403,21 @@ def gradient(h, p, noise, grad=None):
     return grad
 
 
+def gradient_edges(h, p, noise):
+    """gradient(h, p, noise, grad, ws) with the GradientWorkspace's per-edge
+    quantities (src/qew_model.h:126-201): returns (grad, edge dV/dy, edge slope;
+    the slope is None for the linear model)."""
+    torch = _torch()
+    grad, edvdy = torch.empty_like(h), torch.empty_like(h)
+    eslope = torch.empty_like(h) if int(p.model) == ARCLENGTH else None
+    pc = p.c()
+    check(_lib.load().qew_gradient_edges(noise.ctx.handle, noise.handle, C.byref(pc), h.numel(),
+                                         _ptr(h, torch.float64), _ptr(grad, torch.float64),
+                                         _ptr(edvdy, torch.float64),
+                                         None if eslope is None else _ptr(eslope, torch.float64)))
+    return grad, edvdy, eslope
+
+
 def evaluate(h, p, noise, grad=None):
     """Fused gradient and objective in one pass: returns (E, grad)."""
     torch = _torch()
@@ -461,6 +494,22 @@ def rk_site_velocity(i, n, p, noise, hi, hl, hr):
                                            _ptr(hl, torch.float64), _ptr(hr, torch.float64),
                                            _ptr(v, torch.float64)))
     return v
+
+
+def rk_forward_zero(i, n, p, noise, rp, hi, hl, hr):
+    """rk_forward_zero (src/rosso_krauth.h:88-124) at device arrays of sites:
+    returns (heights, found, evaluation counts)."""
+    torch = _torch()
+    out = torch.empty_like(hi)
+    found = torch.empty(hi.numel(), dtype=torch.int32, device=hi.device)
+    evals = torch.empty(hi.numel(), dtype=torch.int64, device=hi.device)
+    pc, rc = p.c(), rp.c()
+    i = i.to(torch.int32).contiguous()
+    check(_lib.load().qew_rk_forward_zero(noise.ctx.handle, noise.handle, C.byref(pc), C.byref(rc),
+                                          n, i.numel(), _ptr(i), _ptr(hi, torch.float64),
+                                          _ptr(hl, torch.float64), _ptr(hr, torch.float64),
+                                          _ptr(out, torch.float64), _ptr(found), _ptr(evals)))
+    return out, found, evals
 
 
 # ---------------------------------------------------------------------------

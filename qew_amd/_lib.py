@@ -87,10 +87,13 @@ SIGNATURES = [
     ("qew_noise_field", [VP, DP]),
     ("qew_noise_coefficients", [VP, DP]),
     ("qew_noise_sample", [VP, VP, I, VP, VP, VP, VP, VP]),
+    ("qew_noise_sample_value", [VP, VP, I, VP, VP, VP]),
+    ("qew_noise_sample_dy", [VP, VP, I, VP, VP, VP]),
     ("qew_objective", [VP, VP, C.POINTER(Params), I, VP, DP]),
     ("qew_objective_trial", [VP, VP, C.POINTER(Params), I, VP, VP, D, DP]),
     ("qew_gradient", [VP, VP, C.POINTER(Params), I, VP, VP]),
     ("qew_eval", [VP, VP, C.POINTER(Params), I, VP, VP, DP]),
+    ("qew_gradient_edges", [VP, VP, C.POINTER(Params), I, VP, VP, VP, VP]),
     ("qew_eval_noalloc", [VP, VP, C.POINTER(Params), I, VP, VP, VP]),
     ("qew_batch_create", [VP, I, C.POINTER(I), C.POINTER(VP), C.POINTER(Params), C.POINTER(VP)]),
     ("qew_batch_destroy", [VP]),
@@ -109,6 +112,8 @@ SIGNATURES = [
     ("qew_rk_block", [VP, VP, C.POINTER(Params), C.POINTER(RkParams), I, VP, C.POINTER(RkResult)]),
     ("qew_batch_rk_block", [VP, C.POINTER(RkParams), VP, C.POINTER(RkResult)]),
     ("qew_rk_site_velocity", [VP, VP, C.POINTER(Params), I, I, VP, VP, VP, VP, VP]),
+    ("qew_rk_forward_zero", [VP, VP, C.POINTER(Params), C.POINTER(RkParams), I, I, VP, VP, VP,
+                             VP, VP, VP, VP]),
 ]
 
 _RESTYPE = {

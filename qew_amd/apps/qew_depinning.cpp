@@ -1,9 +1,10 @@
 // qew_depinning: Rosso-Krauth force ramp toward the depinning threshold f_c.
 // Drop-in for the reference's executables/qew_depinning/main.cpp:31-127: the
 // same option names, the same solver defaults derived from Ly / ny, the same
-// summary lines and exit code 2 on a bad option. Added option: --fc_rel_tol
-// > 0 bisects the ramp's bracket to that relative width, each probe started
-// from the highest blocked configuration (0 keeps the ramp alone).
+// summary lines and exit code 2 on a bad option. Added options: --fc_rel_tol
+// > 0 narrows the ramp's bracket to that relative width, each probe started
+// from the highest blocked configuration (0 keeps the ramp alone), with
+// --fc_probes forces probed at once per round (1 bisects).
 #include <cstdint>
 #include <cstdio>
 #include <exception>
@@ -35,6 +36,7 @@ std::vector<cli::OptionSpec> option_table() {
         {"f_step", Kind::Real, "1", "force increment of the ramp", ""},
         {"f_max", Kind::Real, "40", "largest force of the ramp", ""},
         {"fc_rel_tol", Kind::Real, "0", "bisect f_c to this relative width; 0: ramp only", ""},
+        {"fc_probes", Kind::Integer, "1", "forces probed per round of that search (1: bisection)", ""},
         {"seed", Kind::Seed, "1", "disorder seed", ""},
         {"output", Kind::Word, "", "output file (.kv: text archive) [depinning_<model>.h5]", ""},
         {"dstep", Kind::Real, "", "march step of a site [Ly / (2 ny)]", ""},
@@ -59,6 +61,8 @@ DepinningConfig config_from(const cli::Options &o) {
     c.f_step = o.real("f_step");
     c.f_max = o.real("f_max");
     c.fc_rel_tol = o.real("fc_rel_tol");
+    c.fc_probes = o.integer("fc_probes");
+    if (c.fc_probes < 1) throw std::runtime_error("--fc_probes is at least 1");
     // knobs whose defaults scale with the disorder cell
     const double cell = c.Ly / c.ny;
     c.rp.dstep = o.given("dstep") ? o.real("dstep") : 0.5 * cell;

@@ -113,6 +113,12 @@ __global__ __launch_bounds__(kTile) void eval64_kernel(qew_host::EvalArgs a, Lin
             s_dvdy[tid] = dvdy;
             s_h[tid] = hi;
             if (arc) s_slope[tid] = dh / sqrt(1.0 + dh * dh);
+            // the per-edge quantities of the reference's GradientWorkspace,
+            // edge i written by the thread that owns site i
+            if (owns && a.edge_dvdy) {
+                a.edge_dvdy[L.off + i] = dvdy;
+                if (arc && a.edge_slope) a.edge_slope[L.off + i] = s_slope[tid];
+            }
         }
     }
     if (GRAD) {
@@ -222,6 +228,19 @@ __global__ void sample_kernel(NoiseDesc nd, int m, const double *x, const double
     v[k] = a;
     dvdx[k] = b;
     dvdy[k] = c;
+}
+
+// DeviceNoise::sample_value (what = 0) or sample_dy (what = 1) at m points
+// (src/filtered_noise.h:138-176): the value-only and dy-only samplers the
+// objective and the gradient use.
+__global__ void sample_part_kernel(NoiseDesc nd, int what, int m, const double *x,
+                                   const double *y, double *out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    double r;
+    if (nd.dtype == 1) r = what ? sample_dy64<float>(nd, x[k], y[k]) : sample_v64<float>(nd, x[k], y[k]);
+    else r = what ? sample_dy64<double>(nd, x[k], y[k]) : sample_v64<double>(nd, x[k], y[k]);
+    out[k] = r;
 }
 
 }  // namespace
@@ -380,6 +399,26 @@ qew_status qew_gradient(qew_ctx ctx, qew_noise noise, const qew_params *p, int n
     });
 }
 
+qew_status qew_gradient_edges(qew_ctx ctx, qew_noise noise, const qew_params *p, int n,
+                              const double *h, double *grad, double *edge_dvdy,
+                              double *edge_slope) {
+    return guard([&] {
+        check_line(ctx, noise, p, n);
+        QEW_REQUIRE(h && grad && edge_dvdy, "null pointer");
+        use_device(ctx);
+        const LineDesc l = single_line(n, p);
+        ctx_reserve_eval(ctx, l.ntiles, 1);
+        EvalArgs a{};
+        a.ntiles = l.ntiles;
+        a.h = h;
+        a.grad = grad;
+        a.partial = ctx->partial.ptr;
+        a.edge_dvdy = edge_dvdy;
+        a.edge_slope = edge_slope;
+        launch_eval_f64(ctx->stream, a, &l, &noise->desc);
+    });
+}
+
 qew_status qew_eval(qew_ctx ctx, qew_noise noise, const qew_params *p, int n,
                     const double *h, double *grad, double *energy) {
     return guard([&] {
@@ -478,6 +517,32 @@ qew_status qew_noise_sample(qew_ctx ctx, qew_noise noise, int m, const double *x
                            noise->desc, m, x, y, v, dvdx, dvdy);
         QEW_HIP(hipGetLastError());
     });
+}
+
+static qew_status sample_part(qew_ctx ctx, qew_noise noise, int what, int m, const double *x,
+                              const double *y, double *out) {
+    return guard([&] {
+        ctx_check(ctx);
+        QEW_REQUIRE(noise != nullptr, "null noise");
+        QEW_REQUIRE(m >= 0, "negative point count");
+        if (m == 0) return;
+        QEW_REQUIRE(x && y && out, "null pointer");
+        require_wide_table(noise->desc);
+        use_device(ctx);
+        hipLaunchKernelGGL(sample_part_kernel, dim3((m + 255) / 256), dim3(256), 0, ctx->stream,
+                           noise->desc, what, m, x, y, out);
+        QEW_HIP(hipGetLastError());
+    });
+}
+
+qew_status qew_noise_sample_value(qew_ctx ctx, qew_noise noise, int m, const double *x,
+                                  const double *y, double *v) {
+    return sample_part(ctx, noise, 0, m, x, y, v);
+}
+
+qew_status qew_noise_sample_dy(qew_ctx ctx, qew_noise noise, int m, const double *x,
+                               const double *y, double *dvdy) {
+    return sample_part(ctx, noise, 1, m, x, y, dvdy);
 }
 
 }  // extern "C"

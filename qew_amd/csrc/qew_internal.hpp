@@ -229,6 +229,10 @@ struct EvalArgs {
     void *grad;         // null: energy only
     double *energy;     // device, per line; null: gradient only
     void *partial;
+    // FP64 single-line gradient only: per-edge dV/dy and (arclength) slope
+    // written out when non-null
+    double *edge_dvdy;
+    double *edge_slope;
 };
 
 void launch_eval_f64(hipStream_t s, const EvalArgs &a, const qew_dev::LineDesc *one_line,

@@ -54,6 +54,7 @@ struct RkArgs {
     double *h;
     unsigned char *moved;  // per site: moved in its last colour pass
     double *partial;       // [line][tile][colour] advance sums of the current sweep
+    int *pmoved;           // [line][tile][colour] sites moved in the current sweep
     int tiles;             // workgroups per colour per line
     RkState *state;
     qew_rk_params rp;
@@ -373,10 +374,12 @@ __global__ __launch_bounds__(kRkBlock, 3) void rk_color_kernel(RkArgs a, int col
         ip = (i + 1 == n) ? 0 : i + 1;
         dirty = (mv[i] | mv[im] | mv[ip]) != 0;
     }
-    double *part = a.partial + ((size_t)line * a.tiles + blockIdx.x) * 2 + color;
+    const size_t pidx = ((size_t)line * a.tiles + blockIdx.x) * 2 + color;
+    double *part = a.partial + pidx;
     if (!__syncthreads_or(dirty)) {
         if (tid == 0) {
             *part = 0.0;
+            a.pmoved[pidx] = 0;
             const int cnt = std::min(kRkBlock, std::max(0, half - (int)blockIdx.x * kRkBlock));
             if (cnt) atomicAdd(&a.state[line].vel_evals, (unsigned long long)cnt);
         }
@@ -416,15 +419,20 @@ __global__ __launch_bounds__(kRkBlock, 3) void rk_color_kernel(RkArgs a, int col
     }
     __syncthreads();
     double adv = 0;
+    bool moved = false;
     if (dirty) {
         const double hn = s_new[tid];
         adv = hn - hi;
-        const bool moved = hn != hi;
+        moved = hn != hi;
         if (moved) h[i] = hn;
         a.moved[L.off + i] = moved ? 1 : 0;
     }
     const double total = block_sum(adv, red);
-    if (tid == 0) *part = total;
+    const int nmoved = __syncthreads_count(moved);
+    if (tid == 0) {
+        *part = total;
+        a.pmoved[pidx] = nmoved;
+    }
     // every lane of the wave reaches the reduction (no early return above)
     const unsigned ev = wave_sum(evals);
     if ((tid & (kWave - 1)) == 0 && ev) atomicAdd(&a.state[line].vel_evals, (unsigned long long)ev);
@@ -434,6 +442,8 @@ __global__ __launch_bounds__(kRkBlock, 3) void rk_color_kernel(RkArgs a, int col
 
 // One workgroup per line: the sweep's advance (per-workgroup sums of both
 // colours, added in a fixed order) and the status decision.
+// running[0] counts the lines that keep sweeping, running[1] the sites they
+// moved in this sweep.
 __global__ __launch_bounds__(kRkBlock) void rk_status_kernel(RkArgs a, unsigned *running) {
     __shared__ double red[kRkBlock / kWave];
     const int line = blockIdx.x;
@@ -441,9 +451,14 @@ __global__ __launch_bounds__(kRkBlock) void rk_status_kernel(RkArgs a, unsigned 
     if (!st.active) return;
     const LineDesc L = a.lines[line];
     const double *part = a.partial + (size_t)line * a.tiles * 2;
-    double acc = 0;
-    for (int t = threadIdx.x; t < 2 * a.tiles; t += blockDim.x) acc += part[t];
+    const int *pm = a.pmoved + (size_t)line * a.tiles * 2;
+    double acc = 0, mc = 0;
+    for (int t = threadIdx.x; t < 2 * a.tiles; t += blockDim.x) {
+        acc += part[t];
+        mc += (double)pm[t];
+    }
     const double total = block_sum(acc, red);
+    const double nmoved = block_sum(mc, red);
     if (threadIdx.x != 0) return;
     const bool go = sweep_status(st, total, L.n, a.rp);
     // vel_evals is added to by the colour kernels: leave it out of the store
@@ -454,7 +469,10 @@ __global__ __launch_bounds__(kRkBlock) void rk_status_kernel(RkArgs a, unsigned 
     a.state[line].cum_adv = st.cum_adv;
     a.state[line].total_adv = st.total_adv;
     a.state[line].mean_h = st.mean_h;
-    if (go) atomicAdd(running, 1u);
+    if (go) {
+        atomicAdd(running, 1u);
+        atomicAdd(running + 1, (unsigned)nmoved);
+    }
 }
 
 __global__ __launch_bounds__(kRkBlock) void rk_init_kernel(RkArgs a) {
@@ -486,8 +504,12 @@ __global__ __launch_bounds__(kRkBlock) void rk_init_kernel(RkArgs a) {
 constexpr int kRkLineBlock = 512;
 constexpr int kRkLdsMaxN = 16384;
 
+// With `cont`, the kernel continues rk_block calls that the per-colour
+// launches began: the line state, moved flags and RkState (sweeps, advance,
+// counters) are taken from global memory, lines that already stopped are
+// left alone, and the sweeps go on exactly as the launches would run them.
 template <typename C>
-__global__ __launch_bounds__(kRkLineBlock) void rk_line_kernel(RkArgs a, int lds_n) {
+__global__ __launch_bounds__(kRkLineBlock) void rk_line_kernel(RkArgs a, int lds_n, int cont) {
     extern __shared__ double s_h[];
     unsigned char *s_mv = reinterpret_cast<unsigned char *>(s_h + lds_n);
     __shared__ int s_list[kRkLineBlock];
@@ -502,17 +524,27 @@ __global__ __launch_bounds__(kRkLineBlock) void rk_line_kernel(RkArgs a, int lds
     const double dx = L.L / (double)n;
     const int tid = threadIdx.x;
     double *hg = a.h + L.off;
-    double acc = 0;
-    for (int i = tid; i < n; i += kRkLineBlock) {
-        const double v = hg[i];
-        s_h[i] = v;
-        s_mv[i] = 1;
-        acc += v;
-    }
     RkState st{};
-    st.mean_h0 = block_sum(acc, red) / (double)n;
+    if (cont) {
+        st = a.state[line];
+        if (!st.active) return;  // uniform: every thread read the same flag
+        for (int i = tid; i < n; i += kRkLineBlock) {
+            s_h[i] = hg[i];
+            s_mv[i] = a.moved[L.off + i];
+        }
+        __syncthreads();
+    } else {
+        double acc = 0;
+        for (int i = tid; i < n; i += kRkLineBlock) {
+            const double v = hg[i];
+            s_h[i] = v;
+            s_mv[i] = 1;
+            acc += v;
+        }
+        st.mean_h0 = block_sum(acc, red) / (double)n;
+    }
     unsigned long long evals = 0, done = 0;
-    bool go = a.rp.max_sweeps > 0;
+    bool go = cont ? true : a.rp.max_sweeps > 0;
     while (go) {
         double my_adv = 0;
         for (int color = 0; color < 2; ++color) {
@@ -607,21 +639,78 @@ __global__ void rk_velocity_kernel(NoiseDesc nd, int n, double dx, double lt, do
         v[k] = site_velocity<double>(idx[k], n, dx, lt, f, model, hi[k], hl[k], hr[k], nd);
 }
 
+// rk_forward_zero (src/rosso_krauth.h:88-124) at m independent sites, one
+// thread each, in the reference's sequential order: the velocity at hi; if
+// positive, march b = a + dstep (a = b, advanced += dstep) until v(b) <= 0 or
+// advanced reaches max_advance, then at most 60 bisections while the bracket
+// is wider than root_tol. The march length is bounded on the host.
+template <typename C>
+__global__ void rk_forward_zero_kernel(NoiseDesc nd, int n, double dx, double lt, double f,
+                                       int model, qew_rk_params rp, int m, const int *idx,
+                                       const double *hi, const double *hl, const double *hr,
+                                       double *out, int *found, int64_t *evals) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const int i = idx[k];
+    const double h0 = hi[k], l = hl[k], r = hr[k];
+    int64_t ev = 1;
+    int fd = 1;
+    double res = h0;
+    if (site_velocity<C>(i, n, dx, lt, f, model, h0, l, r, nd) > 0) {
+        double a = h0, advanced = 0;
+        fd = 0;
+        res = a;
+        while (advanced < rp.max_advance) {
+            const double b = a + rp.dstep;
+            ++ev;
+            if (site_velocity<C>(i, n, dx, lt, f, model, b, l, r, nd) <= 0) {
+                double lo = a, hc = b;
+                for (int t = 0; t < 60 && (hc - lo) > rp.root_tol; ++t) {
+                    const double mid = 0.5 * (lo + hc);
+                    ++ev;
+                    if (site_velocity<C>(i, n, dx, lt, f, model, mid, l, r, nd) > 0) lo = mid;
+                    else hc = mid;
+                }
+                fd = 1;
+                res = 0.5 * (lo + hc);
+                break;
+            }
+            a = b;
+            advanced += rp.dstep;
+            res = a;
+        }
+    }
+    out[k] = res;
+    if (found) found[k] = fd;
+    if (evals) evals[k] = ev;
+}
+
+// A march of max_advance / dstep steps must end: dstep positive, and few
+// enough steps that `advanced += dstep` keeps growing (far fewer than 2^52).
+void check_march(const qew_rk_params &rp) {
+    QEW_REQUIRE(rp.dstep > 0 || !(rp.max_advance > 0),
+                "rk_block: dstep must be positive when max_advance > 0");
+    QEW_REQUIRE(!(rp.max_advance > 0) || rp.max_advance / rp.dstep <= (double)(1 << 26),
+                "rk_block: max_advance / dstep exceeds 2^26 march steps per site");
+}
+
 // Device scratch of run_rk for nlines lines of total length `total`.
 struct RkScratch {
     RkState *state;
-    unsigned *running;
+    unsigned *running;  // [0] lines still sweeping, [1] sites they moved
     unsigned char *moved;
     double *partial;
+    int *pmoved;
 };
 
 size_t rk_scratch_bytes(int nlines, int64_t total, int tiles) {
     auto up = [](size_t b) { return (b + 255) / 256 * 256; };
     return up(sizeof(RkState) * nlines) + 256 + up((size_t)total) +
-           up(sizeof(double) * 2 * (size_t)tiles * nlines);
+           up(sizeof(double) * 2 * (size_t)tiles * nlines) +
+           up(sizeof(int) * 2 * (size_t)tiles * nlines);
 }
 
-RkScratch rk_scratch(char *w, int nlines, int64_t total) {
+RkScratch rk_scratch(char *w, int nlines, int64_t total, int tiles) {
     auto up = [](size_t b) { return (b + 255) / 256 * 256; };
     RkScratch s;
     s.state = reinterpret_cast<RkState *>(w);
@@ -631,6 +720,8 @@ RkScratch rk_scratch(char *w, int nlines, int64_t total) {
     s.moved = reinterpret_cast<unsigned char *>(w);
     w += up((size_t)total);
     s.partial = reinterpret_cast<double *>(w);
+    w += up(sizeof(double) * 2 * (size_t)tiles * nlines);
+    s.pmoved = reinterpret_cast<int *>(w);
     return s;
 }
 
@@ -646,47 +737,58 @@ bool line_kernel_fits(qew_ctx ctx, int max_n) {
     return max_n <= kRkLdsMaxN && need <= (size_t)lds_max;
 }
 
-// QEW_RK_KERNEL=line|launch forces a path (tests, tuning): 1 line, 2 launch,
-// 0 not set.
+// QEW_RK_KERNEL=line|launch|hybrid forces a path (tests, tuning): 1 line,
+// 2 launch, 3 hybrid switching at its first poll, 0 not set.
 int rk_env_path() {
     const char *env = std::getenv("QEW_RK_KERNEL");
     if (env && std::strcmp(env, "line") == 0) return 1;
     if (env && std::strcmp(env, "launch") == 0) return 2;
+    if (env && std::strcmp(env, "hybrid") == 0) return 3;
     return 0;
 }
 
+// How run_rk sweeps: all in the LDS line kernel, all with per-colour
+// launches, or per-colour launches until few sites move and then the line
+// kernel for the rest of the call (switch_per_line: the switch happens at a
+// poll where the lines still sweeping moved at most this many sites per line
+// in their last sweep; a negative value switches at the first poll).
+enum class RkMode { Line, Launch, Hybrid };
+
+// Sites moved per still-sweeping line below which the line kernel takes over:
+// a sweep of the line kernel scans its line in LDS (32 chunk barriers for
+// 16384 sites) and resolves the moving sites with lane groups, while every
+// sweep of the launches costs three launches over all lines' tiles.
+constexpr double kRkSwitchMovedPerLine = 128.0;
+
 // Runs the sweeps for `nlines` lines whose descriptors are on the device, on
 // the LDS path (`line`) or with per-colour launches.
+void launch_line_kernel(hipStream_t s, const RkArgs &a, int nlines, int max_n, int table_dtype,
+                        int cont) {
+    const size_t lds = (sizeof(double) + 1) * (size_t)max_n;
+    auto go = [&](auto kernel) {
+        QEW_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(kernel, dim3(nlines), dim3(kRkLineBlock), lds, s, a, max_n, cont);
+    };
+    if (table_dtype == QEW_F32_MID) go(rk_line_kernel<MidF32>);
+    else if (table_dtype == 1) go(rk_line_kernel<float>);
+    else go(rk_line_kernel<double>);
+    QEW_HIP(hipGetLastError());
+}
+
 void run_rk(qew_ctx ctx, const LineDesc *d_lines, const NoiseDesc *d_noises, int nlines, int max_n,
             int table_dtype, const qew_rk_params &rp, double *h, const RkScratch &w,
-            unsigned *running_host, std::vector<RkState> &out, bool line) {
+            unsigned *running_host, std::vector<RkState> &out, RkMode mode,
+            double switch_per_line = kRkSwitchMovedPerLine) {
     // a march step of zero (or NaN) with a positive advance cap never ends;
     // the reference would spin forever, the device must not
-    QEW_REQUIRE(rp.dstep > 0 || !(rp.max_advance > 0),
-                "rk_block: dstep must be positive when max_advance > 0");
+    check_march(rp);
     hipStream_t s = ctx->stream;
     const int tiles = rk_tiles(max_n);
-    RkArgs a{d_lines, d_noises, h, w.moved, w.partial, tiles, w.state, rp};
-    if (line) {
-        const size_t lds = (sizeof(double) + 1) * (size_t)max_n;
+    RkArgs a{d_lines, d_noises, h, w.moved, w.partial, w.pmoved, tiles, w.state, rp};
+    if (mode == RkMode::Line) {
         QEW_HIP(hipMemsetAsync(w.state, 0, sizeof(RkState) * (size_t)nlines, s));
-        if (table_dtype == QEW_F32_MID) {
-            QEW_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(rk_line_kernel<MidF32>),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            hipLaunchKernelGGL(rk_line_kernel<MidF32>, dim3(nlines), dim3(kRkLineBlock), lds, s,
-                               a, max_n);
-        } else if (table_dtype == 1) {
-            QEW_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(rk_line_kernel<float>),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            hipLaunchKernelGGL(rk_line_kernel<float>, dim3(nlines), dim3(kRkLineBlock), lds, s, a,
-                               max_n);
-        } else {
-            QEW_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(rk_line_kernel<double>),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            hipLaunchKernelGGL(rk_line_kernel<double>, dim3(nlines), dim3(kRkLineBlock), lds, s, a,
-                               max_n);
-        }
-        QEW_HIP(hipGetLastError());
+        launch_line_kernel(s, a, nlines, max_n, table_dtype, 0);
     } else {
         hipLaunchKernelGGL(rk_init_kernel, dim3(nlines), dim3(kRkBlock), 0, s, a);
         QEW_HIP(hipGetLastError());
@@ -708,16 +810,24 @@ void run_rk(qew_ctx ctx, const LineDesc *d_lines, const NoiseDesc *d_noises, int
                         hipLaunchKernelGGL(rk_color_kernel<double>, grid, dim3(kRkBlock), 0, s, a,
                                            color);
                 }
-                if (k == todo - 1) QEW_HIP(hipMemsetAsync(w.running, 0, sizeof(unsigned), s));
+                if (k == todo - 1) QEW_HIP(hipMemsetAsync(w.running, 0, 2 * sizeof(unsigned), s));
                 hipLaunchKernelGGL(rk_status_kernel, dim3(nlines), dim3(kRkBlock), 0, s, a,
                                    w.running);
             }
             QEW_HIP(hipGetLastError());
             sweep += todo;
-            QEW_HIP(hipMemcpyAsync(running_host, w.running, sizeof(unsigned), hipMemcpyDeviceToHost,
-                                   s));
+            QEW_HIP(hipMemcpyAsync(running_host, w.running, 2 * sizeof(unsigned),
+                                   hipMemcpyDeviceToHost, s));
             QEW_HIP(hipStreamSynchronize(s));
-            if (*running_host == 0) break;
+            if (running_host[0] == 0) break;
+            if (mode == RkMode::Hybrid &&
+                (switch_per_line < 0 ||
+                 (double)running_host[1] <= switch_per_line * (double)running_host[0])) {
+                // the lines still sweeping finish in the LDS kernel, from the
+                // state and moved flags the launches left
+                launch_line_kernel(s, a, nlines, max_n, table_dtype, 1);
+                break;
+            }
             poll = std::min(poll * 2, 256);
         }
     }
@@ -767,9 +877,13 @@ qew_status qew_rk_block(qew_ctx ctx, qew_noise noise, const qew_params *p,
         std::vector<RkState> out;
         const int dt = noise->desc.kind == TABLE ? noise->desc.dtype : 0;
         // one line: a single workgroup leaves the device idle unless forced
-        const bool line = rk_env_path() == 1 && line_kernel_fits(ctx, n);
+        const int env = rk_env_path();
+        const bool fits = line_kernel_fits(ctx, n);
+        const RkMode mode = (env == 1 && fits) ? RkMode::Line
+                            : (env == 3 && fits) ? RkMode::Hybrid : RkMode::Launch;
         run_rk(ctx, reinterpret_cast<LineDesc *>(w), reinterpret_cast<NoiseDesc *>(w + off_noise), 1, n,
-               dt, *rp, h, rk_scratch(w + off_rk, 1, n), ctx->pinned.as<unsigned>(), out, line);
+               dt, *rp, h, rk_scratch(w + off_rk, 1, n, rk_tiles(n)), ctx->pinned.as<unsigned>(), out,
+               mode, env == 3 ? -1.0 : kRkSwitchMovedPerLine);
         *result = to_result(out[0]);
     });
 }
@@ -797,20 +911,27 @@ qew_status qew_batch_rk_block(qew_batch b, const qew_rk_params *rp, double *h,
         ctx->pinned.ensure(sizeof(double) * 64);
         std::vector<RkState> out;
         const int dt = b->noises[0].kind == TABLE ? b->noises[0].dtype : 0;
-        const RkScratch sc = rk_scratch(b->d_scratch.as<char>(), b->nlines, b->total);
-        auto run = [&](double *hh, const qew_rk_params &p, bool line) {
+        const RkScratch sc =
+            rk_scratch(b->d_scratch.as<char>(), b->nlines, b->total, rk_tiles(b->max_n));
+        auto run = [&](double *hh, const qew_rk_params &p, RkMode mode, double sw) {
             run_rk(ctx, b->d_lines.as<LineDesc>(), b->d_noises.as<NoiseDesc>(), b->nlines, b->max_n,
-                   dt, p, hh, sc, ctx->pinned.as<unsigned>(), out, line);
+                   dt, p, hh, sc, ctx->pinned.as<unsigned>(), out, mode, sw);
         };
         // Path: forced by QEW_RK_KERNEL, else per-colour launches when the
         // lines do not fit in LDS, else the faster of the two timed on the
-        // batch's first call (16 sweeps each on a copy of the line state; both
-        // paths give identical results) and kept for later calls.
-        bool line = false;
+        // batch's first call (16 sweeps each on a copy of the line state; all
+        // paths give identical results) and kept for later calls; when the
+        // launches win, the call still hands its lines to the LDS kernel once
+        // few of their sites move (hybrid).
+        RkMode mode = RkMode::Launch;
+        double sw = kRkSwitchMovedPerLine;
         const int env = rk_env_path();
+        const bool fits = line_kernel_fits(ctx, b->max_n);
         if (env) {
-            line = env == 1 && line_kernel_fits(ctx, b->max_n);
-        } else if (line_kernel_fits(ctx, b->max_n)) {
+            mode = (env == 1 && fits) ? RkMode::Line : (env == 3 && fits) ? RkMode::Hybrid
+                                                                         : RkMode::Launch;
+            if (env == 3) sw = -1.0;
+        } else if (fits) {
             if (b->rk_path == 0) {
                 qew_rk_params trial = *rp;
                 trial.max_sweeps = std::min(rp->max_sweeps, 16);
@@ -822,16 +943,41 @@ qew_status qew_batch_rk_block(qew_batch b, const qew_rk_params *rp, double *h,
                     QEW_HIP(hipMemcpyAsync(copy, h, bytes, hipMemcpyDeviceToDevice, ctx->stream));
                     QEW_HIP(hipStreamSynchronize(ctx->stream));
                     const auto t0 = std::chrono::steady_clock::now();
-                    run(copy, trial, k == 0);
+                    run(copy, trial, k == 0 ? RkMode::Line : RkMode::Launch, sw);
                     t_path[k] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0)
                                     .count();
                 }
                 b->rk_path = t_path[0] < t_path[1] ? 1 : 2;
             }
-            line = b->rk_path == 1;
+            mode = b->rk_path == 1 ? RkMode::Line : RkMode::Hybrid;
         }
-        run(h, *rp, line);
+        run(h, *rp, mode, sw);
         for (int l = 0; l < b->nlines; ++l) results[l] = to_result(out[l]);
+    });
+}
+
+qew_status qew_rk_forward_zero(qew_ctx ctx, qew_noise noise, const qew_params *p,
+                               const qew_rk_params *rp, int n, int m, const int *i,
+                               const double *hi, const double *hl, const double *hr, double *out,
+                               int *found, int64_t *evals) {
+    return guard([&] {
+        ctx_check(ctx);
+        QEW_REQUIRE(noise && p && rp, "null pointer");
+        QEW_REQUIRE(n >= 1 && m >= 0, "bad sizes");
+        check_march(*rp);
+        if (m == 0) return;
+        QEW_REQUIRE(i && hi && hl && hr && out, "null pointer");
+        require_wide_table(noise->desc);
+        use_device(ctx);
+        const double dx = p->physical_size / (double)n;
+        auto launch = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, dim3((m + 127) / 128), dim3(128), 0, ctx->stream,
+                               noise->desc, n, dx, p->line_tension, p->driving_force, p->model,
+                               *rp, m, i, hi, hl, hr, out, found, evals);
+        };
+        if (noise->desc.kind == TABLE && noise->desc.dtype == 1) launch(rk_forward_zero_kernel<float>);
+        else launch(rk_forward_zero_kernel<double>);
+        QEW_HIP(hipGetLastError());
     });
 }
 

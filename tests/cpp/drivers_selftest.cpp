@@ -218,6 +218,14 @@ static void depinning_ramp_brackets_critical_force() {
     CHECK(rb.f_c_upper - rb.f_c_lower <= 1e-6 * rb.f_c_upper * 1.0000001);
     CHECK(std::abs(rb.f_c - fc_exact) < 0.02 * fc_exact);
     CHECK(rb.h_critical.size() == 8u);
+
+    // four forces per round: the same critical force, fewer rounds
+    cfg.fc_probes = 4;
+    const DepinningResult r4 = depinning_ramp(cfg, noise);
+    CHECK(r4.f_c_upper - r4.f_c_lower <= 1e-6 * r4.f_c_upper * 1.0000001);
+    CHECK(r4.f_c_lower <= rb.f_c_upper && rb.f_c_lower <= r4.f_c_upper);  // brackets overlap
+    CHECK(r4.bisection_probes % 4 == 0 && r4.bisection_probes / 4 < rb.bisection_probes);
+    CHECK(r4.h_critical.size() == 8u);
 }
 
 static void facade_semantics() {

@@ -93,6 +93,7 @@ def load(omp=False):
     lib.orc_objective_trial.argtypes = [C.c_int, _DP, _DP, C.c_double, P(Params), P(Noise)]
     lib.orc_objective_trial.restype = C.c_double
     lib.orc_gradient.argtypes = [C.c_int, _DP, P(Params), P(Noise), _DP]
+    lib.orc_gradient_edges.argtypes = [C.c_int, _DP, P(Params), P(Noise), _DP, _DP, _DP]
     lib.orc_fft.argtypes = [C.c_int, _DP, C.c_int]
     lib.orc_fft2.argtypes = [C.c_int, C.c_int, _DP, C.c_int]
     lib.orc_prefilter.argtypes = [C.c_int, C.c_int, _DP, _DP]
@@ -110,9 +111,18 @@ def load(omp=False):
     lib.orc_rk_site_velocity.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                          C.c_int, C.c_double, C.c_double, C.c_double, P(Noise)]
     lib.orc_rk_site_velocity.restype = C.c_double
+    lib.orc_rk_forward_zero.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                        C.c_int, C.c_double, C.c_double, C.c_double, P(Noise),
+                                        P(RkParams), P(C.c_int), P(C.c_long)]
+    lib.orc_rk_forward_zero.restype = C.c_double
     lib.orc_rk_block.argtypes = [C.c_int, _DP, P(Params), P(Noise), P(RkParams), P(RkResult)]
     lib.orc_rk_block.restype = C.c_int
+    lib.orc_langevin_run.argtypes = [C.c_int, _DP, P(Params), P(Noise), C.c_int, _DP, _DP,
+                                     C.c_uint64, C.c_uint64]
+    lib.orc_lbfgs_batch.argtypes = [C.c_int, P(C.c_int64), _DP, P(Params), P(Noise),
+                                    P(LbfgsParams), _DP, P(C.c_int), P(LbfgsResult)]
     lib.orc_num_threads.restype = C.c_int
+    lib.orc_set_num_threads.argtypes = [C.c_int]
     _libs[name] = lib
     return lib
 
@@ -192,6 +202,16 @@ def gradient(h, p, noise, omp=False):
     return g
 
 
+def gradient_edges(h, p, noise):
+    """(grad, edge dV/dy, edge slope or None) of the reference's two passes."""
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    g, e = np.empty_like(h), np.empty_like(h)
+    s = np.empty_like(h) if p.model == 1 else None
+    load().orc_gradient_edges(len(h), dp(h), C.byref(p), noise.ref, dp(g), dp(e),
+                              dp(s) if s is not None else _DP())
+    return g, e, s
+
+
 def fft(x, sign=-1):
     x = np.asarray(x, dtype=np.complex128)
     buf = np.ascontiguousarray(np.stack([x.real, x.imag], axis=-1).reshape(-1))
@@ -268,9 +288,99 @@ def rk_site_velocity(i, n, dx, lt, f, model, hi, hl, hr, noise):
     return load().orc_rk_site_velocity(i, n, dx, lt, f, model, hi, hl, hr, noise.ref)
 
 
+def rk_forward_zero(i, n, dx, lt, f, model, hi, hl, hr, noise, rp):
+    """(height, found, evaluations) of the oracle's rk_forward_zero."""
+    found = C.c_int()
+    evals = C.c_long()
+    h = load().orc_rk_forward_zero(i, n, dx, lt, f, model, hi, hl, hr, noise.ref, C.byref(rp),
+                                   C.byref(found), C.byref(evals))
+    return h, found.value, evals.value
+
+
 def rk_block(h, p, noise, rp, omp=False):
     r = RkResult()
     rc = load(omp).orc_rk_block(len(h), dp(h), C.byref(p), noise.ref, C.byref(rp), C.byref(r))
     if rc != 0:
         raise ValueError("rk_block: line length must be even for the red-black sweep")
     return r
+
+
+def langevin_run(h, p, noise, a, b, seed, step0=0, omp=False):
+    """len(a) Langevin steps of h in place with per-step coefficients a[k], b[k]."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    assert len(a) == len(b)
+    load(omp).orc_langevin_run(len(h), dp(h), C.byref(p), noise.ref, len(a), dp(a), dp(b),
+                               seed, step0)
+
+
+def lbfgs_batch(lines, params, noises, lp, shifts=None, longest_first=True, omp=True):
+    """orc_lbfgs on every line (a list of arrays), lines spread over OpenMP
+    threads; returns (minimised lines, results)."""
+    n = len(lines)
+    off = np.concatenate([[0], np.cumsum([len(x) for x in lines])]).astype(np.int64)
+    h = np.ascontiguousarray(np.concatenate(lines), dtype=np.float64)
+    P = (Params * n)(*params)
+    N = (Noise * n)(*[nz.s for nz in noises])
+    R = (LbfgsResult * n)()
+    sh = None if shifts is None else np.ascontiguousarray(shifts, dtype=np.float64)
+    order = None
+    if longest_first:
+        order = np.ascontiguousarray(np.argsort([-len(x) for x in lines], kind="stable"),
+                                     dtype=np.int32)
+    load(omp).orc_lbfgs_batch(n, off.ctypes.data_as(C.POINTER(C.c_int64)), dp(h), P, N,
+                              C.byref(lp), _DP() if sh is None else dp(sh),
+                              C.POINTER(C.c_int)() if order is None else
+                              order.ctypes.data_as(C.POINTER(C.c_int)), R)
+    return [h[off[k]:off[k + 1]].copy() for k in range(n)], list(R)
+
+
+def fc_search(tables, n, y0, rp, L=None, lt=1.0, model=LINEAR, f_start=0.125, k_way=4,
+              rel_tol=1e-10, workers=16):
+    """The K-way depinning-threshold search (relaxation at f = 0, ramp
+    f_start * 2^k until not blocked, then k_way forces per round between the
+    bracket ends, each started from the highest blocked configuration; a
+    timeout counts as not blocked, src/qew_runs.h:224-241 and
+    tests/test_rosso_krauth.cpp:76-88) on the sequential oracle's rk_block.
+    The lines, and the probes of a round, run on a thread pool (each probe is
+    an independent sequential call, so the results do not depend on the
+    scheduling). Returns [(lo, hi, h_lo, rounds)] per line."""
+    from concurrent.futures import ThreadPoolExecutor
+    L = float(n if L is None else L)
+    pool = ThreadPoolExecutor(max_workers=workers)
+
+    def probe(h0, f, nz):
+        h = h0.copy()
+        return rk_block(h, params(L, lt, f, model), nz, rp).blocked, h
+
+    def one(nz):
+        h_lo = np.full(n, float(y0))
+        assert rk_block(h_lo, params(L, lt, 0.0, model), nz, rp).blocked, \
+            "a line does not block at f = 0"
+        lo, hi, f = 0.0, np.inf, f_start
+        while np.isinf(hi):
+            f_probe = max(2 * lo, f)
+            blocked, h = probe(h_lo, f_probe, nz)
+            if blocked:
+                lo, h_lo = f_probe, h
+            else:
+                hi = f_probe
+            f *= 2
+        rounds = 0
+        while hi - lo > rel_tol * max(abs(lo), abs(hi)):
+            fs = [lo + j * (hi - lo) / (k_way + 1) for j in range(1, k_way + 1)]
+            got = list(pool.map(lambda fj: probe(h_lo, fj, nz), fs))
+            new_lo, new_h, new_hi = lo, h_lo, hi
+            for fj, (blocked, h) in zip(fs, got):
+                if not blocked:
+                    new_hi = fj
+                    break
+                new_lo, new_h = fj, h
+            lo, h_lo, hi = new_lo, new_h, new_hi
+            rounds += 1
+        return lo, hi, h_lo, rounds
+
+    with ThreadPoolExecutor(max_workers=len(tables)) as lines:
+        out = list(lines.map(one, tables))
+    pool.shutdown()
+    return out

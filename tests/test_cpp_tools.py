@@ -55,6 +55,16 @@ def test_facade_and_drivers_selftest(tmp_path):
 
 
 @pytest.mark.gpu
+def test_facade_per_point_api_selftest(tmp_path):
+    """tests/cpp/facade_selftest.cpp: DeviceNoise::sample / sample_value /
+    sample_dy, rk_site_velocity, rk_forward_zero and GradientWorkspace through
+    the C++ facade (the reference's sampler and site-velocity tests)."""
+    exe = compile_cpp("facade_selftest.cpp", str(tmp_path / "facade_selftest"), link=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+@pytest.mark.gpu
 def test_tools_write_reference_keys(tmp_path):
     from qew_amd.io import read_kv, read_output
     # no --output: the reference's default name, static_<model>.h5, in HDF5

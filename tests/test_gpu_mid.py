@@ -212,7 +212,7 @@ def test_f64_mid_batch_lbfgs_matches_oracle(q, precondition):
         b.lbfgs(dev(np.concatenate(h0)), q.LbfgsParams(m=4), dtype=q.F32)
 
 
-@pytest.mark.parametrize("path", ["line", "launch"])
+@pytest.mark.parametrize("path", ["line", "launch", "hybrid"])
 def test_mid_batch_rk_bitwise_vs_oracle(q, monkeypatch, path):
     """Rosso-Krauth sweeps on a midpoint table (FP64 arithmetic on the FP32
     values, as on FP32 tables) are bit for bit the oracle's on the same

@@ -330,13 +330,15 @@ def test_fc_bisection_to_1e10_matches_oracle(q):
     np.testing.assert_array_equal(hc_g, hc_o)
 
 
-@pytest.mark.parametrize("path", ["line", "launch"])
+@pytest.mark.parametrize("path", ["line", "launch", "hybrid"])
 @pytest.mark.parametrize("n,lt,f,max_sweeps", [
     (16384, 0.005, 0.1, 3000),   # largest line the LDS path holds; runs to blocked
     (16384, 0.02, 0.2, 3000),
     (32768, 0.02, 0.2, 300),     # beyond LDS (per-colour launches), stops at max_sweeps
 ])
 def test_rk_long_lines_match_oracle_bitwise(q, monkeypatch, path, n, lt, f, max_sweeps):
+    # hybrid: per-colour launches for the first poll's sweeps, then the LDS
+    # kernel continues the same call from the launches' state and flags
     monkeypatch.setenv("QEW_RK_KERNEL", path)
     # 16 x 16384 and 64 x 32768 sites per column of a 256 x 64 table: many
     # moving sites per workgroup in the first sweeps, few near the end, so
@@ -353,6 +355,9 @@ def test_rk_long_lines_match_oracle_bitwise(q, monkeypatch, path, n, lt, f, max_
     r = q.rk_block(h, q.Params(256.0, lt, f, q.Model.Linear), gn, rp)
     assert (r.blocked, r.runaway, r.sweeps) == (bool(r_ref.blocked), bool(r_ref.runaway), r_ref.sweeps)
     assert r.n_velocity_evals == r_ref.n_velocity_evals
+    # the device skips unchanged sites: it evaluates fewer velocities than the
+    # reference algorithm, but at least one per site in the first sweep
+    assert n <= r.n_velocity_evals_done
     np.testing.assert_array_equal(host(h), h_ref)
     assert abs(r.mean_h - r_ref.mean_h) < 1e-12
 
@@ -376,7 +381,7 @@ def test_rk_site_velocity_is_minus_gradient(q):
         assert np.max(np.abs(host(v) + g)) < 1e-12
 
 
-@pytest.mark.parametrize("path", ["line", "launch", None])
+@pytest.mark.parametrize("path", ["line", "launch", "hybrid", None])
 def test_batch_rk_matches_single_lines(q, monkeypatch, path):
     # the batch runs on the chosen path (None: the path timed on the batch's
     # first call), the single lines on the default one

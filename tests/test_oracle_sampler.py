@@ -270,3 +270,30 @@ def test_oracle_midpoint_table_equals_full_sampler_at_half_cells(nx, ny, dx, dy)
     bad = O.OracleNoise.mid_table(np.roll(_mid_fold(coef), 1, axis=0), nx * dx, ny * dy)
     vb, _, _ = O.sample_points(bad, x, y)
     assert np.max(np.abs(vb - v)) > 1e-3 * scale
+
+
+@pytest.mark.parametrize("model", [0, 1])
+def test_gradient_edges_are_the_sampler_and_slope(model):
+    """The oracle's per-edge workspace values are dV/dy of the (separately
+    pinned) sampler at each edge midpoint, including the wrap edge's at
+    (n - 1) dx / 2, and the arclength slope s / sqrt(1 + s^2); the gradient
+    assembled from them is orc_gradient's."""
+    n, ny = 40, 32
+    field, coef = O.filtered_noise(n, ny, 2.0, 1.0, 0.5, 0.2, 0.2, 9)
+    on = O.OracleNoise.table(coef, 2.0, 1.0)
+    h = 0.5 + 0.05 * np.random.default_rng(2).standard_normal(n)
+    p = O.params(2.0, 1.1, 0.05, model)
+    g, e, s = O.gradient_edges(h, p, on)
+    dx = 2.0 / n
+    i = np.arange(n)
+    ip = (i + 1) % n
+    lib = O.load()
+    e_ref = np.array([lib.orc_sample_dy(on.ref, (a + b) * 0.5 * dx, (h[b] + h[a]) * 0.5)
+                      for a, b in zip(i, ip)])
+    np.testing.assert_array_equal(e, e_ref)
+    if model == 1:
+        sl = (h[ip] - h) / dx
+        np.testing.assert_allclose(s, sl / np.sqrt(1 + sl * sl), rtol=1e-15, atol=0)
+    else:
+        assert s is None
+    np.testing.assert_array_equal(g, O.gradient(h, p, on))
