@@ -6,10 +6,10 @@ unit amplitude, seed 1, grid units dx = dy = 1), arclength elasticity,
 line tension 1, no driving force. The line is a seeded random walk with
 unit-variance Gaussian steps started at Nh/2. One step = 10^4 fused
 gradient + energy evaluations issued as one batched launch; evaluation r sees
-the walk shifted by 8 r cells (mod Nh), so the evaluations in flight read
-disjoint table rows and the 1024-shift cycle spans the whole table (no L2/MALL
-reuse across evaluations). Line states and gradients of all 10^4 evaluations
-are distinct buffers in HBM.
+the walk shifted by 520 r cells (mod Nh), more than the walk's 500-cell band,
+so the evaluations in flight read disjoint table rows, and the 1024 distinct
+shifts cycle through the whole table (no L2/MALL reuse across evaluations).
+Line states and gradients of all 10^4 evaluations are distinct buffers in HBM.
 
 `value` is FP32-compensated evaluations per second (FP32 line offsets from a
 per-line anchor, FP32 table, two-sum energy); the FP64 path (bit-exact to the
@@ -30,8 +30,10 @@ L = 65536
 NY = 8192
 XI = 8.0
 EVALS = 10000
-SHIFT = 8
-NSHIFT = NY // SHIFT
+# evaluation r sees the walk shifted by (r * SHIFT) mod NY cells; the walk's
+# band (its y-range, about 500 cells) is narrower than SHIFT, so evaluations in
+# flight together read disjoint table rows
+SHIFT = 520
 # Rosso-Krauth estimate of the depinning force of the Langevin workload's
 # disorder (65536 x 4096 table, xi = 4, seed 1, linear, line tension 1, FP32
 # table, the depinning tool's knobs): blocked at 0.5078, not blocked at
@@ -129,7 +131,7 @@ def cpu_baseline(coef_rowmajor, h0, budget_s=15.0):
     t0 = time.perf_counter()
     k = 0
     while True:
-        h = h0 + SHIFT * (k % NSHIFT)
+        h = h0 + float((SHIFT * k) % NY)
         O.gradient(h, p, nz, omp=True)
         O.objective(h, p, nz, omp=True)
         k += 1
@@ -600,6 +602,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--evals", type=int, default=EVALS)
+    ap.add_argument("--shift", type=int, default=SHIFT,
+                    help="cells between the walks of consecutive evaluations (mod Ny)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lbfgs", action="store_true", help="skip the batched L-BFGS workload")
     ap.add_argument("--no-langevin", action="store_true", help="skip the Langevin anneal")
@@ -653,7 +657,7 @@ def main():
         return
     h0 = walk()
     B = args.evals
-    shifts = torch.tensor((np.arange(B) % NSHIFT) * SHIFT, dtype=torch.float64, device=dev)
+    shifts = torch.tensor(((np.arange(B) * args.shift) % NY).astype(np.float64), device=dev)
     u_sum = distinct_coefficients(h0, L, L, NY)
     log(f"sum_r U_r = {u_sum} ({u_sum / L:.3f} per row)")
     u_mid = distinct_mid_values(h0, L, L, NY)
