@@ -110,6 +110,58 @@ def distinct_mid_values(h, n, nx, ny):
     return int(np.unique(np.concatenate(keys)).size)
 
 
+def edge_cells(h, n, nx, ny):
+    """(x cell, y cell) of every edge midpoint of line h (line spacing = table
+    spacing): edge i reads x-rows ix-1..ix+2 and y-rows iy-1..iy+2."""
+    i = np.arange(n)
+    ip = (i + 1) % n
+    gx = (i + ip) * 0.5 * (nx / n)
+    ix = np.floor(gx).astype(np.int64) % nx
+    hc = (h[ip] + h) * 0.5
+    iy = np.floor(np.mod(hc, ny)).astype(np.int64)
+    return ix, iy
+
+
+def count_lines(h, n, nx, ny, value_bytes=4, tile=None, mid=False):
+    """Distinct 128-byte lines the table gathers of one evaluation touch.
+    tile=None: x-contiguous rows (the device layout); tile=(tx, ty): blocks of
+    tx x-columns by ty y-rows per 128-byte line, x fastest inside a block.
+    mid: one value per edge column (midpoint-contracted table), else the
+    4 x 4 window."""
+    ix, iy = edge_cells(h, n, nx, ny)
+    per_line = 128 // value_bytes
+    xs = [0] if mid else [-1, 0, 1, 2]
+    keys = []
+    for a in xs:
+        x = (ix + a) % nx
+        for b in (-1, 0, 1, 2):
+            y = (iy + b) % ny
+            if tile is None:
+                keys.append(y * ((nx + per_line - 1) // per_line) + x // per_line)
+            else:
+                tx, ty = tile
+                keys.append((y // ty) * ((nx + tx - 1) // tx) + x // tx)
+    return int(np.unique(np.concatenate(keys)).size)
+
+
+def layout_report(h, n, nx, ny):
+    """Lines per evaluation of the benchmark walk for the candidate layouts;
+    bytes over the compulsory table bytes (s_c sum_r U_r, or 4 values per
+    edge for the midpoint table)."""
+    out = {}
+    comp4 = 4 * distinct_coefficients(h, n, nx, ny)
+    compm = 4 * distinct_mid_values(h, n, nx, ny)
+    for name, tile in (("x-contiguous", None), ("16x2", (16, 2)), ("8x4", (8, 4)),
+                       ("4x8", (4, 8))):
+        for kind, mid, comp in (("fp32_4x4", False, comp4), ("fp32_mid", True, compm)):
+            lines = count_lines(h, n, nx, ny, 4, tile, mid)
+            out[f"{kind}/{name}"] = {"lines": lines, "bytes_over_compulsory": 128 * lines / comp}
+    lines = count_lines(h, n, nx, ny, 8, None, False)
+    out["fp64_4x4/x-contiguous"] = {"lines": lines,
+                                    "bytes_over_compulsory": 128 * lines / (2 * comp4)}
+    return out
+
+
 def traffic_from_profiles():
     p = os.path.join(ROOT, "profiles", "eval_traffic.json")
     if os.path.exists(p):
@@ -483,7 +535,7 @@ def bench_langevin(log, nsteps=100000, with_oracle=True, omp_info=None):
 
 
 def bench_rk(log, nlines=64, budget_s=60.0, precisions=("f64", "f32_mid"), k_way=4,
-             with_cpu=True, omp_info=None, cpu_lines=2):
+             with_cpu=True, omp_info=None, cpu_lines=2, state_path=None, defer_cpu=False):
     """Batched Rosso-Krauth depinning: nlines realizations of L = 16384 sites,
     each on its own 16384 x 1024 table (period M = 1024 in h, xi = 4, seeds
     1..nlines), linear elasticity, line tension 1, the depinning tool's knobs
@@ -496,7 +548,10 @@ def bench_rk(log, nlines=64, budget_s=60.0, precisions=("f64", "f32_mid"), k_way
     lines per second, and the site-velocity evaluations actually performed.
     The CPU baseline runs the oracle (OpenMP over sites) through the same first
     two probes (relaxation at f = 0, ramp probe f = 0.125) on `cpu_lines`
-    lines, whose heights must equal the GPU's bit for bit."""
+    lines, whose heights must equal the GPU's bit for bit. With `state_path`
+    (one precision), the search continues from the brackets and blocked
+    configurations saved there by an earlier run and saves its own at the
+    end, so that f_c to 1e-10 can be reached over several runs."""
     import torch
     import qew_amd as q
     from qew_amd.depinning import fc_search
@@ -523,9 +578,17 @@ def bench_rk(log, nlines=64, budget_s=60.0, precisions=("f64", "f32_mid"), k_way
 
     for name in precisions:
         log(f"rk {name}:")
+        state = None
+        if state_path and os.path.exists(state_path):
+            z = np.load(state_path)
+            state = (z["lo"], z["hi"], z["h_lo"])
+            log(f"rk {name}: continuing from {state_path}")
         lo, hi, h_lo, st = fc_search(tables[name], Lr, ny / 2.0, rp, k_way=k_way,
                                      budget_s=budget_s, log=log,
-                                     record=record if name == "f64" else None)
+                                     record=record if name == "f64" and state is None else None,
+                                     state=state)
+        if state_path:
+            np.savez(state_path, lo=lo, hi=hi, h_lo=h_lo.cpu().numpy())
         brackets[name] = (lo, hi)
         width = (hi - lo) / np.maximum(np.abs(hi), 1e-300)
         sec = st.pop("seconds")
@@ -553,10 +616,10 @@ def bench_rk(log, nlines=64, budget_s=60.0, precisions=("f64", "f32_mid"), k_way
                        f"{budget_s:g} s budget per precision")
     log(f"rk: {json.dumps(out)}")
     if 0 in first and 1 in first:
-        if with_cpu:
+        if with_cpu and not defer_cpu:
             out["cpu_baseline"], out["first_probes_vs_oracle"] = rk_oracle_sample(
                 log, n64[:cpu_lines], Lr, ny, first, omp_info, nlines)
-        else:
+        elif with_cpu:
             # for the caller to run the oracle sample later (not serialised)
             out["first_probes"] = {"noises": n64[:cpu_lines], "Lr": Lr, "ny": ny,
                                    "first": first, "nlines": nlines}
@@ -615,6 +678,9 @@ def main():
                          "(checked between rounds; f_c to 1e-10 takes about 15 rounds)")
     ap.add_argument("--rk-only", action="store_true",
                     help="run only the Rosso-Krauth workload and print its JSON object")
+    ap.add_argument("--rk-state", default=None,
+                    help="with --rk-only and one precision: continue the f_c search from this "
+                         ".npz file if it exists, and save the search state there at the end")
     ap.add_argument("--rk-precisions", default="f64,f32_mid",
                     help="table types of the Rosso-Krauth workload (f64, f32, f32_mid)")
     ap.add_argument("--rk-kway", type=int, default=4,
@@ -624,10 +690,16 @@ def main():
     ap.add_argument("--no-bf16", action="store_true", help="skip the BF16-table evaluation leg")
     ap.add_argument("--no-mid", action="store_true",
                     help="skip the midpoint-contracted FP32 table evaluation leg")
+    ap.add_argument("--count-lines", action="store_true",
+                    help="only count the 128-byte table lines one evaluation of the walk touches "
+                         "under candidate table layouts (CPU) and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (reference point for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE scaling) and exit")
     args = ap.parse_args()
+    if args.count_lines:
+        print(json.dumps(layout_report(walk(), L, L, NY)), flush=True)
+        return
     omp_info = omp_setup()
     if args.copy_calibration:
         import torch
@@ -652,7 +724,7 @@ def main():
     if args.rk_only:
         out = bench_rk(log, budget_s=args.rk_budget,
                        precisions=tuple(args.rk_precisions.split(",")), k_way=args.rk_kway,
-                       with_cpu=not args.no_cpu, omp_info=omp_info)
+                       with_cpu=not args.no_cpu, omp_info=omp_info, state_path=args.rk_state)
         print(json.dumps(out), flush=True)
         return
     h0 = walk()
@@ -830,7 +902,8 @@ def main():
         torch.cuda.empty_cache()
         line["rosso_krauth"] = bench_rk(log, budget_s=args.rk_budget,
                                         precisions=tuple(args.rk_precisions.split(",")),
-                                        k_way=args.rk_kway, with_cpu=False, omp_info=omp_info)
+                                        k_way=args.rk_kway, with_cpu=with_cpu, defer_cpu=True,
+                                        omp_info=omp_info)
     finish_langevin()
     if not args.no_rk and with_cpu:
         # the oracle's Rosso-Krauth sample after the anneal has freed the cores

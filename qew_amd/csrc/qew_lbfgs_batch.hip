@@ -63,9 +63,13 @@ struct BatchArgs {
     double *h;
     void *work;
     qew_lbfgs_result *results;
-    const void *tw;  // C2<T>[tw_m]: exp(2 pi i j / tw_m)
+    // twiddles: power-of-two lengths share C2<T> tw[j] = exp(2 pi i j / tw_m),
+    // tw_m = 4 * the largest N = n / 2; in the MIXED kernels each length has
+    // its own table, line l reading tw[tw_off[l] + j] = exp(2 pi i j / (2 n))
+    const void *tw;
     int tw_m;
     qew_lbfgs_params lp;
+    const int64_t *tw_off;
 };
 
 template <typename T>
@@ -567,16 +571,17 @@ __device__ __forceinline__ void wg_gradient(const LineGeom &G, const T *x, T *g,
     __syncthreads();
 }
 
-// ---- in-place LDS Stockham FFT of N = 2^k complex values ----
-// tw[j] = exp(2 pi i j / M); the stage twiddle exp(sign 2 pi i q k / (Ns R))
-// is tw[q k M / (Ns R)], conjugated for sign < 0.
-// One radix-R stage. N <= kMaxFftN and kLB threads: at most kMaxFftN / (R kLB)
-// butterflies per thread; the loops have compile-time bounds so the values
-// stay in registers.
+// ---- in-place LDS Stockham FFT of N = 2^a 3^b 5^c 7^d complex values ----
+// tw[j] = exp(2 pi i j / M) with M = 4 N; the stage twiddle
+// exp(sign 2 pi i q k / (Ns R)) is tw[q k M / (Ns R)] and the radix-R root
+// exp(sign 2 pi i / R) is tw[M / R], both conjugated for sign < 0.
+// One radix-R stage. N <= kMaxFftN and kLB threads: at most
+// ceil(kMaxFftN / (R kLB)) butterflies per thread; the loops have
+// compile-time bounds so the values stay in registers.
 template <typename T, int R>
 __device__ __forceinline__ void fft_stage(C2<T> *buf, int N, int Ns, int sign, const C2<T> *tw,
                                           int M) {
-    constexpr int CB = kMaxFftN / (R * kLB);
+    constexpr int CB = (kMaxFftN / R + kLB - 1) / kLB;
     const int m = N / R;
     const int step = M / (Ns * R);
     C2<T> v[CB][R];
@@ -604,7 +609,22 @@ __device__ __forceinline__ void fft_stage(C2<T> *buf, int N, int Ns, int sign, c
             }
         }
         const int idxD = (j / Ns) * Ns * R + k;
-        if constexpr (R == 2) {
+        if constexpr (R == 3 || R == 5 || R == 7) {
+            // direct R-point DFT: out_s = sum_q u_q W_R^(q s)
+            const int rs = M / R;
+#pragma unroll
+            for (int o = 0; o < R; ++o) {
+                C2<T> acc = u[0];
+#pragma unroll
+                for (int q = 1; q < R; ++q) {
+                    C2<T> w = tw[((q * o) % R) * rs];
+                    if (sign < 0) w.y = -w.y;
+                    const C2<T> t = cm(u[q], w);
+                    acc = {acc.x + t.x, acc.y + t.y};
+                }
+                buf[idxD + o * Ns] = acc;
+            }
+        } else if constexpr (R == 2) {
             buf[idxD] = {u[0].x + u[1].x, u[0].y + u[1].y};
             buf[idxD + Ns] = {u[0].x - u[1].x, u[0].y - u[1].y};
         } else {
@@ -622,16 +642,48 @@ __device__ __forceinline__ void fft_stage(C2<T> *buf, int N, int Ns, int sign, c
     __syncthreads();
 }
 
-template <typename T>
+// Radices in the order 4, 2 and, in the MIXED instantiation (lengths with
+// factors 3, 5 or 7; its larger butterflies would raise the register pressure
+// of the power-of-two kernels), 3, 5, 7. The host accepts only lengths with
+// no other prime factor; any other remainder stops the loop rather than spin.
+template <typename T, bool MIXED>
 __device__ void lds_fft(C2<T> *buf, int N, int sign, const C2<T> *tw, int M) {
     int Ns = 1;
+    if constexpr (!MIXED) {
+        while (Ns < N) {
+            if ((N / Ns) % 4 == 0) {
+                fft_stage<T, 4>(buf, N, Ns, sign, tw, M);
+                Ns *= 4;
+            } else {
+                fft_stage<T, 2>(buf, N, Ns, sign, tw, M);
+                Ns *= 2;
+            }
+        }
+        return;
+    }
     while (Ns < N) {
-        if ((N / Ns) % 4 == 0) {
+        const int rest = N / Ns;
+        if (rest % 4 == 0) {
             fft_stage<T, 4>(buf, N, Ns, sign, tw, M);
             Ns *= 4;
-        } else {
+        } else if (rest % 2 == 0) {
             fft_stage<T, 2>(buf, N, Ns, sign, tw, M);
             Ns *= 2;
+        } else if constexpr (MIXED) {
+            if (rest % 3 == 0) {
+                fft_stage<T, 3>(buf, N, Ns, sign, tw, M);
+                Ns *= 3;
+            } else if (rest % 5 == 0) {
+                fft_stage<T, 5>(buf, N, Ns, sign, tw, M);
+                Ns *= 5;
+            } else if (rest % 7 == 0) {
+                fft_stage<T, 7>(buf, N, Ns, sign, tw, M);
+                Ns *= 7;
+            } else {
+                break;
+            }
+        } else {
+            break;
         }
     }
 }
@@ -639,11 +691,11 @@ __device__ void lds_fft(C2<T> *buf, int N, int sign, const C2<T> *tw, int M) {
 // d = irfft(rfft(q) / eig), eig[k] = (kappa/dx) 4 sin^2(pi k / n) + shift
 // (src/lbfgs.h:116-146, 173-179); n = 2N. `buf` holds the packed q on entry
 // and d, as n reals, on exit.
-template <typename T>
+template <typename T, bool MIXED>
 __device__ void wg_precondition(int n, double kdx, double shift, C2<T> *buf, const C2<T> *tw,
                                 int M) {
     const int N = n / 2;
-    lds_fft(buf, N, -1, tw, M);
+    lds_fft<T, MIXED>(buf, N, -1, tw, M);
     const int e_w = M / n;        // W_n^k = conj(tw[k e_w])
     const int e_s = M / (2 * n);  // sin(pi k / n) = tw[k e_s].y
     auto eig = [&](int k) {
@@ -683,7 +735,7 @@ __device__ void wg_precondition(int n, double kdx, double shift, C2<T> *buf, con
         }
     }
     __syncthreads();
-    lds_fft(buf, N, +1, tw, M);
+    lds_fft<T, MIXED>(buf, N, +1, tw, M);
     // complex j is reals 2j, 2j + 1: each thread scales its own entries in place
     const T inv = (T)(1.0 / (double)N);
     for (int j = threadIdx.x; j < N; j += blockDim.x) {
@@ -693,7 +745,7 @@ __device__ void wg_precondition(int n, double kdx, double shift, C2<T> *buf, con
     __syncthreads();
 }
 
-template <typename T, typename C, bool MID = false>
+template <typename T, typename C, bool MID = false, bool MIXED = false>
 __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, double *rho, double *la,
                            int *slot, int *sbuf, C2<T> *fftbuf) {
     // elements per thread with loads in flight in the vector passes: 16 for
@@ -707,6 +759,8 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
     const int m = lp.m;
     const int tid = threadIdx.x;
     const C2<T> *tw = static_cast<const C2<T> *>(a.tw);
+    if constexpr (MIXED)
+        if (lp.precondition) tw += a.tw_off[line];
     double *h = a.h + L.off;
     const double anchor = (sizeof(T) == 8) ? 0.0 : h[0];
     const LineGeom G = make_geom(L, nd, anchor);
@@ -795,7 +849,7 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
         };
         if (pre) {
             first_loop(fb);
-            wg_precondition<T>(n, kdx, shift, fftbuf, tw, a.tw_m);
+            wg_precondition<T, MIXED>(n, kdx, shift, fftbuf, tw, MIXED ? 2 * n : a.tw_m);
         } else {
             first_loop(q);
             const T g0 = (T)(nslots == 0 ? 1.0 : gamma);
@@ -974,7 +1028,7 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
     __syncthreads();
 }
 
-template <typename T, typename C, bool MID = false>
+template <typename T, typename C, bool MID = false, bool MIXED = false>
 __global__ __launch_bounds__(kLB) void lbfgs_batch_kernel(BatchArgs a) {
     extern __shared__ unsigned char lds[];
     __shared__ double s_red[kLS * (kLB / kWave)];
@@ -989,7 +1043,8 @@ __global__ __launch_bounds__(kLB) void lbfgs_batch_kernel(BatchArgs a) {
         const int qi = s_line;
         __syncthreads();
         if (qi >= a.nlines) return;
-        solve_line<T, C, MID>(a, a.order[qi], r, rho, la, slot, sbuf, reinterpret_cast<C2<T> *>(lds));
+        solve_line<T, C, MID, MIXED>(a, a.order[qi], r, rho, la, slot, sbuf,
+                                     reinterpret_cast<C2<T> *>(lds));
     }
 }
 
@@ -1119,7 +1174,49 @@ __global__ __launch_bounds__(kLB) void fire_batch_kernel(FireArgs a) {
     }
 }
 
+// Launches one persistent minimisation kernel: as many workgroups as fit at
+// once (each takes lines from the queue), at most one per line.
+template <class K>
+void launch_persistent(K kernel, const BatchArgs &a, int nlines, size_t lds, qew_ctx ctx) {
+    if (lds > 64 * 1024)
+        QEW_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0;
+    QEW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kLB, lds));
+    const int grid = std::max(1, std::min(nlines, std::max(1, per_cu) * ctx->num_cus));
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(kLB), lds, ctx->stream, a);
+    QEW_HIP(hipGetLastError());
+}
+
 }  // namespace
+
+namespace qew_host {
+// The MIXED kernels (radix-3/5/7 preconditioner) are instantiated in their own
+// translation unit, qew_lbfgs_mixed.hip, which compiles this file with
+// QEW_LBFGS_MIXED_UNIT defined: instantiated beside the power-of-two kernels,
+// they change how the compiler inlines the device code all of them share, and
+// the power-of-two kernels then spill more (FP64: 200 -> 296 bytes per lane).
+// `args` is a BatchArgs of this file; variant 0..4: FP64 state on FP64
+// midpoint tables, FP32 on FP32 midpoint tables, FP32 on FP32 tables, FP64 on
+// FP32 tables, FP64 on FP64 tables.
+void launch_lbfgs_mixed(int variant, const void *args, int nlines, size_t lds, qew_ctx ctx);
+}  // namespace qew_host
+
+#ifdef QEW_LBFGS_MIXED_UNIT
+
+void qew_host::launch_lbfgs_mixed(int variant, const void *args, int nlines, size_t lds,
+                                  qew_ctx ctx) {
+    const BatchArgs &a = *static_cast<const BatchArgs *>(args);
+    switch (variant) {
+        case 0: launch_persistent(lbfgs_batch_kernel<double, double, true, true>, a, nlines, lds, ctx); break;
+        case 1: launch_persistent(lbfgs_batch_kernel<float, float, true, true>, a, nlines, lds, ctx); break;
+        case 2: launch_persistent(lbfgs_batch_kernel<float, float, false, true>, a, nlines, lds, ctx); break;
+        case 3: launch_persistent(lbfgs_batch_kernel<double, float, false, true>, a, nlines, lds, ctx); break;
+        default: launch_persistent(lbfgs_batch_kernel<double, double, false, true>, a, nlines, lds, ctx); break;
+    }
+}
+
+#else  // the power-of-two kernels and the entry points
 
 using namespace qew_host;
 
@@ -1153,10 +1250,14 @@ extern "C" qew_status qew_batch_lbfgs(qew_batch b, int dtype, const qew_lbfgs_pa
             QEW_REQUIRE(nd.kind != TABLE || nd.dtype == tdt,
                         "all tables of a batch must share one storage type");
         if (lp->precondition)
-            for (const auto &l : b->lines)
-                QEW_REQUIRE(l.n >= 4 && (l.n & (l.n - 1)) == 0 && l.n / 2 <= kMaxFftN,
-                            "the batched preconditioner needs power-of-two line lengths in "
-                            "[4, 16384]");
+            for (const auto &l : b->lines) {
+                int r = l.n / 2;
+                for (const int p : {2, 3, 5, 7})
+                    while (r > 1 && r % p == 0) r /= p;
+                QEW_REQUIRE(l.n >= 4 && l.n % 2 == 0 && l.n / 2 <= kMaxFftN && r == 1,
+                            "the batched preconditioner needs even line lengths in [4, 16384] "
+                            "whose half has no prime factor above 7");
+            }
         qew_ctx ctx = b->ctx;
         use_device(ctx);
         const size_t es = dtype == QEW_F32 ? sizeof(float) : sizeof(double);
@@ -1164,15 +1265,39 @@ extern "C" qew_status qew_batch_lbfgs(qew_batch b, int dtype, const qew_lbfgs_pa
         int maxN = 1;
         if (lp->precondition)
             for (const auto &l : b->lines) maxN = std::max(maxN, l.n / 2);
-        // twiddle table length: 4 * the largest transform, so that the stage
-        // twiddles, W_n^k (n = 2N) and sin(pi k / n) are all entries
-        const int tw_m = 4 * maxN;
-        // layout: [order nlines ints][counter][results][twiddles][work]
+        // one twiddle table of 4 N entries per distinct transform length N, so
+        // that the stage twiddles, the radix roots, W_n^k (n = 2N) and
+        // sin(pi k / n) are all entries of the line's table
+        std::vector<int64_t> tw_off(b->nlines, 0);
+        std::vector<std::pair<int, int64_t>> tables;  // (N, offset)
+        int64_t tw_total = 0;
+        // lengths whose half is not a power of two need the odd-radix stages
+        bool mixed = false;
+        if (lp->precondition)
+            for (const auto &l : b->lines) mixed = mixed || ((l.n / 2) & (l.n / 2 - 1)) != 0;
+        if (lp->precondition && !mixed) {
+            tables.emplace_back(maxN, 0);  // one table of 4 * max N entries serves them all
+            tw_total = 4 * (int64_t)maxN;
+        } else if (lp->precondition)
+            for (int l = 0; l < b->nlines; ++l) {
+                const int N = b->lines[l].n / 2;
+                auto it = std::find_if(tables.begin(), tables.end(),
+                                       [&](const std::pair<int, int64_t> &t) { return t.first == N; });
+                if (it == tables.end()) {
+                    tables.emplace_back(N, tw_total);
+                    tw_off[l] = tw_total;
+                    tw_total += 4 * (int64_t)N;
+                } else {
+                    tw_off[l] = it->second;
+                }
+            }
+        // layout: [order nlines ints][counter][results][table offsets][twiddles][work]
         auto up = [](size_t v) { return (v + 255) / 256 * 256; };
         const size_t off_next = up(sizeof(int) * b->nlines);
         const size_t off_res = off_next + 256;
-        const size_t off_tw = off_res + up(sizeof(qew_lbfgs_result) * b->nlines);
-        const size_t off_work = off_tw + up(2 * es * (size_t)tw_m);
+        const size_t off_twoff = off_res + up(sizeof(qew_lbfgs_result) * b->nlines);
+        const size_t off_tw = off_twoff + up(sizeof(int64_t) * b->nlines);
+        const size_t off_work = off_tw + up(2 * es * (size_t)std::max<int64_t>(tw_total, 1));
         b->d_scratch.ensure(off_work + es * nvec * (size_t)b->total);
         char *w = b->d_scratch.as<char>();
         std::vector<int> order(b->nlines);
@@ -1183,12 +1308,19 @@ extern "C" qew_status qew_batch_lbfgs(qew_batch b, int dtype, const qew_lbfgs_pa
                                ctx->stream));
         hipLaunchKernelGGL(zero_u32, dim3(1), dim3(1), 0, ctx->stream,
                            reinterpret_cast<unsigned *>(w + off_next));
-        if (dtype == QEW_F32)
-            hipLaunchKernelGGL(twiddle_kernel<float>, dim3((tw_m + 255) / 256), dim3(256), 0,
-                               ctx->stream, reinterpret_cast<C2<float> *>(w + off_tw), tw_m);
-        else
-            hipLaunchKernelGGL(twiddle_kernel<double>, dim3((tw_m + 255) / 256), dim3(256), 0,
-                               ctx->stream, reinterpret_cast<C2<double> *>(w + off_tw), tw_m);
+        QEW_HIP(hipMemcpyAsync(w + off_twoff, tw_off.data(), sizeof(int64_t) * b->nlines,
+                               hipMemcpyHostToDevice, ctx->stream));
+        for (const auto &t : tables) {
+            const int M = 4 * t.first;
+            if (dtype == QEW_F32)
+                hipLaunchKernelGGL(twiddle_kernel<float>, dim3((M + 255) / 256), dim3(256), 0,
+                                   ctx->stream, reinterpret_cast<C2<float> *>(w + off_tw) + t.second,
+                                   M);
+            else
+                hipLaunchKernelGGL(twiddle_kernel<double>, dim3((M + 255) / 256), dim3(256), 0,
+                                   ctx->stream,
+                                   reinterpret_cast<C2<double> *>(w + off_tw) + t.second, M);
+        }
         QEW_HIP(hipGetLastError());
         BatchArgs a{};
         a.lines = b->d_lines.as<LineDesc>();
@@ -1200,25 +1332,26 @@ extern "C" qew_status qew_batch_lbfgs(qew_batch b, int dtype, const qew_lbfgs_pa
         a.work = w + off_work;
         a.results = reinterpret_cast<qew_lbfgs_result *>(w + off_res);
         a.tw = w + off_tw;
-        a.tw_m = tw_m;
+        a.tw_m = (int)(4 * (int64_t)maxN);
+        a.tw_off = reinterpret_cast<const int64_t *>(w + off_twoff);
         a.lp = *lp;
         // the FFT buffer doubles as the packed-q buffer: n reals = N complex
         const size_t lds = (size_t)maxN * 2 * es;
-        auto launch = [&](auto kernel) {
-            if (lds > 64 * 1024)
-                QEW_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            int per_cu = 0;
-            QEW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kLB, lds));
-            const int grid = std::max(1, std::min(b->nlines, std::max(1, per_cu) * ctx->num_cus));
-            hipLaunchKernelGGL(kernel, dim3(grid), dim3(kLB), lds, ctx->stream, a);
-            QEW_HIP(hipGetLastError());
-        };
-        if (dtype == QEW_F64 && tdt == QEW_F64_MID) launch(lbfgs_batch_kernel<double, double, true>);
-        else if (dtype == QEW_F32 && tdt == QEW_F32_MID) launch(lbfgs_batch_kernel<float, float, true>);
-        else if (dtype == QEW_F32) launch(lbfgs_batch_kernel<float, float>);
-        else if (tdt == 1) launch(lbfgs_batch_kernel<double, float>);
-        else launch(lbfgs_batch_kernel<double, double>);
+        auto launch = [&](auto kernel) { launch_persistent(kernel, a, b->nlines, lds, ctx); };
+        if (mixed) {
+            const int variant = (dtype == QEW_F64 && tdt == QEW_F64_MID)   ? 0
+                                : (dtype == QEW_F32 && tdt == QEW_F32_MID) ? 1
+                                : (dtype == QEW_F32)                       ? 2
+                                : (tdt == 1)                               ? 3
+                                                                           : 4;
+            launch_lbfgs_mixed(variant, &a, b->nlines, lds, ctx);
+        } else {
+            if (dtype == QEW_F64 && tdt == QEW_F64_MID) launch(lbfgs_batch_kernel<double, double, true>);
+            else if (dtype == QEW_F32 && tdt == QEW_F32_MID) launch(lbfgs_batch_kernel<float, float, true>);
+            else if (dtype == QEW_F32) launch(lbfgs_batch_kernel<float, float>);
+            else if (tdt == 1) launch(lbfgs_batch_kernel<double, float>);
+            else launch(lbfgs_batch_kernel<double, double>);
+        }
         QEW_HIP(hipMemcpyAsync(results, a.results, sizeof(qew_lbfgs_result) * b->nlines,
                                hipMemcpyDeviceToHost, ctx->stream));
         QEW_HIP(hipStreamSynchronize(ctx->stream));
@@ -1313,3 +1446,5 @@ extern "C" qew_status qew_fire_minimize(qew_ctx ctx, qew_noise noise, const qew_
         check(qew_batch_fire(b, fp, h, result));
     });
 }
+
+#endif  // QEW_LBFGS_MIXED_UNIT

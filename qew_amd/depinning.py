@@ -20,14 +20,16 @@ from . import Batch, Model, Params
 
 def fc_search(noises, n, y0, rp, lengths_L=None, line_tension=1.0, model=Model.Linear,
               f_start=0.125, k_way=4, rel_tol=1e-10, budget_s=float("inf"), log=None,
-              record=None):
+              record=None, state=None):
     """f_c bracket of every potential in `noises` for a line of n sites (box
     length lengths_L, default n) started flat at y0: relaxation at f = 0,
     ramp f = f_start * 2^k per line until it is not blocked, then k_way-way
     rounds until every bracket is within rel_tol (relative) or the budget
     is spent. Returns (lo, hi, h_lo, stats): brackets, the blocked
     configurations at lo (torch, nlines x n) and counters. `record(k, res, h,
-    seconds)` sees the relaxation (k = 0) and the first ramp probe (k = 1)."""
+    seconds)` sees the relaxation (k = 0) and the first ramp probe (k = 1).
+    `state` = (lo, hi, h_lo) of an earlier call continues its rounds (no
+    relaxation or ramp)."""
     import torch
     nlines = len(noises)
     L = float(n if lengths_L is None else lengths_L)
@@ -57,7 +59,11 @@ def fc_search(noises, n, y0, rp, lengths_L=None, line_tension=1.0, model=Model.L
     k = 0
     f_ramp = f_start
     forces = np.zeros(nlines)
-    while True:
+    if state is not None:
+        lo[:] = state[0]
+        hi[:] = state[1]
+        h_lo.copy_(torch.as_tensor(np.asarray(state[2]), device=dev).view(nlines, n))
+    while state is None:
         rows = np.nonzero(np.isinf(hi))[0] if k else np.arange(nlines)
         res, h, sec = run(rows, forces[rows])
         if record and k < 2:

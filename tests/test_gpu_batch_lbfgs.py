@@ -104,11 +104,49 @@ def test_batch_lbfgs_harmonic_and_branches(q):
 
 
 def test_batch_lbfgs_rejects_bad_lengths_for_fft(q):
+    # the LDS preconditioner takes even lengths whose half has no prime factor
+    # above 7: 22 = 2 * 11 and the odd 105 are rejected
     well = q.HarmonicWell(2.0, 0.5)
-    p = q.Params(100.0, 1.0, 0.0, q.Model.Linear)
-    b = q.Batch([100], [well], [p])
-    with pytest.raises(ValueError):
-        b.lbfgs(dev(np.zeros(100)), q.LbfgsParams(precondition=True))
+    for n in (22, 105):
+        b = q.Batch([n], [well], [q.Params(float(n), 1.0, 0.0, q.Model.Linear)])
+        with pytest.raises(ValueError):
+            b.lbfgs(dev(np.zeros(n)), q.LbfgsParams(precondition=True))
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_batch_lbfgs_mixed_radix_preconditioner_matches_oracle(q, dtype):
+    """Preconditioned lines whose half-length has factors 3, 5 and 7 (the
+    reference preconditions any n, src/lbfgs.h:116-146; n = 100 is the
+    length of its FFT test, tests/test_fft_util.cpp:9) in one batch with
+    power-of-two ones: the FP64 kernel reaches the oracle's minima (energy
+    1e-11 relative) in about as many iterations, FP32 within its stated 1e-6."""
+    lengths = [100, 96, 120, 126, 1000, 6000, 14, 256]
+    xis = [2.0, 4.0, 4.0, 2.0, 8.0, 4.0, 2.0, 4.0]
+    models = [0, 1, 0, 1, 0, 1, 0, 1]
+    noises, oracle, params, h0 = make_problem(q, lengths, 128, xis, models,
+                                              list(range(31, 31 + len(lengths))), q.F64)
+    shifts = [0.5 / xi ** 2 for xi in xis]
+    kw = dict(m=10, ftol=1e-6, max_iter=4000, precondition=True)
+    if dtype == "f32":
+        noises = [nz.convert(q.F32) for nz in noises]
+    b = q.Batch(lengths, noises, params)
+    b.set_precond_shifts(shifts)
+    h = dev(np.concatenate(h0))
+    res = b.lbfgs(h, q.LbfgsParams(**{**kw, "ftol": 1e-6 if dtype == "f64" else 1e-4}),
+                  dtype=q.F64 if dtype == "f64" else q.F32)
+    hh = host(h)
+    for k, n in enumerate(lengths):
+        on, po = oracle[k]
+        h_ref, r_ref = O.lbfgs(h0[k], po, on, O.lbfgs_params(**kw, precond_shift=shifts[k]))
+        assert r_ref.converged and res[k].converged, (k, n, res[k], r_ref.iterations)
+        e = O.objective(hh[b.offsets[k]:b.offsets[k + 1]], po, on)
+        e_ref = O.objective(h_ref, po, on)
+        if dtype == "f64":
+            assert rel(e, e_ref) < 1e-11, (k, n, e, e_ref)
+            assert abs(res[k].iterations - r_ref.iterations) <= max(5, r_ref.iterations // 5), \
+                (k, n, res[k].iterations, r_ref.iterations)
+        else:
+            assert rel(e, e_ref) < 1e-6, (k, n, e, e_ref)
 
 
 def test_batch_lbfgs_long_lines_match_oracle(q):

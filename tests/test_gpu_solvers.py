@@ -426,8 +426,9 @@ def test_rk_fp32_table_bitwise_vs_oracle_on_rounded_table(q):
 
 def test_batch_langevin_fp32_vs_fp64_statistics(q):
     """FP32 line state (offsets from an anchor) + FP32 table against FP64 on
-    the same table values and the same Philox normals: mean height and
-    roughness W^2 after 2000 steps agree to 1e-5 relative."""
+    the same table values and the same Philox normals: mean velocity (mean
+    displacement over the elapsed time) and roughness W^2 after 2000 steps
+    agree to 1e-5 relative, the Langevin configuration's tolerance."""
     torch = torch_cuda()
     n, ny = 4096, 1024
     field, coef = O.filtered_noise(n, ny, float(n), float(ny), 1.0, 4.0, 4.0, 3)
@@ -447,6 +448,8 @@ def test_batch_langevin_fp32_vs_fp64_statistics(q):
     b32.langevin(o32, drift, diff, seed=11, dtype=q.F32, anchor=anchor)
     a = host(h64)
     b = ny / 2.0 + host(o32).astype(np.float64)
-    assert rel(b.mean(), a.mean()) < 1e-5
+    t = steps * 0.01
+    va, vb = (a.mean() - ny / 2.0) / t, (b.mean() - ny / 2.0) / t
+    assert rel(vb, va) < 1e-5, (va, vb)
     w2a, w2b = a.var(), b.var()
-    assert rel(w2b, w2a) < 1e-4, (w2a, w2b)
+    assert rel(w2b, w2a) < 1e-5, (w2a, w2b)
