@@ -47,7 +47,8 @@ struct StaticConfig {
     int m = 8;  // history length (the reference fixes the LbfgsParams default)
     // true: the pinning lengths are minimised together by the persistent
     // batched L-BFGS kernel (one workgroup per length) when it supports the
-    // line (nx a power of two in [4, 16384], m <= 32); false, or an
+    // line (even nx in [4, 16384] whose half has no prime factor above 7,
+    // m <= 32); false, or an
     // unsupported line: one host-driven minimisation per length, as the
     // reference does. No reference counterpart.
     bool batched = true;
@@ -65,10 +66,15 @@ inline real_t larkin_line_tension(double lp_over_xi, real_t amplitude) {
 }
 
 // True when the persistent batched L-BFGS kernel accepts a preconditioned
-// line of n sites with history m (its FFT works on power-of-two lengths up to
-// 16384 in one workgroup's LDS).
+// line of n sites with history m: its FFT transforms the n/2 complex values of
+// the packed real line in one workgroup's LDS with radix 2, 3, 4, 5 and 7
+// stages, so n is even, at most 16384, and n/2 has no other prime factor.
 inline bool batch_lbfgs_supports(int n, int m) {
-    return n >= 4 && n <= 16384 && (n & (n - 1)) == 0 && m >= 1 && m <= 32;
+    if (n < 4 || n > 16384 || n % 2 != 0 || m < 1 || m > 32) return false;
+    int r = n / 2;
+    for (int f : {2, 3, 5, 7})
+        while (r % f == 0) r /= f;
+    return r == 1;
 }
 
 template <class Noise>

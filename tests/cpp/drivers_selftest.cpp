@@ -85,10 +85,16 @@ static void static_sweep_batched_equals_sequential() {
             dh = std::max(dh, std::abs(b[k].h[i] - s[k].h[i]));
         CHECK(dh < 1e-5);
     }
+    // the lengths the batched kernel takes: even, <= 16384, half 7-smooth
+    CHECK(batch_lbfgs_supports(24, 8) && batch_lbfgs_supports(210, 8));
+    CHECK(batch_lbfgs_supports(16384, 32) && batch_lbfgs_supports(11250, 10));
+    CHECK(!batch_lbfgs_supports(22, 8) && !batch_lbfgs_supports(25, 8));
+    CHECK(!batch_lbfgs_supports(2, 8) && !batch_lbfgs_supports(16386, 8));
+    CHECK(!batch_lbfgs_supports(256, 33) && !batch_lbfgs_supports(256, 0));
     StaticConfig odd;
     odd.model = Model::Linear;
     odd.Lx = 1.0;
-    odd.nx = 24;  // not a power of two: the sequential path
+    odd.nx = 22;  // half is 11, a prime the LDS FFT has no stage for: the sequential path
     odd.pinning_lengths_over_xi = {0.5, 2.0};
     odd.ftol = 1e-8;
     const auto o = static_sweep(odd, ZeroNoise{});
