@@ -100,7 +100,7 @@ static void static_sweep_batched_equals_sequential() {
     const auto o = static_sweep(odd, ZeroNoise{});
     CHECK(o.size() == 2);
     for (const auto &prof : o) {
-        CHECK(prof.result.converged && prof.h.size() == 24u);
+        CHECK(prof.result.converged && prof.h.size() == 22u);
         for (double v : prof.h) CHECK(std::abs(v - 0.5) < 1e-9);
     }
 }
