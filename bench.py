@@ -262,6 +262,7 @@ def bench_lbfgs(log, nlines=512, plain_sample=8, with_cpu=True, omp_info=None):
         gmax[name] = np.array([ga[o0:o1].max() for o0, o1 in zip(offs[:-1], offs[1:])])
         its = [r.iterations for r in res]
         results[name] = (res, span)
+        wgs = b.lbfgs_workgroups()
         # the critical path: the line with the most site-iterations, solved
         # alone (one workgroup), against the whole launch
         crit = int(np.argmax(np.array(its) * np.array(lengths)))
@@ -273,6 +274,23 @@ def bench_lbfgs(log, nlines=512, plain_sample=8, with_cpu=True, omp_info=None):
         (r1,) = b1.lbfgs(h1, lp, dtype=dtype)
         alone = time.perf_counter() - t2
         del b1, h1
+        # workgroup utilisation, estimated: every 8th line solved alone (on
+        # as many workgroups as in the batch: 4 for L = 16384, 2 for 8192),
+        # its workgroup-seconds scaled to the batch, over the launch's
+        # workgroups x span (solo lines meet no contention, so this
+        # underestimates the busy share somewhat)
+        busy_sample = 0.0
+        sample = list(range(0, nlines, 8))
+        for k in sample:
+            bk = q.Batch([lengths[k]], [noises[k]], [params[k]])
+            bk.set_precond_shifts([shifts[k]])
+            hk = torch.full((lengths[k],), ny / 2.0, dtype=torch.float64, device="cuda:0")
+            torch.cuda.synchronize()
+            t3 = time.perf_counter()
+            bk.lbfgs(hk, lp, dtype=dtype)
+            busy_sample += (time.perf_counter() - t3) * bk.lbfgs_workgroups()
+            del bk, hk
+        busy = busy_sample * nlines / len(sample)
         out[name] = {"minima_per_s": nlines / dt, "seconds": dt, "ftol": ftol,
                      "launch_s": span,
                      "critical_line": {"line": crit, "L": int(lengths[crit]),
@@ -282,7 +300,11 @@ def bench_lbfgs(log, nlines=512, plain_sample=8, with_cpu=True, omp_info=None):
                      "curvature_skips": int(sum(r.n_curvature_skips for r in res)),
                      "ls_restarts": int(sum(r.n_ls_restarts for r in res)),
                      "slowest_line": int(np.argmax(its)),
-                     "iterations_mean": float(np.mean(its)), "iterations_max": int(max(its))}
+                     "iterations_mean": float(np.mean(its)), "iterations_max": int(max(its)),
+                     "workgroup_utilisation": {
+                         "value": busy / (wgs * span), "workgroups": wgs,
+                         "busy_workgroup_s_estimate": busy,
+                         "sample": f"every 8th line solved alone ({len(sample)} lines)"}}
         log(f"lbfgs {name}: {json.dumps(out[name])}")
         del b, h
     # algorithmic bytes of each minimisation: per iteration the vector algebra
@@ -555,6 +577,8 @@ def bench_rk(log, nlines=64, budget_s=60.0, precisions=("f64", "f32_mid"), k_way
     import torch
     import qew_amd as q
     from qew_amd.depinning import fc_search
+    # probes near f_c sweep for minutes: rk_block reports its sweeps meanwhile
+    os.environ.setdefault("QEW_RK_PROGRESS", "30")
     Lr, ny = 16384, 1024
     t0 = time.perf_counter()
     specs = [(Lr, ny, float(Lr), float(ny), 1.0, 4.0, 4.0, 1 + k) for k in range(nlines)]
@@ -583,12 +607,13 @@ def bench_rk(log, nlines=64, budget_s=60.0, precisions=("f64", "f32_mid"), k_way
             z = np.load(state_path)
             state = (z["lo"], z["hi"], z["h_lo"])
             log(f"rk {name}: continuing from {state_path}")
+        def save(lo_, hi_, h_lo_):
+            np.savez(state_path, lo=lo_, hi=hi_, h_lo=h_lo_.cpu().numpy())
+
         lo, hi, h_lo, st = fc_search(tables[name], Lr, ny / 2.0, rp, k_way=k_way,
                                      budget_s=budget_s, log=log,
                                      record=record if name == "f64" and state is None else None,
-                                     state=state)
-        if state_path:
-            np.savez(state_path, lo=lo, hi=hi, h_lo=h_lo.cpu().numpy())
+                                     state=state, checkpoint=save if state_path else None)
         brackets[name] = (lo, hi)
         width = (hi - lo) / np.maximum(np.abs(hi), 1e-300)
         sec = st.pop("seconds")
@@ -680,12 +705,15 @@ def main():
                     help="run only the Rosso-Krauth workload and print its JSON object")
     ap.add_argument("--rk-state", default=None,
                     help="with --rk-only and one precision: continue the f_c search from this "
-                         ".npz file if it exists, and save the search state there at the end")
+                         ".npz file if it exists, and save the search state there after the "
+                         "ramp and after every round")
     ap.add_argument("--rk-precisions", default="f64,f32_mid",
                     help="table types of the Rosso-Krauth workload (f64, f32, f32_mid)")
     ap.add_argument("--rk-kway", type=int, default=4,
                     help="forces probed per line per round of the f_c search")
     ap.add_argument("--lbfgs-lines", type=int, default=512)
+    ap.add_argument("--lbfgs-only", action="store_true",
+                    help="run only the batched L-BFGS workload and print its JSON object")
     ap.add_argument("--only", choices=["both", "f32", "f64"], default="both")
     ap.add_argument("--no-bf16", action="store_true", help="skip the BF16-table evaluation leg")
     ap.add_argument("--no-mid", action="store_true",
@@ -721,6 +749,11 @@ def main():
 
     dev = torch.device("cuda:0")
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    if args.lbfgs_only:
+        out = bench_lbfgs(log, nlines=args.lbfgs_lines, with_cpu=not args.no_cpu,
+                          omp_info=omp_info)
+        print(json.dumps(out), flush=True)
+        return
     if args.rk_only:
         out = bench_rk(log, budget_s=args.rk_budget,
                        precisions=tuple(args.rk_precisions.split(",")), k_way=args.rk_kway,

@@ -320,9 +320,15 @@ qew_status qew_lbfgs_minimize(qew_ctx ctx, qew_noise noise, const qew_params *p,
  * h: device FP64 line state in/out. dtype QEW_F32 runs the line state and
  * gathers in FP32 (FP32 tables) with Kahan-compensated reductions and keeps
  * the Armijo test and the curvature pair update in FP64. Results: host array
- * of nlines. Line lengths must be powers of two when lp->precondition. */
+ * of nlines. With lp->precondition, line lengths must be even, at most 16384,
+ * and their half must have no prime factor above 7. */
 qew_status qew_batch_lbfgs(qew_batch batch, int dtype, const qew_lbfgs_params *lp,
                            double *h, qew_lbfgs_result *results);
+/* Persistent workgroups of the batch's last qew_batch_lbfgs launch (lines of
+ * 8192 sites and more are solved by teams of 2 or 4 workgroups, the others by
+ * one each; QEW_LBFGS_TEAM=k sets k workgroups for every line whose length k
+ * divides into segments of at least 64 sites). No reference counterpart. */
+qew_status qew_batch_lbfgs_workgroups(qew_batch batch, int *workgroups);
 
 /* ---- FIRE (src/fire.h:41-131) ------------------------------------------- */
 
@@ -384,6 +390,19 @@ qew_status qew_rk_forward_zero(qew_ctx ctx, qew_noise noise, const qew_params *p
 qew_status qew_rk_site_velocity(qew_ctx ctx, qew_noise noise, const qew_params *p,
                                 int n, int m, const int *i, const double *hi,
                                 const double *hl, const double *hr, double *v);
+
+/* ---- roughness analysis -------------------------------------------------- */
+
+/* Roughness of m periodic profiles of n heights on a box of length L (h:
+ * device FP64[m * n], profile p at p * n). Replaces the height_difference_acf
+ * / psd calls of the reference's post-processing
+ * (tools/postprocess_static.py:20-47; the module they come from is not in the
+ * reference). Outputs, each device FP64 or NULL to skip: acf[r - 1] =
+ * A(r L / n) = 1/2 <(h(x + r) - h(x))^2>, r = 1 .. n/2, and psd[k - 1] =
+ * C(2 pi k / L) = (L / n^2) |sum_x h(x) exp(-2 pi i k x / n)|^2, k = 1 .. n/2,
+ * both averaged over the m profiles; w2[p] = <(h_p - <h_p>)^2>. */
+qew_status qew_roughness(qew_ctx ctx, int m, int n, double L, const double *h,
+                         double *acf, double *psd, double *w2);
 
 #ifdef __cplusplus
 }

@@ -403,6 +403,31 @@ def gradient(h, p, noise, grad=None):
     return grad
 
 
+def roughness(h, L, acf=True, psd=True):
+    """Roughness of a stack of periodic profiles on the device (h: CUDA FP64
+    tensor (n,) or (m, n); box length L): returns (ell, A, q, C, w2) with the
+    height-difference ACF A(ell) and the PSD C(q) averaged over the profiles
+    (None when not asked for) and each profile's squared width w2, all torch
+    tensors on h's device; conventions of qew_amd.analysis."""
+    torch = _torch()
+    h2 = h.reshape(1, -1) if h.dim() == 1 else h
+    m, n = h2.shape
+    h2 = h2.contiguous()
+    kw = dict(dtype=torch.float64, device=h.device)
+    a = torch.empty(n // 2, **kw) if acf else None
+    c = torch.empty(n // 2, **kw) if psd else None
+    w2 = torch.empty(m, **kw)
+    check(_lib.load().qew_roughness(context(_dev(h)).handle, m, n, float(L),
+                                    _ptr(h2, torch.float64),
+                                    None if a is None else _ptr(a, torch.float64),
+                                    None if c is None else _ptr(c, torch.float64),
+                                    _ptr(w2, torch.float64)))
+    k = torch.arange(1, n // 2 + 1, **kw)
+    ell = k * (float(L) / n)
+    q = 2.0 * np.pi * k / float(L)
+    return ell, a, q, c, w2
+
+
 def gradient_edges(h, p, noise):
     """gradient(h, p, noise, grad, ws) with the GradientWorkspace's per-edge
     quantities (src/qew_model.h:126-201): returns (grad, edge dV/dy, edge slope;
@@ -572,6 +597,12 @@ class Batch:
         check(_lib.load().qew_batch_lbfgs(self.handle, dtype, C.byref(lc), _ptr(h, torch.float64),
                                           res))
         return [LbfgsResult.from_c(r) for r in res]
+
+    def lbfgs_workgroups(self):
+        """Persistent workgroups of the last lbfgs() launch."""
+        wg = C.c_int(0)
+        check(_lib.load().qew_batch_lbfgs_workgroups(self.handle, C.byref(wg)))
+        return wg.value
 
     def fire(self, h, fp):
         """FIRE on every line (FP64 line state h, one persistent launch)."""

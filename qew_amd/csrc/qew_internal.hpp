@@ -162,6 +162,8 @@ struct qew_batch_s {
     // 2 per-colour launches
     int rk_path = 0;
     qew_host::DevBuf d_ltile, d_tune;
+    // persistent workgroups of the last qew_batch_lbfgs launch
+    int lbfgs_workgroups = 0;
 };
 
 namespace qew_host {

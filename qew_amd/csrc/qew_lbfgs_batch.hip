@@ -31,6 +31,7 @@
 // Armijo test, the energies it compares and the curvature pair s.y, s.s, y.y
 // stay FP64.
 #include <algorithm>
+#include <cstdlib>
 #include <numeric>
 #include <type_traits>
 #include <vector>
@@ -48,6 +49,10 @@ constexpr int kU = 8;           // elements per thread with loads in flight (vec
 constexpr int kLS = 4;
 constexpr int kMaxM = 32;       // history length limit
 constexpr int kMaxFftN = 8192;  // complex LDS transform length (line length 16384)
+
+// Workspace vectors per line for history length m: x g gn d q edy esl, m + 1
+// S and m + 1 Y rows, and two for a team's vector exchange.
+__host__ __device__ constexpr int lbfgs_nvec(int m) { return 9 + 2 * (m + 1); }
 
 template <typename T>
 struct C2 {
@@ -70,6 +75,15 @@ struct BatchArgs {
     int tw_m;
     qew_lbfgs_params lp;
     const int64_t *tw_off;
+    // slots: slot s is member slot_member[s] of the team of team_size[l]
+    // workgroups that solves line l = slot_line[s]; the members of a team
+    // are consecutive slots, teams ordered longest line first
+    const int *slot_line;
+    const int *slot_member;
+    const int *team_size;
+    int nslots;
+    unsigned long long *xchg;  // per line [2][kMaxTeam][kXchgVals][2] tagged words
+    int *abort;
 };
 
 template <typename T>
@@ -81,11 +95,12 @@ __device__ __forceinline__ C2<T> cm(C2<T> a, C2<T> b) {
 // consecutive iterations issued before any of their uses. Full chunks run
 // without per-element guards, so the uses (and any loads inside them, such
 // as table gathers) share one basic block and can overlap.
+// The same over the index range [i0, i1) (i = i0 + tid, i0 + tid + kLB, ...).
 template <int U = kU, typename Ld, typename Use>
-__device__ __forceinline__ void stream(int n, Ld ld, Use use) {
-    for (int b = threadIdx.x; b < n; b += U * kLB) {
+__device__ __forceinline__ void stream(int i0, int i1, Ld ld, Use use) {
+    for (int b = i0 + (int)threadIdx.x; b < i1; b += U * kLB) {
         decltype(ld(0)) v[U];
-        if (b + (U - 1) * kLB < n) {
+        if (b + (U - 1) * kLB < i1) {
 #pragma unroll
             for (int u = 0; u < U; ++u) v[u] = ld(b + u * kLB);
 #pragma unroll
@@ -94,15 +109,20 @@ __device__ __forceinline__ void stream(int n, Ld ld, Use use) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int i = b + u * kLB;
-                if (i < n) v[u] = ld(i);
+                if (i < i1) v[u] = ld(i);
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int i = b + u * kLB;
-                if (i < n) use(i, v[u]);
+                if (i < i1) use(i, v[u]);
             }
         }
     }
+}
+
+template <int U = kU, typename Ld, typename Use>
+__device__ __forceinline__ void stream(int n, Ld ld, Use use) {
+    stream<U>(0, n, ld, use);
 }
 
 template <typename T>
@@ -118,6 +138,20 @@ struct P4 {
     T a, b, c, d;
 };
 
+// Team exchange area and device-scope (cross-XCD coherent) scalar access,
+// used by the team machinery below.
+constexpr int kMaxTeam = 8;
+constexpr int kXchgVals = 8;  // values one member publishes per exchange
+
+template <typename T>
+__device__ __forceinline__ void st_dev(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T ld_dev(const T *p) {
+    return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ---- workgroup reductions (uniform results) ----
 struct Red {
     double *d;
@@ -127,11 +161,30 @@ struct Red {
 // dot-product accumulator: FP64 sum, or a Kahan pair for FP32 vectors
 template <typename T>
 struct Acc;
+// With a team (below), reduce(r, tm) combines the members' workgroup
+// partials; part() gives this workgroup's partial as two doubles and
+// combine() the team value from every member's pair at offset j of an
+// exchange's values (member k's at k * kXchgVals).
+struct Team;
+__device__ __forceinline__ double team_sum1(Team &tm, double v);
+__device__ __forceinline__ KSum team_ksum1(Team &tm, KSum k);
 template <>
 struct Acc<double> {
     double s = 0;
     __device__ void add(double p) { s += p; }
     __device__ __forceinline__ double reduce(Red r) { return block_sum(s, r.d); }
+    __device__ __forceinline__ double reduce(Red r, Team &tm) {
+        return team_sum1(tm, block_sum(s, r.d));
+    }
+    __device__ __forceinline__ void part(Red r, double *o) {
+        o[0] = block_sum(s, r.d);
+        o[1] = 0.0;
+    }
+    __device__ static double combine(const double *x, int size, int j) {
+        double acc = x[j];
+        for (int k = 1; k < size; ++k) acc += x[k * kXchgVals + j];
+        return acc;
+    }
 };
 template <>
 struct Acc<float> {
@@ -140,6 +193,21 @@ struct Acc<float> {
     __device__ __forceinline__ double reduce(Red r) {
         const KSum k = block_ksum(s, r.k);
         return (double)k.s + (double)k.c;
+    }
+    __device__ __forceinline__ double reduce(Red r, Team &tm) {
+        const KSum k = team_ksum1(tm, block_ksum(s, r.k));
+        return (double)k.s + (double)k.c;
+    }
+    __device__ __forceinline__ void part(Red r, double *o) {
+        const KSum k = block_ksum(s, r.k);
+        o[0] = (double)k.s;
+        o[1] = (double)k.c;
+    }
+    __device__ static double combine(const double *x, int size, int j) {
+        KSum t{(float)x[j], (float)x[j + 1]};
+        for (int k = 1; k < size; ++k)
+            t = ksum_add(t, KSum{(float)x[k * kXchgVals + j], (float)x[k * kXchgVals + j + 1]});
+        return (double)t.s + (double)t.c;
     }
 };
 
@@ -199,14 +267,170 @@ __device__ __forceinline__ double wg_dot(const T *a, const T *b, int n, Red r) {
 }
 
 template <int U = kU, typename T>
-__device__ __forceinline__ double wg_maxabs(const T *a, int n, Red r) {
+__device__ __forceinline__ double wg_maxabs_seg(const T *a, int i0, int i1, Red r) {
     double m = 0;
-    stream<U>(n, [&](int i) { return a[i]; }, [&](int, T v) {
+    stream<U>(i0, i1, [&](int i) { return a[i]; }, [&](int, T v) {
         const double f = fabs((double)v);
         m = f > m ? f : m;
     });
     return block_max(m, r.d);
 }
+
+template <int U = kU, typename T>
+__device__ __forceinline__ double wg_maxabs(const T *a, int n, Red r) {
+    return wg_maxabs_seg<U>(a, 0, n, r);
+}
+
+// ---- teams: several workgroups on one line ----
+// A line may be solved by a team of `size` workgroups, member k owning the
+// sites [k n / size, (k + 1) n / size). Decisions stay uniform across the
+// team: every workgroup reduces its own sites, publishes the partial, and
+// every member combines all partials in member order, so all members see the
+// same value.
+//
+// Data crosses workgroups (and XCDs, whose L2 caches are not coherent with
+// each other) only in 64-bit words written and read with device-scope
+// atomic stores and loads, each word holding a 32-bit tag (the team's
+// exchange count, the same on every member) beside 32 payload bits. A reader
+// polls each word until its tag is the one it waits for. Device-scope atomics
+// are coherent per location, so a word with the right tag carries the
+// payload written with it; no ordering between different words (which across
+// XCDs would need L2 write-backs and invalidations) is relied on. Every wait
+// is bounded: a member that gives up raises the launch's abort flag, after
+// which no member waits any more and the host reports the launch as failed.
+typedef unsigned long long u64;
+
+struct Team {
+    int size, member, s0, s1;
+    u64 *xchg;     // this line's small exchanges: [2][kMaxTeam][kXchgVals][2 words]
+    u64 *bulk;     // this line's vector exchange (preconditioner), 2n words
+    int *abort;    // launch-wide flag: a wait gave up
+    int epoch;     // exchanges on this line so far (the next tag is epoch + 1)
+    int *s_ok;     // LDS: 1 while no wait has failed
+    double *s_x;   // LDS [2][kMaxTeam * kXchgVals]: the last two small exchanges
+};
+
+// Polls of one wait before it gives up: each poll is a device-scope load
+// (~1 us) and a short sleep, so this is seconds, longer than any line's
+// whole minimisation (a member's first wait can last until its team is
+// complete, i.e. until another line's solve ends).
+constexpr long kTeamMaxPolls = 1L << 22;
+
+__device__ __forceinline__ u64 tag_word(unsigned tag, unsigned payload) {
+    return ((u64)tag << 32) | (u64)payload;
+}
+
+// The payload of word w once its tag is `tag`; `polls` counts this thread's
+// polls in the current wait; gives up (ok = false, abort raised) past the
+// limit or once any other wait of the launch gave up.
+__device__ __forceinline__ unsigned wait_word(const u64 *w, unsigned tag, long &polls, bool &ok,
+                                              int *abort) {
+    for (;;) {
+        const u64 u = ld_dev(w);
+        if ((unsigned)(u >> 32) == tag) return (unsigned)u;
+        if (!ok) return 0u;  // after a failed wait: zeros, never stale payloads
+        if (++polls > kTeamMaxPolls || ((polls & 63) == 0 && ld_dev(abort) != 0)) {
+            st_dev(abort, 1);
+            ok = false;
+            return 0u;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+__device__ __forceinline__ void put_f64(u64 *w, unsigned tag, double v) {
+    const u64 b = (u64)__double_as_longlong(v);
+    st_dev(w, tag_word(tag, (unsigned)b));
+    st_dev(w + 1, tag_word(tag, (unsigned)(b >> 32)));
+}
+
+__device__ __forceinline__ double get_f64(const u64 *w, unsigned tag, long &polls, bool &ok,
+                                          int *abort) {
+    const unsigned lo = wait_word(w, tag, polls, ok, abort);
+    const unsigned hi = wait_word(w + 1, tag, polls, ok, abort);
+    return __longlong_as_double((long long)(((u64)hi << 32) | lo));
+}
+
+// One line element of type T through the vector exchange (1 word for FP32,
+// 2 for FP64, element i at i or 2i).
+template <typename T>
+__device__ __forceinline__ void put_elem(u64 *xw, int i, unsigned tag, T v) {
+    if constexpr (sizeof(T) == 4) st_dev(xw + i, tag_word(tag, __float_as_uint(v)));
+    else put_f64(xw + 2 * (size_t)i, tag, v);
+}
+template <typename T>
+__device__ __forceinline__ T get_elem(const u64 *xw, int i, unsigned tag, long &polls, bool &ok,
+                                      int *abort) {
+    if constexpr (sizeof(T) == 4) return __uint_as_float(wait_word(xw + i, tag, polls, ok, abort));
+    else return get_f64(xw + 2 * (size_t)i, tag, polls, ok, abort);
+}
+
+// Every member's v[0..NV) of this exchange, for all threads of the
+// workgroup (in LDS, member k's value j at k * kXchgVals + j). Two slots
+// alternate, in memory and in LDS: a member publishes into a memory slot
+// only after it has read every member's values of the exchange before, which
+// every member publishes only after reading the slot's previous contents;
+// a wave writes an LDS slot only after every wave has passed the barrier of
+// the exchange before, i.e. finished reading the slot's previous contents.
+template <int NV>
+__device__ __forceinline__ const double *team_exchange(Team &tm, const double *v) {
+    const unsigned tag = (unsigned)++tm.epoch;
+    u64 *sl = tm.xchg + (size_t)(tag & 1u) * kMaxTeam * kXchgVals * 2;
+    double *out = tm.s_x + (tag & 1u) * kMaxTeam * kXchgVals;
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) put_f64(sl + 2 * (tm.member * kXchgVals + j), tag, v[j]);
+    if (threadIdx.x < kWave) {
+        bool ok = *tm.s_ok != 0;
+        long polls = 0;
+        for (int idx = threadIdx.x; idx < tm.size * NV; idx += kWave) {
+            const int k = idx / NV, j = idx % NV;
+            out[k * kXchgVals + j] = get_f64(sl + 2 * (k * kXchgVals + j), tag, polls, ok, tm.abort);
+        }
+        if (!ok) *tm.s_ok = 0;
+    }
+    __syncthreads();
+    return out;
+}
+
+// v[j] <- sum over members (in member order) of their v[j], j < NV.
+template <int NV>
+__device__ __forceinline__ void team_sum(Team &tm, double *v) {
+    if (tm.size == 1) return;
+    const double *x = team_exchange<NV>(tm, v);
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        double acc = x[j];
+        for (int k = 1; k < tm.size; ++k) acc += x[k * kXchgVals + j];
+        v[j] = acc;
+    }
+}
+
+__device__ __forceinline__ double team_max(Team &tm, double a) {
+    if (tm.size == 1) return a;
+    const double *x = team_exchange<1>(tm, &a);
+    double m = x[0];
+    for (int k = 1; k < tm.size; ++k) m = x[k * kXchgVals] > m ? x[k * kXchgVals] : m;
+    return m;
+}
+
+__device__ __forceinline__ double team_sum1(Team &tm, double v) {
+    team_sum<1>(tm, &v);
+    return v;
+}
+
+// Kahan pairs combined in member order
+__device__ __forceinline__ KSum team_ksum(Team &tm, KSum k) {
+    if (tm.size == 1) return k;
+    const double v[2] = {(double)k.s, (double)k.c};
+    const double *x = team_exchange<2>(tm, v);
+    KSum t{(float)x[0], (float)x[1]};
+    for (int m = 1; m < tm.size; ++m)
+        t = ksum_add(t, KSum{(float)x[m * kXchgVals], (float)x[m * kXchgVals + 1]});
+    return t;
+}
+
+__device__ __forceinline__ KSum team_ksum1(Team &tm, KSum k) { return team_ksum(tm, k); }
 
 // ---- objective and gradient of one line ----
 // table gathers in flight per thread: 16 coefficients each, held as FP64 for
@@ -214,16 +438,26 @@ __device__ __forceinline__ double wg_maxabs(const T *a, int n, Red r) {
 template <typename T>
 constexpr int kUG = sizeof(T) == 8 ? 2 : 4;
 
-template <typename T, typename C, bool MID = false>
-__device__ double wg_objective(const LineGeom &G, const T *x, const T *d, double alpha, Red r) {
+// A team member's x and d are private copies of its segment with one halo
+// site on each side: the pointers it passes index sites s0 - 1 .. s1 (its
+// left neighbour's last site, its own, its right neighbour's first), so the
+// right site of its edge i is at i + 1 even for the line's last edge. One
+// workgroup on the whole line indexes the wrapped neighbour ip.
+#define QEW_XR(i, ip) (x[TEAM ? (i) + 1 : (ip)])
+#define QEW_DR(i, ip) (d[TEAM ? (i) + 1 : (ip)])
+
+// This workgroup's share (sites [s0, s1)) of the energy at x + alpha d.
+template <typename T, typename C, bool MID = false, bool TEAM = false>
+__device__ double wg_objective(const LineGeom &G, const T *x, const T *d, double alpha, Red r,
+                               const Team &tm) {
     const int n = G.n;
     double acc = 0;
     if constexpr (sizeof(T) == 8) {
         // src/qew_model.h:53-99 (TrialAt: x0(i) + alpha * d(i))
-        stream<kUG<T>>(n, [&](int i) {
+        stream<kUG<T>>((TEAM ? tm.s0 : 0), (TEAM ? tm.s1 : n), [&](int i) {
                    const int ip = (i + 1 == n) ? 0 : i + 1;
                    return P2<double>{d ? x[i] + alpha * d[i] : x[i],
-                                     d ? x[ip] + alpha * d[ip] : x[ip]};
+                                     d ? QEW_XR(i, ip) + alpha * QEW_DR(i, ip) : QEW_XR(i, ip)};
                },
                [&](int i, P2<double> v) {
                    const int ip = (i + 1 == n) ? 0 : i + 1;
@@ -245,10 +479,11 @@ __device__ double wg_objective(const LineGeom &G, const T *x, const T *d, double
         // energy of the FP32 trial point (the same FP32 values the commit
         // writes) is evaluated in FP64 on the FP32 table.
         const float al = (float)alpha;
-        stream<kUG<T>>(n, [&](int i) {
+        stream<kUG<T>>((TEAM ? tm.s0 : 0), (TEAM ? tm.s1 : n), [&](int i) {
                    const int ip = (i + 1 == n) ? 0 : i + 1;
                    return P2<double>{d ? (double)__builtin_fmaf(al, d[i], x[i]) : (double)x[i],
-                                     d ? (double)__builtin_fmaf(al, d[ip], x[ip]) : (double)x[ip]};
+                                     d ? (double)__builtin_fmaf(al, QEW_DR(i, ip), QEW_XR(i, ip))
+                                       : (double)QEW_XR(i, ip)};
                },
                [&](int i, P2<double> v) {
                    const int ip = (i + 1 == n) ? 0 : i + 1;
@@ -288,9 +523,10 @@ constexpr int kUT = sizeof(T) == 8 ? 1 : 2;
 // against 5480 with four, whose spills grew from 224 to 400 B per lane).
 constexpr int kUTM = 2;
 
-template <typename T>
+template <typename T, bool TEAM>
 __device__ __forceinline__ void objective_trials_mid(const LineGeom &G, const T *x, const T *d,
-                                                     const double *alpha, double (&acc)[kLS]) {
+                                                     const double *alpha, double (&acc)[kLS],
+                                                     const Team &tm) {
     const int n = G.n;
     constexpr int K = kLS;
     // FP32 state: FP32 offsets from the anchor on an FP32 midpoint table;
@@ -311,11 +547,11 @@ __device__ __forceinline__ void objective_trials_mid(const LineGeom &G, const T 
         }
     };
     stream<kUTM>(
-        n,
+        (TEAM ? tm.s0 : 0), (TEAM ? tm.s1 : n),
         [&](int i) {
             const int ip = (i + 1 == n) ? 0 : i + 1;
             Item it;
-            it.v = P4<T>{x[i], d[i], x[ip], d[ip]};
+            it.v = P4<T>{x[i], d[i], QEW_XR(i, ip), QEW_DR(i, ip)};
             double oi, oip, fy;
             offsets(it.v, 0, oi, oip);
             int iyn[4];
@@ -357,13 +593,14 @@ __device__ __forceinline__ void objective_trials_mid(const LineGeom &G, const T 
         });
 }
 
-template <typename T, typename C, bool MID = false>
+template <typename T, typename C, bool MID = false, bool TEAM = false>
 __device__ __forceinline__ void objective_trials_table(const LineGeom &G, const T *x, const T *d,
-                                                       const double *alpha, double (&acc)[kLS]) {
+                                                       const double *alpha, double (&acc)[kLS],
+                                                       const Team &tm) {
     const int n = G.n;
     constexpr int K = kLS;
     if constexpr (MID) {
-        objective_trials_mid<T>(G, x, d, alpha, acc);
+        objective_trials_mid<T, TEAM>(G, x, d, alpha, acc, tm);
         return;
     }
     using B = typename std::conditional<sizeof(T) == 8, C, float>::type;  // block storage
@@ -385,11 +622,11 @@ __device__ __forceinline__ void objective_trials_table(const LineGeom &G, const 
         B c[4][4];
     };
     stream<kUT<T>>(
-        n,
+        (TEAM ? tm.s0 : 0), (TEAM ? tm.s1 : n),
         [&](int i) {
             const int ip = (i + 1 == n) ? 0 : i + 1;
             Item it;
-            it.v = P4<T>{x[i], d[i], x[ip], d[ip]};
+            it.v = P4<T>{x[i], d[i], QEW_XR(i, ip), QEW_DR(i, ip)};
             double hi, hip, fx, fy;
             heights(it.v, 0, hi, hip);
             int ixn[4], iyn[4];
@@ -455,25 +692,26 @@ __device__ __forceinline__ void objective_trials_table(const LineGeom &G, const 
 // Energies at the kLS trial points x + alpha[j] d in one pass over the line.
 // Each site term, each thread's summation order and the workgroup sum are
 // those of wg_objective, so phi[j] equals wg_objective(G, x, d, alpha[j]).
-template <typename T, typename C, bool MID = false>
+template <typename T, typename C, bool MID = false, bool TEAM = false>
 __device__ __forceinline__ void wg_objective_trials(const LineGeom &G, const T *x, const T *d,
-                                    const double *alpha, double *phi, Red r) {
+                                    const double *alpha, double *phi, Red r, Team &tm) {
     const int n = G.n;
     constexpr int K = kLS;
     double acc[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) acc[j] = 0;
     if (G.nd.kind == TABLE) {
-        objective_trials_table<T, C, MID>(G, x, d, alpha, acc);
+        objective_trials_table<T, C, MID, TEAM>(G, x, d, alpha, acc, tm);
         block_sumk<K>(acc, r.d);
+        if constexpr (TEAM) team_sum<K>(tm, acc);
 #pragma unroll
         for (int j = 0; j < K; ++j) phi[j] = acc[j];
         return;
     }
     stream<1>(
-        n, [&](int i) {
+        (TEAM ? tm.s0 : 0), (TEAM ? tm.s1 : n), [&](int i) {
             const int ip = (i + 1 == n) ? 0 : i + 1;
-            return P4<T>{x[i], d[i], x[ip], d[ip]};
+            return P4<T>{x[i], d[i], QEW_XR(i, ip), QEW_DR(i, ip)};
         },
         [&](int i, P4<T> v) {
             const int ip = (i + 1 == n) ? 0 : i + 1;
@@ -502,50 +740,60 @@ __device__ __forceinline__ void wg_objective_trials(const LineGeom &G, const T *
             }
         });
     block_sumk<K>(acc, r.d);
+    if constexpr (TEAM) team_sum<K>(tm, acc);
 #pragma unroll
     for (int j = 0; j < K; ++j) phi[j] = acc[j];
 }
 
-template <typename T, typename C, bool MID = false>
-__device__ __forceinline__ void wg_gradient(const LineGeom &G, const T *x, T *g, T *edy, T *esl) {
+// Gradient at the sites [s0, s1) of the line (the whole line for one
+// workgroup). Edge pass: dV/dy (and the arclength slope) of the edges (i,
+// i + 1); a team member also forms its left halo edge (s0 - 1, s0) from its
+// halo site, into its private edy / esl copies (indexed like x, from s0 - 1).
+// Site pass: g(i) from the edges i - 1 and i.
+template <typename T, typename C, bool MID = false, bool TEAM = false>
+__device__ __forceinline__ void wg_gradient(const LineGeom &G, const T *x, T *g, T *edy, T *esl,
+                                            const Team &tm) {
     const int n = G.n;
     const bool arc = G.model == 1;
-    stream<kUG<T>>(n, [&](int i) {
-               const int ip = (i + 1 == n) ? 0 : i + 1;
-               return P2<T>{x[i], x[ip]};
+    const int s0 = TEAM ? tm.s0 : 0, s1 = TEAM ? tm.s1 : n;
+    stream<kUG<T>>(TEAM ? s0 - 1 : s0, s1, [&](int e) {
+               const int ip = (e + 1 == n) ? 0 : e + 1;
+               return P2<T>{x[e], QEW_XR(e, ip)};
            },
-           [&](int i, P2<T> v) {
+           [&](int e, P2<T> v) {
+               // the edge's place on the line (e = -1 is the wrap edge n - 1)
+               const int i = e < 0 ? n - 1 : e;
                const int ip = (i + 1 == n) ? 0 : i + 1;
                if constexpr (sizeof(T) == 8) {
                    const double dh = div1(v.b - v.a, G.dx);
                    if constexpr (MID)
-                       edy[i] = mid_dy64<double>(G.nd, mid_column(G.nd, i, ip), (v.b + v.a) * 0.5);
+                       edy[e] = mid_dy64<double>(G.nd, mid_column(G.nd, i, ip), (v.b + v.a) * 0.5);
                    else
-                       edy[i] = sample_dy64<C>(G.nd, (double)(i + ip) * 0.5 * G.dx, (v.b + v.a) * 0.5);
-                   if (arc) esl[i] = dh / sqrt(1.0 + dh * dh);
+                       edy[e] = sample_dy64<C>(G.nd, (double)(i + ip) * 0.5 * G.dx, (v.b + v.a) * 0.5);
+                   if (arc) esl[e] = dh / sqrt(1.0 + dh * dh);
                } else {
                    float vv, dy;
                    if constexpr (MID) edge32_mid<false, true>(G, i, ip, v.a, v.b, vv, dy);
                    else edge32_grid<false, true>(G, i, ip, v.a, v.b, vv, dy);
-                   edy[i] = dy;
+                   edy[e] = dy;
                    const float dh = (v.b - v.a) * G.inv_dx;
-                   if (arc) esl[i] = dh * rsqrtf(1.0f + dh * dh);
+                   if (arc) esl[e] = dh * rsqrtf(1.0f + dh * dh);
                }
            });
     __syncthreads();
     struct Nb {
         T xm, xi, xp, em, ei, sm, si;
     };
-    stream<4>(n, [&](int i) {
+    stream<4>(s0, s1, [&](int i) {
                const int ip = (i + 1 == n) ? 0 : i + 1;
-               const int im = (i == 0) ? n - 1 : i - 1;
+               const int im = TEAM ? i - 1 : (i == 0) ? n - 1 : i - 1;
                Nb b;
                b.em = edy[im];
                b.ei = edy[i];
                if (!arc) {
                    b.xm = x[im];
                    b.xi = x[i];
-                   b.xp = x[ip];
+                   b.xp = QEW_XR(i, ip);
                    b.sm = b.si = 0;
                } else {
                    b.sm = esl[im];
@@ -745,9 +993,63 @@ __device__ void wg_precondition(int n, double kdx, double shift, C2<T> *buf, con
     __syncthreads();
 }
 
-template <typename T, typename C, bool MID = false, bool MIXED = false>
+// Team version of wg_precondition: the members send their q sites through
+// the line's vector exchange, the leader (member 0) receives the rest of the
+// line into its LDS buffer, transforms it and sends back the others' sites of
+// d, which they receive into their own buffers.
+template <typename T, bool MIXED>
+__device__ __forceinline__ void team_precondition(Team &tm, int n, double kdx, double shift,
+                                                  C2<T> *buf, const C2<T> *tw, int M) {
+    T *fb = reinterpret_cast<T *>(buf);
+    const unsigned tag_q = (unsigned)++tm.epoch, tag_d = (unsigned)++tm.epoch;
+    bool ok = *tm.s_ok != 0;
+    long polls = 0;
+    if (tm.member == 0) {
+        for (int i = tm.s1 + (int)threadIdx.x; i < n; i += kLB)
+            fb[i] = get_elem<T>(tm.bulk, i, tag_q, polls, ok, tm.abort);
+        if (!ok) *tm.s_ok = 0;
+        __syncthreads();
+        wg_precondition<T, MIXED>(n, kdx, shift, buf, tw, M);
+        for (int i = tm.s1 + (int)threadIdx.x; i < n; i += kLB) put_elem<T>(tm.bulk, i, tag_d, fb[i]);
+    } else {
+        for (int i = tm.s0 + (int)threadIdx.x; i < tm.s1; i += kLB)
+            put_elem<T>(tm.bulk, i, tag_q, fb[i]);
+        for (int i = tm.s0 + (int)threadIdx.x; i < tm.s1; i += kLB)
+            fb[i] = get_elem<T>(tm.bulk, i, tag_d, polls, ok, tm.abort);
+        if (!ok) *tm.s_ok = 0;
+        __syncthreads();
+    }
+}
+
+// The first and last site values of d and g of every member, exchanged so
+// that each member holds its neighbours' d (the halo of its boundary edges)
+// and g (for a restart with d = -g).
+template <typename T>
+__device__ __forceinline__ void team_halo_dg(Team &tm, T *d, const T *g, T *hal_g) {
+    const double v[4] = {(double)d[tm.s0], (double)d[tm.s1 - 1], (double)g[tm.s0],
+                         (double)g[tm.s1 - 1]};
+    const double *x = team_exchange<4>(tm, v);
+    if (threadIdx.x == 0) {
+        const double *lft = x + ((tm.member + tm.size - 1) % tm.size) * kXchgVals;
+        const double *rgt = x + ((tm.member + 1) % tm.size) * kXchgVals;
+        d[tm.s0 - 1] = (T)lft[1];
+        d[tm.s1] = (T)rgt[0];
+        hal_g[0] = (T)lft[3];
+        hal_g[1] = (T)rgt[2];
+    }
+    __syncthreads();
+}
+
+// Sites [s0, s1) of one line, solved by one workgroup (TEAM false: the
+// whole line) or as member tm.member of a team. A member's x, d, edy and esl
+// are private copies of its segment with a halo site on each side (each
+// line's workspace vectors have 2 kMaxTeam spare elements for them); it forms
+// its neighbours' x updates itself (x + alpha d, the same arithmetic as
+// theirs, so the same values) and gets their d (and g, in hal_g = {g[s0 - 1],
+// g[s1]}, for a restart with d = -g) at one exchange per iteration.
+template <typename T, typename C, bool MID = false, bool MIXED = false, bool TEAM = false>
 __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, double *rho, double *la,
-                           int *slot, int *sbuf, C2<T> *fftbuf) {
+                           int *slot, int *sbuf, C2<T> *fftbuf, Team &tm, T *hal_g) {
     // elements per thread with loads in flight in the vector passes: 16 for
     // the FP32 midpoint-table kernel (its line search frees the registers), 8
     // elsewhere (wider passes spill); the per-thread order is the same
@@ -758,21 +1060,49 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
     const int n = L.n;
     const int m = lp.m;
     const int tid = threadIdx.x;
+    const int s0 = TEAM ? tm.s0 : 0, s1 = TEAM ? tm.s1 : n;
+    constexpr bool team = TEAM;
+    // this workgroup's share of a sum, combined over the team
+    auto red = [&](Acc<T> &acc) {
+        if constexpr (TEAM) return acc.reduce(r, tm);
+        else return acc.reduce(r);
+    };
     const C2<T> *tw = static_cast<const C2<T> *>(a.tw);
     if constexpr (MIXED)
         if (lp.precondition) tw += a.tw_off[line];
     double *h = a.h + L.off;
     const double anchor = (sizeof(T) == 8) ? 0.0 : h[0];
     const LineGeom G = make_geom(L, nd, anchor);
-    // per-line workspace: x g gn d q edy esl, then (m + 1) S and (m + 1) Y buffers
-    T *ws = static_cast<T *>(a.work) + (size_t)L.off * (size_t)(7 + 2 * (m + 1));
-    T *x = ws, *g = ws + n, *gn = ws + 2 * (size_t)n, *d = ws + 3 * (size_t)n,
-      *q = ws + 4 * (size_t)n, *edy = ws + 5 * (size_t)n, *esl = ws + 6 * (size_t)n;
-    auto Sb = [&](int k) { return ws + (size_t)(7 + k) * n; };
-    auto Yb = [&](int k) { return ws + (size_t)(7 + (m + 1) + k) * n; };
-    for (int i = tid; i < n; i += blockDim.x) x[i] = (T)(h[i] - anchor);
-    if (tid == 0)
+    // per-line workspace: x g gn d q edy esl, (m + 1) S and (m + 1) Y
+    // buffers and two for a team's vector exchange, each of n + 2 kMaxTeam
+    // elements (lines back to back, in batch order); a team member's x, d,
+    // edy, esl start at its private block
+    const size_t np = (size_t)n + 2 * kMaxTeam;
+    T *ws = static_cast<T *>(a.work) +
+            ((size_t)L.off + 2 * (size_t)kMaxTeam * (size_t)line) * (size_t)lbfgs_nvec(m);
+    T *x = ws, *g = ws + np, *gn = ws + 2 * np, *d = ws + 3 * np, *q = ws + 4 * np,
+      *edy = ws + 5 * np, *esl = ws + 6 * np;
+    if constexpr (TEAM) {
+        const ptrdiff_t pb = (ptrdiff_t)tm.member * (tm.s1 - tm.s0 + 2) + 1 - tm.s0;
+        x += pb;
+        d += pb;
+        edy += pb;
+        esl += pb;
+        // 8-byte aligned words in the last two vectors: n words for FP32
+        // elements, 2n for FP64 (the two vectors hold 2 np elements)
+        const uintptr_t bw = reinterpret_cast<uintptr_t>(ws + (size_t)(lbfgs_nvec(m) - 2) * np);
+        tm.bulk = reinterpret_cast<u64 *>((bw + 7) & ~(uintptr_t)7);
+    }
+    auto Sb = [&](int k) { return ws + (size_t)(7 + k) * np; };
+    auto Yb = [&](int k) { return ws + (size_t)(7 + (m + 1) + k) * np; };
+    for (int i = s0 + tid; i < s1; i += blockDim.x) x[i] = (T)(h[i] - anchor);
+    if (tid == 0) {
         for (int k = 0; k <= m; ++k) sbuf[k] = k;  // sbuf[m] is the spare (s_tmp / y_tmp)
+        if (team) {
+            x[s0 - 1] = (T)(h[(s0 + n - 1) % n] - anchor);
+            x[s1] = (T)(h[s1 % n] - anchor);
+        }
+    }
     __syncthreads();
 
     double kdx = 0, shift = 0;
@@ -781,12 +1111,12 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
         kdx = L.lt / G.dx;
         shift = L.shift > 0 ? L.shift : lp.precond_shift;
         if (shift <= 0) {
-            const double s1 = sin(pi / (double)n);
-            shift = kdx * 4 * s1 * s1;
+            const double s1n = sin(pi / (double)n);
+            shift = kdx * 4 * s1n * s1n;
         }
     }
 
-    wg_gradient<T, C, MID>(G, x, g, edy, esl);
+    wg_gradient<T, C, MID, TEAM>(G, x, g, edy, esl, tm);
     qew_lbfgs_result res{};
     int nslots = 0;  // slot[0..nslots) chronological rows, slot[nslots-1] newest
     double gamma = 1.0;
@@ -794,6 +1124,12 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
     double phi = 0;
     bool finished = false;
     for (int it = 0; it < lp.max_iter; ++it) {
+        if (TEAM && !*tm.s_ok) {  // a team wait timed out: the launch reports failure
+            res.iterations = it;
+            res.converged = 0;
+            finished = true;
+            break;
+        }
         const int hs = nslots;
         // With the preconditioner, q and then d live in the LDS FFT buffer
         // (n reals) from the first pass to the second loop's last pass, which
@@ -807,7 +1143,7 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
         auto first_pass = [&](T *qv) {
             const T *S0 = hs > 0 ? Sb(sbuf[slot[hs - 1]]) : nullptr;
             Acc<T> acc;
-            stream<UV>(n, [&](int i) { return P2<T>{g[i], S0 ? S0[i] : (T)0}; },
+            stream<UV>(s0, s1, [&](int i) { return P2<T>{g[i], S0 ? S0[i] : (T)0}; },
                    [&](int i, P2<T> v) {
                        const double f = fabs((double)v.a);
                        gmax = f > gmax ? f : gmax;
@@ -815,7 +1151,19 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
                        if (S0) acc.add(v.b * v.a);
                    });
             gmax = block_max(gmax, r.d);
-            if (S0) al_dot = acc.reduce(r);
+            if constexpr (!TEAM) {
+                if (S0) al_dot = acc.reduce(r);
+            } else {
+                double o[3] = {gmax, 0.0, 0.0};
+                acc.part(r, o + 1);
+                const double *xv = team_exchange<3>(tm, o);
+                gmax = xv[0];
+                for (int k = 1; k < tm.size; ++k) {
+                    const double gk = xv[k * kXchgVals];
+                    gmax = gk > gmax ? gk : gmax;
+                }
+                if (S0) al_dot = Acc<T>::combine(xv, tm.size, 1);
+            }
         };
         if (pre) first_pass(fb);
         else first_pass(q);
@@ -837,23 +1185,26 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
                 const T *Y = Yb(sbuf[row]);
                 const T *Sn = idx > 0 ? Sb(sbuf[slot[idx - 1]]) : nullptr;
                 Acc<T> acc;
-                stream<UV>(n, [&](int i) { return P3<T>{qv[i], Y[i], Sn ? Sn[i] : (T)0}; },
+                stream<UV>(s0, s1, [&](int i) { return P3<T>{qv[i], Y[i], Sn ? Sn[i] : (T)0}; },
                        [&](int i, P3<T> v) {
                            const T qn = v.a + c * v.b;
                            qv[i] = qn;
                            if (Sn) acc.add(v.c * qn);
                        });
-                if (Sn) al_dot = acc.reduce(r);
+                if (Sn) al_dot = red(acc);
                 else __syncthreads();
             }
         };
         if (pre) {
             first_loop(fb);
-            wg_precondition<T, MIXED>(n, kdx, shift, fftbuf, tw, MIXED ? 2 * n : a.tw_m);
+            if constexpr (TEAM)
+                team_precondition<T, MIXED>(tm, n, kdx, shift, fftbuf, tw, MIXED ? 2 * n : a.tw_m);
+            else
+                wg_precondition<T, MIXED>(n, kdx, shift, fftbuf, tw, MIXED ? 2 * n : a.tw_m);
         } else {
             first_loop(q);
             const T g0 = (T)(nslots == 0 ? 1.0 : gamma);
-            stream<UV>(n, [&](int i) { return q[i]; }, [&](int i, T v) { d[i] = g0 * v; });
+            stream<UV>(s0, s1, [&](int i) { return q[i]; }, [&](int i, T v) { d[i] = g0 * v; });
             __syncthreads();
         }
         // second loop, oldest pair first: d += (la - b) S with b = rho Y . d;
@@ -861,7 +1212,13 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
         double dphi0 = 0;
         auto second_loop = [&](T *dv) {
             double b_dot = 0;
-            if (hs > 0) b_dot = wg_dot<UV>(Yb(sbuf[slot[0]]), dv, n, r);
+            if (hs > 0) {
+                const T *Y0 = Yb(sbuf[slot[0]]);
+                Acc<T> acc;
+                stream<UV>(s0, s1, [&](int i) { return P2<T>{Y0[i], dv[i]}; },
+                           [&](int, P2<T> v) { acc.add(v.a * v.b); });
+                b_dot = red(acc);
+            }
             for (int idx = 0; idx < hs; ++idx) {
                 const int row = slot[idx];
                 const double bb = rho[row] * b_dot;
@@ -873,7 +1230,7 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
                 // separate call sites for the two destinations keep each
                 // store in its own address space
                 auto pass = [&](T *dout, bool neg) {
-                    stream<UV>(n, [&](int i) { return P3<T>{dv[i], S[i], Yn[i]}; },
+                    stream<UV>(s0, s1, [&](int i) { return P3<T>{dv[i], S[i], Yn[i]}; },
                            [&](int i, P3<T> v) {
                                T dn = v.a + c * v.b;
                                if (neg) dn = -dn;
@@ -883,39 +1240,50 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
                 };
                 if (last) pass(d, true);
                 else pass(dv, false);
-                const double s = acc.reduce(r);
+                const double s = red(acc);
                 if (last) dphi0 = s;
                 else b_dot = s;
             }
             if (hs == 0) {
                 Acc<T> acc;
-                stream<UV>(n, [&](int i) { return P2<T>{dv[i], g[i]}; },
+                stream<UV>(s0, s1, [&](int i) { return P2<T>{dv[i], g[i]}; },
                        [&](int i, P2<T> v) {
                            const T dn = -v.a;
                            d[i] = dn;
                            acc.add(v.b * dn);
                        });
-                dphi0 = acc.reduce(r);
+                dphi0 = red(acc);
             }
         };
         if (pre) second_loop(fb);
         else second_loop(d);
+        if constexpr (TEAM) team_halo_dg(tm, d, g, hal_g);
         if (dphi0 >= 0) {
             nslots = 0;
             Acc<T> acc;
-            stream<UV>(n, [&](int i) { return g[i]; }, [&](int i, T v) {
+            stream<UV>(s0, s1, [&](int i) { return g[i]; }, [&](int i, T v) {
                 d[i] = -v;
                 acc.add(v * v);
             });
-            dphi0 = -acc.reduce(r);
+            dphi0 = -red(acc);
+            if (team && tid == 0) {
+                d[s0 - 1] = -hal_g[0];
+                d[s1] = -hal_g[1];
+            }
+            __syncthreads();
         }
         if (it == 0) {
-            phi = wg_objective<T, C, MID>(G, x, nullptr, 0.0, r);
+            phi = wg_objective<T, C, MID, TEAM>(G, x, nullptr, 0.0, r, tm);
+            if constexpr (TEAM) phi = team_sum1(tm, phi);
             ++res.n_func_evals;
         }
         const double phi0 = phi;
         double alpha = 1.0;
-        if (!lp.precondition && nslots == 0) alpha = 1.0 / fmax(1.0, sqrt(wg_dot<UV>(g, g, n, r)));
+        if (!lp.precondition && nslots == 0) {
+            Acc<T> acc;
+            stream<UV>(s0, s1, [&](int i) { return g[i]; }, [&](int, T v) { acc.add(v * v); });
+            alpha = 1.0 / fmax(1.0, sqrt(red(acc)));
+        }
         // Armijo backtracking alpha, alpha s, alpha s^2, ... (the sequence and
         // the first accepted step of the reference loop); the energies of K
         // consecutive trials come from one pass, the evaluation count is the
@@ -927,7 +1295,7 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
             al[0] = alpha;
 #pragma unroll
             for (int j = 1; j < K; ++j) al[j] = al[j - 1] * lp.ls_shrink;
-            wg_objective_trials<T, C, MID>(G, x, d, al, ph, r);
+            wg_objective_trials<T, C, MID, TEAM>(G, x, d, al, ph, r, tm);
 #pragma unroll
             for (int j = 0; j < K; ++j) {
                 if (ok || ls >= lp.max_ls) continue;
@@ -962,28 +1330,36 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
         // near convergence alpha d falls below the offsets' resolution and
         // the intended step no longer matches the realised one.
         T *st = Sb(sbuf[m]), *yt = Yb(sbuf[m]);
-        stream<UV>(n, [&](int i) { return P2<T>{x[i], d[i]}; }, [&](int i, P2<T> v) {
+        stream<UV>(s0, s1, [&](int i) { return P2<T>{x[i], d[i]}; }, [&](int i, P2<T> v) {
             const T xn = v.a + alT * v.b;
             x[i] = xn;
             st[i] = (sizeof(T) == 8) ? alT * v.b : xn - v.a;
         });
+        if (team && tid == 0) {
+            // the neighbours' boundary sites take the same step
+            x[s0 - 1] = x[s0 - 1] + alT * d[s0 - 1];
+            x[s1] = x[s1] + alT * d[s1];
+        }
         __syncthreads();
-        wg_gradient<T, C, MID>(G, x, gn, edy, esl);
-        double sy = 0, ss = 0, yy = 0;
-        stream<UV>(n, [&](int i) { return P3<T>{st[i], gn[i], g[i]}; }, [&](int i, P3<T> v) {
+        wg_gradient<T, C, MID, TEAM>(G, x, gn, edy, esl, tm);
+        double sy3[3] = {0, 0, 0};
+        stream<UV>(s0, s1, [&](int i) { return P3<T>{st[i], gn[i], g[i]}; }, [&](int i, P3<T> v) {
             const T y = v.b - v.c;
             yt[i] = y;
-            sy += (double)v.a * (double)y;
-            ss += (double)v.a * (double)v.a;
-            yy += (double)y * (double)y;
+            sy3[0] += (double)v.a * (double)y;
+            sy3[1] += (double)v.a * (double)v.a;
+            sy3[2] += (double)y * (double)y;
         });
-        block_sum3(sy, ss, yy, r.d);
+        block_sum3(sy3[0], sy3[1], sy3[2], r.d);
+        if constexpr (TEAM) team_sum<3>(tm, sy3);
+        const double sy = sy3[0], ss = sy3[1], yy = sy3[2];
         if (sizeof(T) == 4 && ss == 0.0) {
             // FP32 state: the accepted step is below the offsets' resolution
             // everywhere, so the line can no longer move; stop (not
             // converged) instead of iterating in place until max_iter
             res.iterations = it + 1;
-            res.max_force = wg_maxabs<UV>(gn, n, r);
+            res.max_force = wg_maxabs_seg<UV>(gn, s0, s1, r);
+            if constexpr (TEAM) res.max_force = team_max(tm, res.max_force);
             res.converged = res.max_force < lp.ftol ? 1 : 0;
             g = gn;
             finished = true;
@@ -1020,11 +1396,12 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
     }
     if (!finished) {
         res.iterations = lp.max_iter;
-        res.max_force = wg_maxabs<UV>(g, n, r);
+        res.max_force = wg_maxabs_seg<UV>(g, s0, s1, r);
+        if constexpr (TEAM) res.max_force = team_max(tm, res.max_force);
         res.converged = 0;
     }
-    for (int i = tid; i < n; i += blockDim.x) h[i] = anchor + (double)x[i];
-    if (tid == 0) a.results[line] = res;
+    for (int i = s0 + tid; i < s1; i += blockDim.x) h[i] = anchor + (double)x[i];
+    if (tid == 0 && (!TEAM || tm.member == 0)) a.results[line] = res;
     __syncthreads();
 }
 
@@ -1035,16 +1412,42 @@ __global__ __launch_bounds__(kLB) void lbfgs_batch_kernel(BatchArgs a) {
     __shared__ KSum s_kred[kLB / kWave];
     __shared__ double rho[kMaxM], la[kMaxM];
     __shared__ int slot[kMaxM], sbuf[kMaxM + 1];
-    __shared__ int s_line;
+    __shared__ T s_hal_g[2];
+    __shared__ double s_x[2 * kMaxTeam * kXchgVals];
+    __shared__ int s_slot, s_ok;
     Red r{s_red, s_kred};
     for (;;) {
-        if (threadIdx.x == 0) s_line = (int)atomicAdd(a.next, 1u);
+        if (threadIdx.x == 0) s_slot = (int)atomicAdd(a.next, 1u);
         __syncthreads();
-        const int qi = s_line;
+        // wave-uniform values in scalar registers: the slot, its line and
+        // team, and everything indexed by them
+        const int si = __builtin_amdgcn_readfirstlane(s_slot);
         __syncthreads();
-        if (qi >= a.nlines) return;
-        solve_line<T, C, MID, MIXED>(a, a.order[qi], r, rho, la, slot, sbuf,
-                                     reinterpret_cast<C2<T> *>(lds));
+        if (si >= a.nslots) return;
+        // slot si: member slot_member[si] of the team of team_size[line]
+        // workgroups that solves line slot_line[si]
+        const int line = __builtin_amdgcn_readfirstlane(a.slot_line[si]);
+        const int member = __builtin_amdgcn_readfirstlane(a.slot_member[si]);
+        const int size = __builtin_amdgcn_readfirstlane(a.team_size[line]);
+        const int seg = __builtin_amdgcn_readfirstlane(a.lines[line].n) / size;
+        Team tm{size,
+                member,
+                member * seg,
+                (member + 1) * seg,
+                a.xchg + (size_t)line * 2 * kMaxTeam * kXchgVals * 2,
+                nullptr,  // the vector exchange, in the line's workspace
+                a.abort,
+                0,
+                &s_ok,
+                s_x};
+        if (threadIdx.x == 0) s_ok = 1;
+        __syncthreads();
+        if (size == 1)
+            solve_line<T, C, MID, MIXED, false>(a, line, r, rho, la, slot, sbuf,
+                                                reinterpret_cast<C2<T> *>(lds), tm, s_hal_g);
+        else
+            solve_line<T, C, MID, MIXED, true>(a, line, r, rho, la, slot, sbuf,
+                                               reinterpret_cast<C2<T> *>(lds), tm, s_hal_g);
     }
 }
 
@@ -1059,6 +1462,52 @@ __global__ void twiddle_kernel(C2<T> *tw, int M) {
 }
 
 __global__ void zero_u32(unsigned *p) { *p = 0u; }
+__global__ void zero_u32_n(unsigned *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// Workgroups per line in the batched L-BFGS, from a policy of (minimum
+// length, team size) rows, longest first: a line takes the team size of the
+// first row whose minimum it reaches (1 if none), if that size divides it
+// into segments of at least 64 sites. QEW_LBFGS_TEAM=k (read per call) sets
+// k for every line; QEW_LBFGS_TEAMS="n1:k1,n2:k2,..." sets the rows.
+struct TeamPolicy {
+    std::vector<std::pair<int, int>> rows;  // (min length, team size), decreasing length
+    int forced = 0;
+};
+
+TeamPolicy lbfgs_team_policy() {
+    TeamPolicy p;
+    p.rows = {{16384, 4}, {8192, 2}};
+    if (const char *e = std::getenv("QEW_LBFGS_TEAM")) p.forced = std::atoi(e);
+    if (const char *e = std::getenv("QEW_LBFGS_TEAMS")) {
+        p.rows.clear();
+        const char *c = e;
+        while (*c) {
+            char *end = nullptr;
+            const long len = std::strtol(c, &end, 10);
+            if (end == c || *end != ':') break;
+            const long k = std::strtol(end + 1, &end, 10);
+            p.rows.emplace_back((int)len, (int)k);
+            c = (*end == ',') ? end + 1 : end;
+            if (!*end) break;
+        }
+        std::sort(p.rows.begin(), p.rows.end(),
+                  [](const std::pair<int, int> &x, const std::pair<int, int> &y) {
+                      return x.first > y.first;
+                  });
+    }
+    return p;
+}
+
+int lbfgs_team_size(int n, const TeamPolicy &p) {
+    auto fits = [&](int k) { return k >= 1 && k <= kMaxTeam && n % k == 0 && n / k >= 64; };
+    if (p.forced > 0) return fits(p.forced) ? p.forced : 1;
+    for (const auto &row : p.rows)
+        if (n >= row.first) return fits(row.second) ? row.second : 1;
+    return 1;
+}
 
 // ---- FIRE (src/fire.h:41-131), one workgroup per line ----
 // The reference's passes per iteration are max|F|; v += dt F; P = F.v;
@@ -1090,6 +1539,8 @@ __device__ __forceinline__ void fire_line(const FireArgs &a, int line, Red r) {
     double *x = a.h + L.off;
     double *ws = a.work + (size_t)L.off * 4;
     double *g = ws, *v = ws + n, *edy = ws + 2 * (size_t)n, *esl = ws + 3 * (size_t)n;
+    // one workgroup per line: a team of one, no halo
+    const Team solo{1, 0, 0, n, nullptr, nullptr, nullptr, 0, nullptr};
     // every pass maps element i to thread i mod kLB, so element-wise passes
     // need no barrier between them; the gradient reads neighbours
     stream(n, [&](int) { return 0; }, [&](int i, int) { v[i] = 0.0; });
@@ -1097,7 +1548,7 @@ __device__ __forceinline__ void fire_line(const FireArgs &a, int line, Red r) {
     double alpha = fp.alpha_start;
     int n_pos = 0;
     __syncthreads();
-    wg_gradient<double, C>(G, x, g, edy, esl);
+    wg_gradient<double, C>(G, x, g, edy, esl, solo);
     qew_fire_result res{};
     res.iterations = fp.max_iter;
     res.converged = 0;
@@ -1151,7 +1602,7 @@ __device__ __forceinline__ void fire_line(const FireArgs &a, int line, Red r) {
                    x[i] = e.c + dt * vi;
                });
         __syncthreads();
-        wg_gradient<double, C>(G, x, g, edy, esl);
+        wg_gradient<double, C>(G, x, g, edy, esl, solo);
     }
     if (!done) res.max_force = wg_maxabs(g, n, r);
     if (threadIdx.x == 0) a.results[line] = res;
@@ -1176,8 +1627,10 @@ __global__ __launch_bounds__(kLB) void fire_batch_kernel(FireArgs a) {
 
 // Launches one persistent minimisation kernel: as many workgroups as fit at
 // once (each takes lines from the queue), at most one per line.
+// Launches a persistent kernel with as many workgroups as fit on the device at
+// once (at most one per line); returns that number.
 template <class K>
-void launch_persistent(K kernel, const BatchArgs &a, int nlines, size_t lds, qew_ctx ctx) {
+int launch_persistent(K kernel, const BatchArgs &a, int nlines, size_t lds, qew_ctx ctx) {
     if (lds > 64 * 1024)
         QEW_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -1186,6 +1639,7 @@ void launch_persistent(K kernel, const BatchArgs &a, int nlines, size_t lds, qew
     const int grid = std::max(1, std::min(nlines, std::max(1, per_cu) * ctx->num_cus));
     hipLaunchKernelGGL(kernel, dim3(grid), dim3(kLB), lds, ctx->stream, a);
     QEW_HIP(hipGetLastError());
+    return grid;
 }
 
 }  // namespace
@@ -1199,20 +1653,20 @@ namespace qew_host {
 // `args` is a BatchArgs of this file; variant 0..4: FP64 state on FP64
 // midpoint tables, FP32 on FP32 midpoint tables, FP32 on FP32 tables, FP64 on
 // FP32 tables, FP64 on FP64 tables.
-void launch_lbfgs_mixed(int variant, const void *args, int nlines, size_t lds, qew_ctx ctx);
+int launch_lbfgs_mixed(int variant, const void *args, int nlines, size_t lds, qew_ctx ctx);
 }  // namespace qew_host
 
 #ifdef QEW_LBFGS_MIXED_UNIT
 
-void qew_host::launch_lbfgs_mixed(int variant, const void *args, int nlines, size_t lds,
-                                  qew_ctx ctx) {
+int qew_host::launch_lbfgs_mixed(int variant, const void *args, int nlines, size_t lds,
+                                 qew_ctx ctx) {
     const BatchArgs &a = *static_cast<const BatchArgs *>(args);
     switch (variant) {
-        case 0: launch_persistent(lbfgs_batch_kernel<double, double, true, true>, a, nlines, lds, ctx); break;
-        case 1: launch_persistent(lbfgs_batch_kernel<float, float, true, true>, a, nlines, lds, ctx); break;
-        case 2: launch_persistent(lbfgs_batch_kernel<float, float, false, true>, a, nlines, lds, ctx); break;
-        case 3: launch_persistent(lbfgs_batch_kernel<double, float, false, true>, a, nlines, lds, ctx); break;
-        default: launch_persistent(lbfgs_batch_kernel<double, double, false, true>, a, nlines, lds, ctx); break;
+        case 0: return launch_persistent(lbfgs_batch_kernel<double, double, true, true>, a, nlines, lds, ctx);
+        case 1: return launch_persistent(lbfgs_batch_kernel<float, float, true, true>, a, nlines, lds, ctx);
+        case 2: return launch_persistent(lbfgs_batch_kernel<float, float, false, true>, a, nlines, lds, ctx);
+        case 3: return launch_persistent(lbfgs_batch_kernel<double, float, false, true>, a, nlines, lds, ctx);
+        default: return launch_persistent(lbfgs_batch_kernel<double, double, false, true>, a, nlines, lds, ctx);
     }
 }
 
@@ -1261,7 +1715,7 @@ extern "C" qew_status qew_batch_lbfgs(qew_batch b, int dtype, const qew_lbfgs_pa
         qew_ctx ctx = b->ctx;
         use_device(ctx);
         const size_t es = dtype == QEW_F32 ? sizeof(float) : sizeof(double);
-        const size_t nvec = 7 + 2 * ((size_t)lp->m + 1);
+        const size_t nvec = (size_t)lbfgs_nvec(lp->m);
         int maxN = 1;
         if (lp->precondition)
             for (const auto &l : b->lines) maxN = std::max(maxN, l.n / 2);
@@ -1291,23 +1745,53 @@ extern "C" qew_status qew_batch_lbfgs(qew_batch b, int dtype, const qew_lbfgs_pa
                     tw_off[l] = it->second;
                 }
             }
-        // layout: [order nlines ints][counter][results][table offsets][twiddles][work]
-        auto up = [](size_t v) { return (v + 255) / 256 * 256; };
-        const size_t off_next = up(sizeof(int) * b->nlines);
-        const size_t off_res = off_next + 256;
-        const size_t off_twoff = off_res + up(sizeof(qew_lbfgs_result) * b->nlines);
-        const size_t off_tw = off_twoff + up(sizeof(int64_t) * b->nlines);
-        const size_t off_work = off_tw + up(2 * es * (size_t)std::max<int64_t>(tw_total, 1));
-        b->d_scratch.ensure(off_work + es * nvec * (size_t)b->total);
-        char *w = b->d_scratch.as<char>();
+        // Teams: the longest lines are split over several workgroups, each
+        // owning a contiguous segment, so that they stop setting the launch's
+        // length (one line on one CU otherwise); slots are queued longest
+        // line first, a team's members consecutively.
         std::vector<int> order(b->nlines);
         std::iota(order.begin(), order.end(), 0);
         std::stable_sort(order.begin(), order.end(),
                          [&](int x, int y) { return b->lines[x].n > b->lines[y].n; });
-        QEW_HIP(hipMemcpyAsync(w, order.data(), sizeof(int) * b->nlines, hipMemcpyHostToDevice,
+        std::vector<int> team(b->nlines), slot_line, slot_member;
+        const TeamPolicy policy = lbfgs_team_policy();
+        for (int l = 0; l < b->nlines; ++l) team[l] = lbfgs_team_size(b->lines[l].n, policy);
+        for (const int l : order)
+            for (int k = 0; k < team[l]; ++k) {
+                slot_line.push_back(l);
+                slot_member.push_back(k);
+            }
+        const int nslots = (int)slot_line.size();
+        // layout: [slot lines][slot members][team sizes][counter][abort]
+        // [results][table offsets][team exchange][twiddles][work]
+        auto up = [](size_t v) { return (v + 255) / 256 * 256; };
+        const size_t off_smem = up(sizeof(int) * nslots);
+        const size_t off_team = off_smem + up(sizeof(int) * nslots);
+        const size_t off_next = off_team + up(sizeof(int) * b->nlines);
+        const size_t off_abort = off_next + 256;
+        const size_t off_res = off_abort + 256;
+        const size_t off_twoff = off_res + up(sizeof(qew_lbfgs_result) * b->nlines);
+        const size_t off_xchg = off_twoff + up(sizeof(int64_t) * b->nlines);
+        // tagged words of the small exchanges, zeroed (no tag) every launch
+        const int xchg_u32 = 2 * 2 * kMaxTeam * kXchgVals * 2 * b->nlines;
+        const size_t off_tw = off_xchg + up(sizeof(unsigned) * (size_t)xchg_u32);
+        const size_t off_work = off_tw + up(2 * es * (size_t)std::max<int64_t>(tw_total, 1));
+        // each line's workspace vectors hold n + 2 kMaxTeam elements
+        b->d_scratch.ensure(off_work + es * nvec *
+                                           ((size_t)b->total + 2 * (size_t)kMaxTeam * b->nlines));
+        char *w = b->d_scratch.as<char>();
+        QEW_HIP(hipMemcpyAsync(w, slot_line.data(), sizeof(int) * nslots, hipMemcpyHostToDevice,
                                ctx->stream));
+        QEW_HIP(hipMemcpyAsync(w + off_smem, slot_member.data(), sizeof(int) * nslots,
+                               hipMemcpyHostToDevice, ctx->stream));
+        QEW_HIP(hipMemcpyAsync(w + off_team, team.data(), sizeof(int) * b->nlines,
+                               hipMemcpyHostToDevice, ctx->stream));
         hipLaunchKernelGGL(zero_u32, dim3(1), dim3(1), 0, ctx->stream,
                            reinterpret_cast<unsigned *>(w + off_next));
+        hipLaunchKernelGGL(zero_u32, dim3(1), dim3(1), 0, ctx->stream,
+                           reinterpret_cast<unsigned *>(w + off_abort));
+        hipLaunchKernelGGL(zero_u32_n, dim3((xchg_u32 + 255) / 256), dim3(256), 0, ctx->stream,
+                           reinterpret_cast<unsigned *>(w + off_xchg), xchg_u32);
         QEW_HIP(hipMemcpyAsync(w + off_twoff, tw_off.data(), sizeof(int64_t) * b->nlines,
                                hipMemcpyHostToDevice, ctx->stream));
         for (const auto &t : tables) {
@@ -1325,7 +1809,13 @@ extern "C" qew_status qew_batch_lbfgs(qew_batch b, int dtype, const qew_lbfgs_pa
         BatchArgs a{};
         a.lines = b->d_lines.as<LineDesc>();
         a.noises = b->d_noises.as<NoiseDesc>();
-        a.order = reinterpret_cast<int *>(w);
+        a.order = nullptr;
+        a.slot_line = reinterpret_cast<const int *>(w);
+        a.slot_member = reinterpret_cast<const int *>(w + off_smem);
+        a.team_size = reinterpret_cast<const int *>(w + off_team);
+        a.nslots = nslots;
+        a.xchg = reinterpret_cast<unsigned long long *>(w + off_xchg);
+        a.abort = reinterpret_cast<int *>(w + off_abort);
         a.nlines = b->nlines;
         a.next = reinterpret_cast<unsigned *>(w + off_next);
         a.h = h;
@@ -1337,14 +1827,15 @@ extern "C" qew_status qew_batch_lbfgs(qew_batch b, int dtype, const qew_lbfgs_pa
         a.lp = *lp;
         // the FFT buffer doubles as the packed-q buffer: n reals = N complex
         const size_t lds = (size_t)maxN * 2 * es;
-        auto launch = [&](auto kernel) { launch_persistent(kernel, a, b->nlines, lds, ctx); };
+        int grid = 0;
+        auto launch = [&](auto kernel) { grid = launch_persistent(kernel, a, nslots, lds, ctx); };
         if (mixed) {
             const int variant = (dtype == QEW_F64 && tdt == QEW_F64_MID)   ? 0
                                 : (dtype == QEW_F32 && tdt == QEW_F32_MID) ? 1
                                 : (dtype == QEW_F32)                       ? 2
                                 : (tdt == 1)                               ? 3
                                                                            : 4;
-            launch_lbfgs_mixed(variant, &a, b->nlines, lds, ctx);
+            grid = launch_lbfgs_mixed(variant, &a, nslots, lds, ctx);
         } else {
             if (dtype == QEW_F64 && tdt == QEW_F64_MID) launch(lbfgs_batch_kernel<double, double, true>);
             else if (dtype == QEW_F32 && tdt == QEW_F32_MID) launch(lbfgs_batch_kernel<float, float, true>);
@@ -1354,7 +1845,22 @@ extern "C" qew_status qew_batch_lbfgs(qew_batch b, int dtype, const qew_lbfgs_pa
         }
         QEW_HIP(hipMemcpyAsync(results, a.results, sizeof(qew_lbfgs_result) * b->nlines,
                                hipMemcpyDeviceToHost, ctx->stream));
+        int aborted = 0;
+        QEW_HIP(hipMemcpyAsync(&aborted, a.abort, sizeof(int), hipMemcpyDeviceToHost,
+                               ctx->stream));
         QEW_HIP(hipStreamSynchronize(ctx->stream));
+        if (aborted)
+            throw Error(QEW_ERR_HIP,
+                        "batched L-BFGS: a team workgroup gave up waiting for its team "
+                        "(not all of the launch's workgroups were resident)");
+        b->lbfgs_workgroups = grid;
+    });
+}
+
+extern "C" qew_status qew_batch_lbfgs_workgroups(qew_batch b, int *workgroups) {
+    return guard([&] {
+        QEW_REQUIRE(b && workgroups, "null pointer");
+        *workgroups = b->lbfgs_workgroups;
     });
 }
 

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <type_traits>
 #include <chrono>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -505,11 +506,15 @@ constexpr int kRkLineBlock = 512;
 constexpr int kRkLdsMaxN = 16384;
 
 // With `cont`, the kernel continues rk_block calls that the per-colour
-// launches began: the line state, moved flags and RkState (sweeps, advance,
-// counters) are taken from global memory, lines that already stopped are
-// left alone, and the sweeps go on exactly as the launches would run them.
+// launches or an earlier launch of this kernel began: the line state, moved
+// flags and RkState (sweeps, advance, counters) are taken from global memory,
+// lines that already stopped are left alone, and the sweeps go on exactly as
+// one uninterrupted run would take them. A launch does at most `sweep_cap`
+// sweeps per line; a line still sweeping then saves its moved flags and stays
+// active for the next launch.
 template <typename C>
-__global__ __launch_bounds__(kRkLineBlock) void rk_line_kernel(RkArgs a, int lds_n, int cont) {
+__global__ __launch_bounds__(kRkLineBlock) void rk_line_kernel(RkArgs a, int lds_n, int cont,
+                                                               int sweep_cap) {
     extern __shared__ double s_h[];
     unsigned char *s_mv = reinterpret_cast<unsigned char *>(s_h + lds_n);
     __shared__ int s_list[kRkLineBlock];
@@ -545,6 +550,8 @@ __global__ __launch_bounds__(kRkLineBlock) void rk_line_kernel(RkArgs a, int lds
     }
     unsigned long long evals = 0, done = 0;
     bool go = cont ? true : a.rp.max_sweeps > 0;
+    int chunk = 0;
+    bool capped = false;
     while (go) {
         double my_adv = 0;
         for (int color = 0; color < 2; ++color) {
@@ -609,15 +616,21 @@ __global__ __launch_bounds__(kRkLineBlock) void rk_line_kernel(RkArgs a, int lds
         if (tid == 0) s_go = sweep_status(st, total, n, a.rp) ? 1 : 0;
         __syncthreads();
         go = s_go != 0;
+        if (go && ++chunk >= sweep_cap) {  // uniform: s_go and chunk are
+            capped = true;
+            go = false;
+        }
     }
     for (int i = tid; i < n; i += kRkLineBlock) hg[i] = s_h[i];
+    if (capped)
+        for (int i = tid; i < n; i += kRkLineBlock) a.moved[L.off + i] = s_mv[i];
     const unsigned long long ev = wave_sum(evals);
     if ((tid & (kWave - 1)) == 0 && ev) atomicAdd(&a.state[line].vel_evals, ev);
     const unsigned long long dn = wave_sum(done);
     if ((tid & (kWave - 1)) == 0 && dn) atomicAdd(&a.state[line].vel_done, dn);
     if (tid == 0) {
         // the counter is added to by every wave: leave it out of the store
-        a.state[line].active = 0;
+        a.state[line].active = capped ? 1 : 0;
         a.state[line].sweeps = st.sweeps;
         a.state[line].blocked = st.blocked;
         a.state[line].runaway = st.runaway;
@@ -760,20 +773,68 @@ enum class RkMode { Line, Launch, Hybrid };
 // sweep of the launches costs three launches over all lines' tiles.
 constexpr double kRkSwitchMovedPerLine = 128.0;
 
-// Runs the sweeps for `nlines` lines whose descriptors are on the device, on
-// the LDS path (`line`) or with per-colour launches.
-void launch_line_kernel(hipStream_t s, const RkArgs &a, int nlines, int max_n, int table_dtype,
-                        int cont) {
+// Sweeps per line kernel launch: between launches the host reads the lines'
+// states (and may report progress), so a call of many sweeps never runs as
+// one kernel of unbounded length. QEW_RK_LINE_SWEEPS sets another positive
+// value (the tests use a small one to cross many launch boundaries).
+int rk_line_sweeps_per_launch() {
+    const char *env = std::getenv("QEW_RK_LINE_SWEEPS");
+    const int v = env ? std::atoi(env) : 0;
+    return v > 0 ? v : 2048;
+}
+
+// QEW_RK_PROGRESS=<seconds>: rk_block writes a line to stderr at most this
+// often while it sweeps (lines still sweeping, their largest sweep count).
+double rk_progress_interval() {
+    const char *env = std::getenv("QEW_RK_PROGRESS");
+    return env ? std::atof(env) : -1.0;
+}
+
+struct RkProgress {
+    double every = rk_progress_interval();
+    std::chrono::steady_clock::time_point last = std::chrono::steady_clock::now();
+    void report(int active, int max_sweeps) {
+        if (every < 0) return;
+        const auto now = std::chrono::steady_clock::now();
+        if (std::chrono::duration<double>(now - last).count() < every) return;
+        last = now;
+        std::fprintf(stderr, "rk_block: %d lines sweeping, %d sweeps\n", active, max_sweeps);
+        std::fflush(stderr);
+    }
+};
+
+// Runs the sweeps of the lines whose descriptors are on the device in the LDS
+// line kernel, rk_line_sweeps_per_launch() sweeps per launch until every line
+// has stopped; `cont` = 0 starts the call, 1 continues the states in `a`.
+// Leaves the final states in `out`.
+void run_line_kernel(hipStream_t s, const RkArgs &a, int nlines, int max_n, int table_dtype,
+                     int cont, std::vector<RkState> &out) {
     const size_t lds = (sizeof(double) + 1) * (size_t)max_n;
-    auto go = [&](auto kernel) {
+    const int cap = rk_line_sweeps_per_launch();
+    auto go = [&](auto kernel, int c) {
         QEW_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(kernel, dim3(nlines), dim3(kRkLineBlock), lds, s, a, max_n, cont);
+        hipLaunchKernelGGL(kernel, dim3(nlines), dim3(kRkLineBlock), lds, s, a, max_n, c, cap);
     };
-    if (table_dtype == QEW_F32_MID) go(rk_line_kernel<MidF32>);
-    else if (table_dtype == 1) go(rk_line_kernel<float>);
-    else go(rk_line_kernel<double>);
-    QEW_HIP(hipGetLastError());
+    RkProgress progress;
+    out.resize(nlines);
+    for (;;) {
+        if (table_dtype == QEW_F32_MID) go(rk_line_kernel<MidF32>, cont);
+        else if (table_dtype == 1) go(rk_line_kernel<float>, cont);
+        else go(rk_line_kernel<double>, cont);
+        QEW_HIP(hipGetLastError());
+        QEW_HIP(hipMemcpyAsync(out.data(), a.state, sizeof(RkState) * nlines,
+                               hipMemcpyDeviceToHost, s));
+        QEW_HIP(hipStreamSynchronize(s));
+        int active = 0, most = 0;
+        for (const RkState &st : out) {
+            active += st.active ? 1 : 0;
+            most = std::max(most, st.sweeps);
+        }
+        if (!active) return;
+        progress.report(active, most);
+        cont = 1;
+    }
 }
 
 void run_rk(qew_ctx ctx, const LineDesc *d_lines, const NoiseDesc *d_noises, int nlines, int max_n,
@@ -788,7 +849,8 @@ void run_rk(qew_ctx ctx, const LineDesc *d_lines, const NoiseDesc *d_noises, int
     RkArgs a{d_lines, d_noises, h, w.moved, w.partial, w.pmoved, tiles, w.state, rp};
     if (mode == RkMode::Line) {
         QEW_HIP(hipMemsetAsync(w.state, 0, sizeof(RkState) * (size_t)nlines, s));
-        launch_line_kernel(s, a, nlines, max_n, table_dtype, 0);
+        run_line_kernel(s, a, nlines, max_n, table_dtype, 0, out);
+        return;
     } else {
         hipLaunchKernelGGL(rk_init_kernel, dim3(nlines), dim3(kRkBlock), 0, s, a);
         QEW_HIP(hipGetLastError());
@@ -796,6 +858,7 @@ void run_rk(qew_ctx ctx, const LineDesc *d_lines, const NoiseDesc *d_noises, int
         // check the running flag every `poll` sweeps; lines that finished in
         // between are frozen, so the extra launches are no-ops for them
         int poll = 4;
+        RkProgress progress;
         for (int sweep = 0; sweep < rp.max_sweeps;) {
             const int todo = std::min(poll, rp.max_sweeps - sweep);
             for (int k = 0; k < todo; ++k) {
@@ -825,9 +888,10 @@ void run_rk(qew_ctx ctx, const LineDesc *d_lines, const NoiseDesc *d_noises, int
                  (double)running_host[1] <= switch_per_line * (double)running_host[0])) {
                 // the lines still sweeping finish in the LDS kernel, from the
                 // state and moved flags the launches left
-                launch_line_kernel(s, a, nlines, max_n, table_dtype, 1);
-                break;
+                run_line_kernel(s, a, nlines, max_n, table_dtype, 1, out);
+                return;
             }
+            progress.report((int)running_host[0], sweep);
             poll = std::min(poll * 2, 256);
         }
     }

@@ -20,7 +20,7 @@ from . import Batch, Model, Params
 
 def fc_search(noises, n, y0, rp, lengths_L=None, line_tension=1.0, model=Model.Linear,
               f_start=0.125, k_way=4, rel_tol=1e-10, budget_s=float("inf"), log=None,
-              record=None, state=None):
+              record=None, state=None, checkpoint=None):
     """f_c bracket of every potential in `noises` for a line of n sites (box
     length lengths_L, default n) started flat at y0: relaxation at f = 0,
     ramp f = f_start * 2^k per line until it is not blocked, then k_way-way
@@ -29,7 +29,8 @@ def fc_search(noises, n, y0, rp, lengths_L=None, line_tension=1.0, model=Model.L
     configurations at lo (torch, nlines x n) and counters. `record(k, res, h,
     seconds)` sees the relaxation (k = 0) and the first ramp probe (k = 1).
     `state` = (lo, hi, h_lo) of an earlier call continues its rounds (no
-    relaxation or ramp)."""
+    relaxation or ramp); `checkpoint(lo, hi, h_lo)` is called after the ramp
+    and after every round with the state such a call continues from."""
     import torch
     nlines = len(noises)
     L = float(n if lengths_L is None else lengths_L)
@@ -81,6 +82,8 @@ def fc_search(noises, n, y0, rp, lengths_L=None, line_tension=1.0, model=Model.L
         forces = np.where(np.isinf(hi), np.maximum(2 * lo, f_ramp), forces)
         f_ramp *= 2
     stats["ramp_probes"] = k
+    if checkpoint and state is None:
+        checkpoint(lo, hi, h_lo)
     while True:
         todo = np.nonzero(hi - lo > rel_tol * np.maximum(np.abs(lo), np.abs(hi)))[0]
         if todo.size == 0 or time.perf_counter() - t0 > budget_s:
@@ -109,5 +112,7 @@ def fc_search(noises, n, y0, rp, lengths_L=None, line_tension=1.0, model=Model.L
         if log:
             log(f"  round {stats['rounds']}: {todo.size} lines x {k_way}, {sec:.2f} s, "
                 f"max sweeps {sweeps.max()}, max rel width {width.max():.3g}")
+        if checkpoint:
+            checkpoint(lo, hi, h_lo)
     stats["seconds"] = time.perf_counter() - t0
     return lo, hi, h_lo, stats

@@ -330,7 +330,7 @@ def test_fc_bisection_to_1e10_matches_oracle(q):
     np.testing.assert_array_equal(hc_g, hc_o)
 
 
-@pytest.mark.parametrize("path", ["line", "launch", "hybrid"])
+@pytest.mark.parametrize("path", ["line", "launch", "hybrid", "line/7", "hybrid/5"])
 @pytest.mark.parametrize("n,lt,f,max_sweeps", [
     (16384, 0.005, 0.1, 3000),   # largest line the LDS path holds; runs to blocked
     (16384, 0.02, 0.2, 3000),
@@ -338,8 +338,15 @@ def test_fc_bisection_to_1e10_matches_oracle(q):
 ])
 def test_rk_long_lines_match_oracle_bitwise(q, monkeypatch, path, n, lt, f, max_sweeps):
     # hybrid: per-colour launches for the first poll's sweeps, then the LDS
-    # kernel continues the same call from the launches' state and flags
+    # kernel continues the same call from the launches' state and flags;
+    # "path/k": the LDS kernel stops after k sweeps per launch and the next
+    # launch continues from the saved heights, moved flags and counters
+    path, _, per_launch = path.partition("/")
     monkeypatch.setenv("QEW_RK_KERNEL", path)
+    if per_launch:
+        monkeypatch.setenv("QEW_RK_LINE_SWEEPS", per_launch)
+    else:
+        monkeypatch.delenv("QEW_RK_LINE_SWEEPS", raising=False)
     # 16 x 16384 and 64 x 32768 sites per column of a 256 x 64 table: many
     # moving sites per workgroup in the first sweeps, few near the end, so
     # the lane-group sizes of the cooperative march/bisection all occur
