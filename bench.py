@@ -243,9 +243,18 @@ def bench_lbfgs(log, nlines=512, plain_sample=8, with_cpu=True, omp_info=None):
         b = q.Batch(lengths, noises, params)
         b.set_precond_shifts(shifts)
         lp = q.LbfgsParams(m=10, ftol=ftol, max_iter=10000, precondition=True)
+        # untimed warm-up: the batch's first call picks its team policy by
+        # timing the candidates on a copy of the line state
         h = h0.clone()
         torch.cuda.synchronize()
-        log(f"lbfgs {name}: launching")
+        t_tune = time.perf_counter()
+        b.lbfgs(h, lp, dtype=dtype)
+        t_tune = time.perf_counter() - t_tune
+        wgs, teams = b.lbfgs_plan()
+        team_env = ",".join(f"{n_}:{k_}" for n_, k_ in teams) or "1:1"
+        h = h0.clone()
+        torch.cuda.synchronize()
+        log(f"lbfgs {name}: launching (team policy {team_env}, picked in {t_tune:.2f} s)")
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         t1 = time.perf_counter()
         ev0.record()
@@ -262,9 +271,10 @@ def bench_lbfgs(log, nlines=512, plain_sample=8, with_cpu=True, omp_info=None):
         gmax[name] = np.array([ga[o0:o1].max() for o0, o1 in zip(offs[:-1], offs[1:])])
         its = [r.iterations for r in res]
         results[name] = (res, span)
-        wgs = b.lbfgs_workgroups()
+        # single lines below run with the batch's team sizes
+        os.environ["QEW_LBFGS_TEAMS"] = team_env
         # the critical path: the line with the most site-iterations, solved
-        # alone (one workgroup), against the whole launch
+        # alone (on its team size in the batch), against the whole launch
         crit = int(np.argmax(np.array(its) * np.array(lengths)))
         b1 = q.Batch([lengths[crit]], [noises[crit]], [params[crit]])
         b1.set_precond_shifts([shifts[crit]])
@@ -275,7 +285,7 @@ def bench_lbfgs(log, nlines=512, plain_sample=8, with_cpu=True, omp_info=None):
         alone = time.perf_counter() - t2
         del b1, h1
         # workgroup utilisation, estimated: every 8th line solved alone (on
-        # as many workgroups as in the batch: 4 for L = 16384, 2 for 8192),
+        # as many workgroups as it has in the batch),
         # its workgroup-seconds scaled to the batch, over the launch's
         # workgroups x span (solo lines meet no contention, so this
         # underestimates the busy share somewhat)
@@ -291,8 +301,12 @@ def bench_lbfgs(log, nlines=512, plain_sample=8, with_cpu=True, omp_info=None):
             busy_sample += (time.perf_counter() - t3) * bk.lbfgs_workgroups()
             del bk, hk
         busy = busy_sample * nlines / len(sample)
+        del os.environ["QEW_LBFGS_TEAMS"]
         out[name] = {"minima_per_s": nlines / dt, "seconds": dt, "ftol": ftol,
                      "launch_s": span,
+                     "team_policy": {"rows": teams, "picked_in_s": t_tune,
+                                     "rule": "a line of length >= n runs on k workgroups "
+                                             "(first matching row), else one"},
                      "critical_line": {"line": crit, "L": int(lengths[crit]),
                                        "iterations": int(its[crit]), "alone_s": alone,
                                        "alone_over_launch": alone / span},

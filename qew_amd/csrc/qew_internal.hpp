@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/qew_amd.h"
@@ -162,8 +163,15 @@ struct qew_batch_s {
     // 2 per-colour launches
     int rk_path = 0;
     qew_host::DevBuf d_ltile, d_tune;
-    // persistent workgroups of the last qew_batch_lbfgs launch
+    // persistent workgroups of the last qew_batch_lbfgs launch and the team
+    // policy rows it ran with (minimum line length, workgroups per line)
     int lbfgs_workgroups = 0;
+    std::vector<std::pair<int, int>> lbfgs_last_teams;
+    // team policy picked by timing, per (FP32 state, preconditioner) pair
+    struct TunedTeams {
+        bool tuned = false;
+        std::vector<std::pair<int, int>> rows;
+    } lbfgs_teams[4];
 };
 
 namespace qew_host {

@@ -31,6 +31,8 @@
 // Armijo test, the energies it compares and the curvature pair s.y, s.s, y.y
 // stay FP64.
 #include <algorithm>
+#include <chrono>
+#include <functional>
 #include <cstdlib>
 #include <numeric>
 #include <type_traits>
@@ -51,8 +53,9 @@ constexpr int kMaxM = 32;       // history length limit
 constexpr int kMaxFftN = 8192;  // complex LDS transform length (line length 16384)
 
 // Workspace vectors per line for history length m: x g gn d q edy esl, m + 1
-// S and m + 1 Y rows, and two for a team's vector exchange.
-__host__ __device__ constexpr int lbfgs_nvec(int m) { return 9 + 2 * (m + 1); }
+// S and m + 1 Y rows, two for a team's vector exchange and kLS for the line
+// search's trial dV/dy.
+__host__ __device__ constexpr int lbfgs_nvec(int m) { return 9 + 2 * (m + 1) + kLS; }
 
 template <typename T>
 struct C2 {
@@ -593,10 +596,14 @@ __device__ __forceinline__ void objective_trials_mid(const LineGeom &G, const T 
         });
 }
 
+// With dyt (FP64 state on FP64 tables): dV/dy at each trial point of each
+// edge, from the held block, into dyt[j * dstride + i] (NaN where the trial
+// left the held cell), for the gradient at the accepted step.
 template <typename T, typename C, bool MID = false, bool TEAM = false>
 __device__ __forceinline__ void objective_trials_table(const LineGeom &G, const T *x, const T *d,
                                                        const double *alpha, double (&acc)[kLS],
-                                                       const Team &tm) {
+                                                       const Team &tm, double *dyt = nullptr,
+                                                       size_t dstride = 0) {
     const int n = G.n;
     constexpr int K = kLS;
     if constexpr (MID) {
@@ -656,6 +663,14 @@ __device__ __forceinline__ void objective_trials_table(const LineGeom &G, const 
                     bspline_w(fx, wx);
                     bspline_w(fy, wy);
                     vt[j] = v_from_block(it.c, wx, wy);
+                    if constexpr (sizeof(T) == 8 && std::is_same<C, double>::value)
+                        if (dyt) {
+                            double dwy[4];
+                            bspline_dw(fy, dwy);
+                            dyt[j * dstride + i] = dy_from_block(it.c, wx, dwy, G.nd.dy);
+                        }
+                } else if constexpr (sizeof(T) == 8 && std::is_same<C, double>::value) {
+                    if (dyt) dyt[j * dstride + i] = __builtin_nan("");
                 }
             }
 #pragma unroll
@@ -694,14 +709,15 @@ __device__ __forceinline__ void objective_trials_table(const LineGeom &G, const 
 // those of wg_objective, so phi[j] equals wg_objective(G, x, d, alpha[j]).
 template <typename T, typename C, bool MID = false, bool TEAM = false>
 __device__ __forceinline__ void wg_objective_trials(const LineGeom &G, const T *x, const T *d,
-                                    const double *alpha, double *phi, Red r, Team &tm) {
+                                    const double *alpha, double *phi, Red r, Team &tm,
+                                    double *dyt = nullptr, size_t dstride = 0) {
     const int n = G.n;
     constexpr int K = kLS;
     double acc[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) acc[j] = 0;
     if (G.nd.kind == TABLE) {
-        objective_trials_table<T, C, MID, TEAM>(G, x, d, alpha, acc, tm);
+        objective_trials_table<T, C, MID, TEAM>(G, x, d, alpha, acc, tm, dyt, dstride);
         block_sumk<K>(acc, r.d);
         if constexpr (TEAM) team_sum<K>(tm, acc);
 #pragma unroll
@@ -750,17 +766,24 @@ __device__ __forceinline__ void wg_objective_trials(const LineGeom &G, const T *
 // i + 1); a team member also forms its left halo edge (s0 - 1, s0) from its
 // halo site, into its private edy / esl copies (indexed like x, from s0 - 1).
 // Site pass: g(i) from the edges i - 1 and i.
+// With dyr (FP64 state on FP64 tables, right after an accepted line-search
+// step): the edges' dV/dy as the line search formed them at this step's
+// heights, NaN where it did not (those edges, and a team member's halo edge,
+// are sampled here).
 template <typename T, typename C, bool MID = false, bool TEAM = false>
 __device__ __forceinline__ void wg_gradient(const LineGeom &G, const T *x, T *g, T *edy, T *esl,
-                                            const Team &tm) {
+                                            const Team &tm, const double *dyr = nullptr) {
     const int n = G.n;
     const bool arc = G.model == 1;
     const int s0 = TEAM ? tm.s0 : 0, s1 = TEAM ? tm.s1 : n;
+    constexpr bool kReuse = sizeof(T) == 8 && std::is_same<C, double>::value && !MID;
     stream<kUG<T>>(TEAM ? s0 - 1 : s0, s1, [&](int e) {
                const int ip = (e + 1 == n) ? 0 : e + 1;
-               return P2<T>{x[e], QEW_XR(e, ip)};
+               return P3<T>{x[e], QEW_XR(e, ip),
+                            (kReuse && dyr && !(TEAM && e == s0 - 1)) ? (T)dyr[e]
+                                                                       : (T)__builtin_nan("")};
            },
-           [&](int e, P2<T> v) {
+           [&](int e, P3<T> v) {
                // the edge's place on the line (e = -1 is the wrap edge n - 1)
                const int i = e < 0 ? n - 1 : e;
                const int ip = (i + 1 == n) ? 0 : i + 1;
@@ -768,6 +791,8 @@ __device__ __forceinline__ void wg_gradient(const LineGeom &G, const T *x, T *g,
                    const double dh = div1(v.b - v.a, G.dx);
                    if constexpr (MID)
                        edy[e] = mid_dy64<double>(G.nd, mid_column(G.nd, i, ip), (v.b + v.a) * 0.5);
+                   else if (kReuse && v.c == v.c)
+                       edy[e] = v.c;
                    else
                        edy[e] = sample_dy64<C>(G.nd, (double)(i + ip) * 0.5 * G.dx, (v.b + v.a) * 0.5);
                    if (arc) esl[e] = dh / sqrt(1.0 + dh * dh);
@@ -1090,11 +1115,19 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
         esl += pb;
         // 8-byte aligned words in the last two vectors: n words for FP32
         // elements, 2n for FP64 (the two vectors hold 2 np elements)
-        const uintptr_t bw = reinterpret_cast<uintptr_t>(ws + (size_t)(lbfgs_nvec(m) - 2) * np);
+        const uintptr_t bw =
+            reinterpret_cast<uintptr_t>(ws + (size_t)(lbfgs_nvec(m) - kLS - 2) * np);
         tm.bulk = reinterpret_cast<u64 *>((bw + 7) & ~(uintptr_t)7);
     }
     auto Sb = [&](int k) { return ws + (size_t)(7 + k) * np; };
     auto Yb = [&](int k) { return ws + (size_t)(7 + (m + 1) + k) * np; };
+    // the line search's dV/dy at its trial points (FP64 state on FP64
+    // tables), read back by the gradient at the accepted step
+    constexpr bool kReuseDy = sizeof(T) == 8 && std::is_same<C, double>::value && !MID;
+    double *dyt = nullptr;
+    if constexpr (kReuseDy)
+        if (nd.kind == TABLE)
+            dyt = reinterpret_cast<double *>(ws + (size_t)(lbfgs_nvec(m) - kLS) * np);
     for (int i = s0 + tid; i < s1; i += blockDim.x) x[i] = (T)(h[i] - anchor);
     if (tid == 0) {
         for (int k = 0; k <= m; ++k) sbuf[k] = k;  // sbuf[m] is the spare (s_tmp / y_tmp)
@@ -1290,12 +1323,13 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
         // sequential one
         constexpr int K = kLS;
         bool ok = false;
+        int jacc = 0;  // the accepted trial's place in its pass
         for (int ls = 0; ls < lp.max_ls && !ok;) {
             double al[K], ph[K];
             al[0] = alpha;
 #pragma unroll
             for (int j = 1; j < K; ++j) al[j] = al[j - 1] * lp.ls_shrink;
-            wg_objective_trials<T, C, MID, TEAM>(G, x, d, al, ph, r, tm);
+            wg_objective_trials<T, C, MID, TEAM>(G, x, d, al, ph, r, tm, dyt, np);
 #pragma unroll
             for (int j = 0; j < K; ++j) {
                 if (ok || ls >= lp.max_ls) continue;
@@ -1304,6 +1338,7 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
                 if (ph[j] <= phi0 + lp.c1 * al[j] * dphi0) {
                     phi = ph[j];
                     alpha = al[j];
+                    jacc = j;
                     ok = true;
                 }
             }
@@ -1341,7 +1376,8 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
             x[s1] = x[s1] + alT * d[s1];
         }
         __syncthreads();
-        wg_gradient<T, C, MID, TEAM>(G, x, gn, edy, esl, tm);
+        wg_gradient<T, C, MID, TEAM>(G, x, gn, edy, esl, tm,
+                                     dyt ? dyt + (size_t)jacc * np : nullptr);
         double sy3[3] = {0, 0, 0};
         stream<UV>(s0, s1, [&](int i) { return P3<T>{st[i], gn[i], g[i]}; }, [&](int i, P3<T> v) {
             const T y = v.b - v.c;
@@ -1479,7 +1515,8 @@ struct TeamPolicy {
 
 TeamPolicy lbfgs_team_policy() {
     TeamPolicy p;
-    p.rows = {{16384, 4}, {8192, 2}};
+    // measured best on the 512-line batch of lengths 1024 ... 16384
+    p.rows = {{16384, 2}};
     if (const char *e = std::getenv("QEW_LBFGS_TEAM")) p.forced = std::atoi(e);
     if (const char *e = std::getenv("QEW_LBFGS_TEAMS")) {
         p.rows.clear();
@@ -1499,6 +1536,29 @@ TeamPolicy lbfgs_team_policy() {
                   });
     }
     return p;
+}
+
+// Policies the tuner times for a batch: one workgroup per line first, then
+// teams of 2 or 4 on the longest length present and of 2 on the next one
+// (lengths of at least 4096 sites; shorter lines gain nothing from a team).
+std::vector<TeamPolicy> lbfgs_team_candidates(const std::vector<LineDesc> &lines) {
+    std::vector<int> lens;
+    for (const auto &l : lines) lens.push_back(l.n);
+    std::sort(lens.begin(), lens.end(), std::greater<int>());
+    lens.erase(std::unique(lens.begin(), lens.end()), lens.end());
+    std::vector<TeamPolicy> out(1);  // all single
+    if (lens.empty() || lens[0] < 4096) return out;
+    const int l1 = lens[0], l2 = lens.size() > 1 && lens[1] >= 4096 ? lens[1] : 0;
+    for (int k1 : {2, 4}) {
+        TeamPolicy p;
+        p.rows = {{l1, k1}};
+        out.push_back(p);
+        if (l2) {
+            p.rows = {{l1, k1}, {l2, 2}};
+            out.push_back(p);
+        }
+    }
+    return out;
 }
 
 int lbfgs_team_size(int n, const TeamPolicy &p) {
@@ -1745,7 +1805,7 @@ extern "C" qew_status qew_batch_lbfgs(qew_batch b, int dtype, const qew_lbfgs_pa
                     tw_off[l] = it->second;
                 }
             }
-        // Teams: the longest lines are split over several workgroups, each
+        // Teams: the longest lines may be split over several workgroups, each
         // owning a contiguous segment, so that they stop setting the launch's
         // length (one line on one CU otherwise); slots are queued longest
         // line first, a team's members consecutively.
@@ -1753,20 +1813,12 @@ extern "C" qew_status qew_batch_lbfgs(qew_batch b, int dtype, const qew_lbfgs_pa
         std::iota(order.begin(), order.end(), 0);
         std::stable_sort(order.begin(), order.end(),
                          [&](int x, int y) { return b->lines[x].n > b->lines[y].n; });
-        std::vector<int> team(b->nlines), slot_line, slot_member;
-        const TeamPolicy policy = lbfgs_team_policy();
-        for (int l = 0; l < b->nlines; ++l) team[l] = lbfgs_team_size(b->lines[l].n, policy);
-        for (const int l : order)
-            for (int k = 0; k < team[l]; ++k) {
-                slot_line.push_back(l);
-                slot_member.push_back(k);
-            }
-        const int nslots = (int)slot_line.size();
+        const int max_slots = kMaxTeam * b->nlines;
         // layout: [slot lines][slot members][team sizes][counter][abort]
         // [results][table offsets][team exchange][twiddles][work]
         auto up = [](size_t v) { return (v + 255) / 256 * 256; };
-        const size_t off_smem = up(sizeof(int) * nslots);
-        const size_t off_team = off_smem + up(sizeof(int) * nslots);
+        const size_t off_smem = up(sizeof(int) * max_slots);
+        const size_t off_team = off_smem + up(sizeof(int) * max_slots);
         const size_t off_next = off_team + up(sizeof(int) * b->nlines);
         const size_t off_abort = off_next + 256;
         const size_t off_res = off_abort + 256;
@@ -1780,18 +1832,6 @@ extern "C" qew_status qew_batch_lbfgs(qew_batch b, int dtype, const qew_lbfgs_pa
         b->d_scratch.ensure(off_work + es * nvec *
                                            ((size_t)b->total + 2 * (size_t)kMaxTeam * b->nlines));
         char *w = b->d_scratch.as<char>();
-        QEW_HIP(hipMemcpyAsync(w, slot_line.data(), sizeof(int) * nslots, hipMemcpyHostToDevice,
-                               ctx->stream));
-        QEW_HIP(hipMemcpyAsync(w + off_smem, slot_member.data(), sizeof(int) * nslots,
-                               hipMemcpyHostToDevice, ctx->stream));
-        QEW_HIP(hipMemcpyAsync(w + off_team, team.data(), sizeof(int) * b->nlines,
-                               hipMemcpyHostToDevice, ctx->stream));
-        hipLaunchKernelGGL(zero_u32, dim3(1), dim3(1), 0, ctx->stream,
-                           reinterpret_cast<unsigned *>(w + off_next));
-        hipLaunchKernelGGL(zero_u32, dim3(1), dim3(1), 0, ctx->stream,
-                           reinterpret_cast<unsigned *>(w + off_abort));
-        hipLaunchKernelGGL(zero_u32_n, dim3((xchg_u32 + 255) / 256), dim3(256), 0, ctx->stream,
-                           reinterpret_cast<unsigned *>(w + off_xchg), xchg_u32);
         QEW_HIP(hipMemcpyAsync(w + off_twoff, tw_off.data(), sizeof(int64_t) * b->nlines,
                                hipMemcpyHostToDevice, ctx->stream));
         for (const auto &t : tables) {
@@ -1813,12 +1853,10 @@ extern "C" qew_status qew_batch_lbfgs(qew_batch b, int dtype, const qew_lbfgs_pa
         a.slot_line = reinterpret_cast<const int *>(w);
         a.slot_member = reinterpret_cast<const int *>(w + off_smem);
         a.team_size = reinterpret_cast<const int *>(w + off_team);
-        a.nslots = nslots;
         a.xchg = reinterpret_cast<unsigned long long *>(w + off_xchg);
         a.abort = reinterpret_cast<int *>(w + off_abort);
         a.nlines = b->nlines;
         a.next = reinterpret_cast<unsigned *>(w + off_next);
-        a.h = h;
         a.work = w + off_work;
         a.results = reinterpret_cast<qew_lbfgs_result *>(w + off_res);
         a.tw = w + off_tw;
@@ -1827,40 +1865,119 @@ extern "C" qew_status qew_batch_lbfgs(qew_batch b, int dtype, const qew_lbfgs_pa
         a.lp = *lp;
         // the FFT buffer doubles as the packed-q buffer: n reals = N complex
         const size_t lds = (size_t)maxN * 2 * es;
-        int grid = 0;
-        auto launch = [&](auto kernel) { grid = launch_persistent(kernel, a, nslots, lds, ctx); };
-        if (mixed) {
-            const int variant = (dtype == QEW_F64 && tdt == QEW_F64_MID)   ? 0
-                                : (dtype == QEW_F32 && tdt == QEW_F32_MID) ? 1
-                                : (dtype == QEW_F32)                       ? 2
-                                : (tdt == 1)                               ? 3
-                                                                           : 4;
-            grid = launch_lbfgs_mixed(variant, &a, nslots, lds, ctx);
-        } else {
-            if (dtype == QEW_F64 && tdt == QEW_F64_MID) launch(lbfgs_batch_kernel<double, double, true>);
-            else if (dtype == QEW_F32 && tdt == QEW_F32_MID) launch(lbfgs_batch_kernel<float, float, true>);
-            else if (dtype == QEW_F32) launch(lbfgs_batch_kernel<float, float>);
-            else if (tdt == 1) launch(lbfgs_batch_kernel<double, float>);
-            else launch(lbfgs_batch_kernel<double, double>);
+        // One launch on line state hh with the team sizes of `policy`;
+        // returns its workgroups.
+        auto run = [&](const TeamPolicy &policy, double *hh) {
+            std::vector<int> team(b->nlines), slot_line, slot_member;
+            for (int l = 0; l < b->nlines; ++l) team[l] = lbfgs_team_size(b->lines[l].n, policy);
+            for (const int l : order)
+                for (int k = 0; k < team[l]; ++k) {
+                    slot_line.push_back(l);
+                    slot_member.push_back(k);
+                }
+            const int nslots = (int)slot_line.size();
+            QEW_HIP(hipMemcpyAsync(w, slot_line.data(), sizeof(int) * nslots,
+                                   hipMemcpyHostToDevice, ctx->stream));
+            QEW_HIP(hipMemcpyAsync(w + off_smem, slot_member.data(), sizeof(int) * nslots,
+                                   hipMemcpyHostToDevice, ctx->stream));
+            QEW_HIP(hipMemcpyAsync(w + off_team, team.data(), sizeof(int) * b->nlines,
+                                   hipMemcpyHostToDevice, ctx->stream));
+            hipLaunchKernelGGL(zero_u32, dim3(1), dim3(1), 0, ctx->stream,
+                               reinterpret_cast<unsigned *>(w + off_next));
+            hipLaunchKernelGGL(zero_u32, dim3(1), dim3(1), 0, ctx->stream,
+                               reinterpret_cast<unsigned *>(w + off_abort));
+            hipLaunchKernelGGL(zero_u32_n, dim3((xchg_u32 + 255) / 256), dim3(256), 0,
+                               ctx->stream, reinterpret_cast<unsigned *>(w + off_xchg), xchg_u32);
+            BatchArgs ar = a;
+            ar.nslots = nslots;
+            ar.h = hh;
+            int grid = 0;
+            auto launch = [&](auto kernel) {
+                grid = launch_persistent(kernel, ar, nslots, lds, ctx);
+            };
+            if (mixed) {
+                const int variant = (dtype == QEW_F64 && tdt == QEW_F64_MID)   ? 0
+                                    : (dtype == QEW_F32 && tdt == QEW_F32_MID) ? 1
+                                    : (dtype == QEW_F32)                       ? 2
+                                    : (tdt == 1)                               ? 3
+                                                                               : 4;
+                grid = launch_lbfgs_mixed(variant, &ar, nslots, lds, ctx);
+            } else {
+                if (dtype == QEW_F64 && tdt == QEW_F64_MID)
+                    launch(lbfgs_batch_kernel<double, double, true>);
+                else if (dtype == QEW_F32 && tdt == QEW_F32_MID)
+                    launch(lbfgs_batch_kernel<float, float, true>);
+                else if (dtype == QEW_F32) launch(lbfgs_batch_kernel<float, float>);
+                else if (tdt == 1) launch(lbfgs_batch_kernel<double, float>);
+                else launch(lbfgs_batch_kernel<double, double>);
+            }
+            int aborted = 0;
+            QEW_HIP(hipMemcpyAsync(&aborted, a.abort, sizeof(int), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+            QEW_HIP(hipStreamSynchronize(ctx->stream));
+            if (aborted)
+                throw Error(QEW_ERR_HIP,
+                            "batched L-BFGS: a team workgroup gave up waiting for its team "
+                            "(not all of the launch's workgroups were resident)");
+            return grid;
+        };
+        // Team sizes: forced by QEW_LBFGS_TEAM(S), else the batch's tuned
+        // choice for this precision and preconditioner setting, picked on
+        // the first call by timing every candidate on a copy of the line
+        // state (each candidate reaches the same minima within the
+        // reductions' rounding; only time differs).
+        TeamPolicy policy = lbfgs_team_policy();
+        const bool forced = policy.forced > 0 || std::getenv("QEW_LBFGS_TEAMS") != nullptr;
+        const int key = (dtype == QEW_F32 ? 1 : 0) + (lp->precondition ? 2 : 0);
+        if (!forced) {
+            auto &tuned = b->lbfgs_teams[key];
+            if (!tuned.tuned) {
+                const std::vector<TeamPolicy> cands = lbfgs_team_candidates(b->lines);
+                tuned.tuned = true;
+                tuned.rows = cands.front().rows;
+                if (cands.size() > 1) {
+                    const size_t bytes = sizeof(double) * (size_t)b->total;
+                    b->d_tune.ensure(bytes);
+                    double *copy = b->d_tune.as<double>();
+                    double best = 1e300;
+                    for (const TeamPolicy &c : cands) {
+                        QEW_HIP(hipMemcpyAsync(copy, h, bytes, hipMemcpyDeviceToDevice,
+                                               ctx->stream));
+                        QEW_HIP(hipStreamSynchronize(ctx->stream));
+                        const auto t0 = std::chrono::steady_clock::now();
+                        run(c, copy);
+                        const double t = std::chrono::duration<double>(
+                                             std::chrono::steady_clock::now() - t0)
+                                             .count();
+                        if (t < best) {
+                            best = t;
+                            tuned.rows = c.rows;
+                        }
+                    }
+                }
+            }
+            policy.rows = tuned.rows;
         }
+        const int grid = run(policy, h);
         QEW_HIP(hipMemcpyAsync(results, a.results, sizeof(qew_lbfgs_result) * b->nlines,
                                hipMemcpyDeviceToHost, ctx->stream));
-        int aborted = 0;
-        QEW_HIP(hipMemcpyAsync(&aborted, a.abort, sizeof(int), hipMemcpyDeviceToHost,
-                               ctx->stream));
         QEW_HIP(hipStreamSynchronize(ctx->stream));
-        if (aborted)
-            throw Error(QEW_ERR_HIP,
-                        "batched L-BFGS: a team workgroup gave up waiting for its team "
-                        "(not all of the launch's workgroups were resident)");
         b->lbfgs_workgroups = grid;
+        b->lbfgs_last_teams.clear();
+        for (const auto &row : policy.rows) b->lbfgs_last_teams.push_back(row);
     });
 }
 
-extern "C" qew_status qew_batch_lbfgs_workgroups(qew_batch b, int *workgroups) {
+extern "C" qew_status qew_batch_lbfgs_plan(qew_batch b, int *workgroups, int *rows, int max_rows,
+                                           int *nrows) {
     return guard([&] {
-        QEW_REQUIRE(b && workgroups, "null pointer");
+        QEW_REQUIRE(b && workgroups && nrows && (rows || max_rows == 0), "null pointer");
         *workgroups = b->lbfgs_workgroups;
+        *nrows = (int)b->lbfgs_last_teams.size();
+        for (int k = 0; k < std::min(*nrows, max_rows); ++k) {
+            rows[2 * k] = b->lbfgs_last_teams[k].first;
+            rows[2 * k + 1] = b->lbfgs_last_teams[k].second;
+        }
     });
 }
 
