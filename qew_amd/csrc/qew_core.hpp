@@ -211,23 +211,6 @@ __device__ __forceinline__ double v_from_block(const C (&c)[4][4], const double 
     return vv;
 }
 
-// dV/dy from a held block and the x- and dy-weights: the arithmetic and
-// order of sample_dy64 (rows over y first, then over x, then / dy), so the
-// same value bit for bit.
-template <typename C>
-__device__ __forceinline__ double dy_from_block(const C (&c)[4][4], const double wx[4],
-                                                const double dwy[4], double dy) {
-    double d = 0;
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        double row_dy = 0;
-#pragma unroll
-        for (int b = 0; b < 4; ++b) row_dy += (double)c[a][b] * dwy[b];
-        d += wx[a] * row_dy;
-    }
-    return (dy == 1.0) ? d : d / dy;
-}
-
 // Analytic potentials: the reference tests' functors, evaluated with the
 // device math library (ocml).
 __device__ inline void analytic_sample(const NoiseDesc &nd, double x, double y, double &v,

@@ -53,9 +53,8 @@ constexpr int kMaxM = 32;       // history length limit
 constexpr int kMaxFftN = 8192;  // complex LDS transform length (line length 16384)
 
 // Workspace vectors per line for history length m: x g gn d q edy esl, m + 1
-// S and m + 1 Y rows, two for a team's vector exchange and kLS for the line
-// search's trial dV/dy.
-__host__ __device__ constexpr int lbfgs_nvec(int m) { return 9 + 2 * (m + 1) + kLS; }
+// S and m + 1 Y rows, and two for a team's vector exchange.
+__host__ __device__ constexpr int lbfgs_nvec(int m) { return 9 + 2 * (m + 1); }
 
 template <typename T>
 struct C2 {
@@ -596,14 +595,10 @@ __device__ __forceinline__ void objective_trials_mid(const LineGeom &G, const T 
         });
 }
 
-// With dyt (FP64 state on FP64 tables): dV/dy at each trial point of each
-// edge, from the held block, into dyt[j * dstride + i] (NaN where the trial
-// left the held cell), for the gradient at the accepted step.
 template <typename T, typename C, bool MID = false, bool TEAM = false>
 __device__ __forceinline__ void objective_trials_table(const LineGeom &G, const T *x, const T *d,
                                                        const double *alpha, double (&acc)[kLS],
-                                                       const Team &tm, double *dyt = nullptr,
-                                                       size_t dstride = 0) {
+                                                       const Team &tm) {
     const int n = G.n;
     constexpr int K = kLS;
     if constexpr (MID) {
@@ -663,14 +658,6 @@ __device__ __forceinline__ void objective_trials_table(const LineGeom &G, const 
                     bspline_w(fx, wx);
                     bspline_w(fy, wy);
                     vt[j] = v_from_block(it.c, wx, wy);
-                    if constexpr (sizeof(T) == 8 && std::is_same<C, double>::value)
-                        if (dyt) {
-                            double dwy[4];
-                            bspline_dw(fy, dwy);
-                            dyt[j * dstride + i] = dy_from_block(it.c, wx, dwy, G.nd.dy);
-                        }
-                } else if constexpr (sizeof(T) == 8 && std::is_same<C, double>::value) {
-                    if (dyt) dyt[j * dstride + i] = __builtin_nan("");
                 }
             }
 #pragma unroll
@@ -709,15 +696,14 @@ __device__ __forceinline__ void objective_trials_table(const LineGeom &G, const 
 // those of wg_objective, so phi[j] equals wg_objective(G, x, d, alpha[j]).
 template <typename T, typename C, bool MID = false, bool TEAM = false>
 __device__ __forceinline__ void wg_objective_trials(const LineGeom &G, const T *x, const T *d,
-                                    const double *alpha, double *phi, Red r, Team &tm,
-                                    double *dyt = nullptr, size_t dstride = 0) {
+                                    const double *alpha, double *phi, Red r, Team &tm) {
     const int n = G.n;
     constexpr int K = kLS;
     double acc[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) acc[j] = 0;
     if (G.nd.kind == TABLE) {
-        objective_trials_table<T, C, MID, TEAM>(G, x, d, alpha, acc, tm, dyt, dstride);
+        objective_trials_table<T, C, MID, TEAM>(G, x, d, alpha, acc, tm);
         block_sumk<K>(acc, r.d);
         if constexpr (TEAM) team_sum<K>(tm, acc);
 #pragma unroll
@@ -766,24 +752,17 @@ __device__ __forceinline__ void wg_objective_trials(const LineGeom &G, const T *
 // i + 1); a team member also forms its left halo edge (s0 - 1, s0) from its
 // halo site, into its private edy / esl copies (indexed like x, from s0 - 1).
 // Site pass: g(i) from the edges i - 1 and i.
-// With dyr (FP64 state on FP64 tables, right after an accepted line-search
-// step): the edges' dV/dy as the line search formed them at this step's
-// heights, NaN where it did not (those edges, and a team member's halo edge,
-// are sampled here).
 template <typename T, typename C, bool MID = false, bool TEAM = false>
 __device__ __forceinline__ void wg_gradient(const LineGeom &G, const T *x, T *g, T *edy, T *esl,
-                                            const Team &tm, const double *dyr = nullptr) {
+                                            const Team &tm) {
     const int n = G.n;
     const bool arc = G.model == 1;
     const int s0 = TEAM ? tm.s0 : 0, s1 = TEAM ? tm.s1 : n;
-    constexpr bool kReuse = sizeof(T) == 8 && std::is_same<C, double>::value && !MID;
     stream<kUG<T>>(TEAM ? s0 - 1 : s0, s1, [&](int e) {
                const int ip = (e + 1 == n) ? 0 : e + 1;
-               return P3<T>{x[e], QEW_XR(e, ip),
-                            (kReuse && dyr && !(TEAM && e == s0 - 1)) ? (T)dyr[e]
-                                                                       : (T)__builtin_nan("")};
+               return P2<T>{x[e], QEW_XR(e, ip)};
            },
-           [&](int e, P3<T> v) {
+           [&](int e, P2<T> v) {
                // the edge's place on the line (e = -1 is the wrap edge n - 1)
                const int i = e < 0 ? n - 1 : e;
                const int ip = (i + 1 == n) ? 0 : i + 1;
@@ -791,8 +770,6 @@ __device__ __forceinline__ void wg_gradient(const LineGeom &G, const T *x, T *g,
                    const double dh = div1(v.b - v.a, G.dx);
                    if constexpr (MID)
                        edy[e] = mid_dy64<double>(G.nd, mid_column(G.nd, i, ip), (v.b + v.a) * 0.5);
-                   else if (kReuse && v.c == v.c)
-                       edy[e] = v.c;
                    else
                        edy[e] = sample_dy64<C>(G.nd, (double)(i + ip) * 0.5 * G.dx, (v.b + v.a) * 0.5);
                    if (arc) esl[e] = dh / sqrt(1.0 + dh * dh);
@@ -1115,19 +1092,11 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
         esl += pb;
         // 8-byte aligned words in the last two vectors: n words for FP32
         // elements, 2n for FP64 (the two vectors hold 2 np elements)
-        const uintptr_t bw =
-            reinterpret_cast<uintptr_t>(ws + (size_t)(lbfgs_nvec(m) - kLS - 2) * np);
+        const uintptr_t bw = reinterpret_cast<uintptr_t>(ws + (size_t)(lbfgs_nvec(m) - 2) * np);
         tm.bulk = reinterpret_cast<u64 *>((bw + 7) & ~(uintptr_t)7);
     }
     auto Sb = [&](int k) { return ws + (size_t)(7 + k) * np; };
     auto Yb = [&](int k) { return ws + (size_t)(7 + (m + 1) + k) * np; };
-    // the line search's dV/dy at its trial points (FP64 state on FP64
-    // tables), read back by the gradient at the accepted step
-    constexpr bool kReuseDy = sizeof(T) == 8 && std::is_same<C, double>::value && !MID;
-    double *dyt = nullptr;
-    if constexpr (kReuseDy)
-        if (nd.kind == TABLE)
-            dyt = reinterpret_cast<double *>(ws + (size_t)(lbfgs_nvec(m) - kLS) * np);
     for (int i = s0 + tid; i < s1; i += blockDim.x) x[i] = (T)(h[i] - anchor);
     if (tid == 0) {
         for (int k = 0; k <= m; ++k) sbuf[k] = k;  // sbuf[m] is the spare (s_tmp / y_tmp)
@@ -1323,13 +1292,12 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
         // sequential one
         constexpr int K = kLS;
         bool ok = false;
-        int jacc = 0;  // the accepted trial's place in its pass
         for (int ls = 0; ls < lp.max_ls && !ok;) {
             double al[K], ph[K];
             al[0] = alpha;
 #pragma unroll
             for (int j = 1; j < K; ++j) al[j] = al[j - 1] * lp.ls_shrink;
-            wg_objective_trials<T, C, MID, TEAM>(G, x, d, al, ph, r, tm, dyt, np);
+            wg_objective_trials<T, C, MID, TEAM>(G, x, d, al, ph, r, tm);
 #pragma unroll
             for (int j = 0; j < K; ++j) {
                 if (ok || ls >= lp.max_ls) continue;
@@ -1338,7 +1306,6 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
                 if (ph[j] <= phi0 + lp.c1 * al[j] * dphi0) {
                     phi = ph[j];
                     alpha = al[j];
-                    jacc = j;
                     ok = true;
                 }
             }
@@ -1376,8 +1343,7 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
             x[s1] = x[s1] + alT * d[s1];
         }
         __syncthreads();
-        wg_gradient<T, C, MID, TEAM>(G, x, gn, edy, esl, tm,
-                                     dyt ? dyt + (size_t)jacc * np : nullptr);
+        wg_gradient<T, C, MID, TEAM>(G, x, gn, edy, esl, tm);
         double sy3[3] = {0, 0, 0};
         stream<UV>(s0, s1, [&](int i) { return P3<T>{st[i], gn[i], g[i]}; }, [&](int i, P3<T> v) {
             const T y = v.b - v.c;
