@@ -239,7 +239,10 @@ def bench_lbfgs(log, nlines=512, plain_sample=8, with_cpu=True, omp_info=None):
     # FP32 stops there and is compared by its energy against the FP64 minimum
     for name, noises, dtype, ftol in (("f64", n64, q.F64, 1e-5), ("f32", n32, q.F32, 1e-4),
                                       ("f32_mid", nmid, q.F32, 1e-4),
-                                      ("f64_mid", n64mid, q.F64, 1e-5)):
+                                      ("f64_mid", n64mid, q.F64, 1e-5),
+                                      # FP64 stopped where FP32 stops, for a
+                                      # like-for-like minima/s comparison
+                                      ("f64_ftol1e-4", n64, q.F64, 1e-4)):
         b = q.Batch(lengths, noises, params)
         b.set_precond_shifts(shifts)
         lp = q.LbfgsParams(m=10, ftol=ftol, max_iter=10000, precondition=True)
@@ -335,7 +338,7 @@ def bench_lbfgs(log, nlines=512, plain_sample=8, with_cpu=True, omp_info=None):
     for name, (res, span) in results.items():
         s_state = 8 if name.startswith("f64") else 4
         s_c = 8 if name.startswith("f64") else 4
-        u = u_mid if name.endswith("mid") else u_line
+        u = u_mid if "mid" in name else u_line
         its = np.array([r.iterations for r in res], dtype=np.float64)
         nfe = np.array([r.n_func_evals for r in res], dtype=np.float64)
         per_eval = s_c * u + 2 * n_arr * s_state
@@ -396,6 +399,13 @@ def bench_lbfgs(log, nlines=512, plain_sample=8, with_cpu=True, omp_info=None):
     out["speedup_f64_batched_vs_plain_f64"] = out["f64"]["minima_per_s"] / plain
     out["speedup_f32_mid_batched_vs_plain_f64"] = out["f32_mid"]["minima_per_s"] / plain
     out["speedup_f64_mid_batched_vs_plain_f64"] = out["f64_mid"]["minima_per_s"] / plain
+    # FP32 against FP64 at the same stopping rule (max|grad| < 1e-4)
+    out["f32_vs_f64_at_same_ftol"] = {
+        "ftol": 1e-4, "f64_minima_per_s": out["f64_ftol1e-4"]["minima_per_s"],
+        "f32_minima_per_s": out["f32"]["minima_per_s"],
+        "f32_mid_minima_per_s": out["f32_mid"]["minima_per_s"],
+        "f64_iterations_mean": out["f64_ftol1e-4"]["iterations_mean"],
+        "f32_iterations_mean": out["f32"]["iterations_mean"]}
     out["workload"] = (f"{nlines} lines, L from {{1024..16384}} (sum {sum(lengths)}), "
                        f"tables L x {ny}, xi in {{2,4,8}}, m=10, FFT preconditioner")
     if with_cpu:
@@ -665,6 +675,31 @@ def bench_rk(log, nlines=64, budget_s=60.0, precisions=("f64", "f32_mid"), k_way
     return out
 
 
+def rk_compare(ref_path, path):
+    """f_c brackets of two saved searches of the same batch (--rk-state files):
+    the bracket widths reached, and f_c (bracket midpoints) of the second
+    against the first."""
+    a, b = np.load(ref_path), np.load(path)
+    out = {}
+    for name, z in (("ref", a), ("other", b)):
+        lo, hi = z["lo"], z["hi"]
+        width = (hi - lo) / np.maximum(np.abs(hi), 1e-300)
+        out[name] = {"file": os.path.basename(ref_path if name == "ref" else path),
+                     "lines": int(lo.size), "bracket_rel_width_max": float(width.max()),
+                     "lines_at_1e-10": int((width <= 1e-10).sum()),
+                     "f_c_min_max": [float(lo.min()), float(hi.max())],
+                     "f_c_median": float(np.median(0.5 * (lo + hi)))}
+    fa, fb = 0.5 * (a["lo"] + a["hi"]), 0.5 * (b["lo"] + b["hi"])
+    rel = np.abs(fb - fa) / np.abs(fa)
+    out["other_vs_ref"] = {"f_c_rel_err_max": float(rel.max()),
+                           "f_c_rel_err_median": float(np.median(rel)),
+                           "identical_brackets": int(np.sum((a["lo"] == b["lo"]) &
+                                                            (a["hi"] == b["hi"]))),
+                           "brackets_overlap": int(np.sum((np.maximum(a["lo"], b["lo"]) <=
+                                                           np.minimum(a["hi"], b["hi"]))))}
+    return out
+
+
 def rk_oracle_sample(log, noises, Lr, ny, first, omp_info, nlines):
     """The oracle's rk_block (OpenMP over the sites of a colour) through the
     relaxation at f = 0 and the ramp probe f = 0.125 on the first lines of the
@@ -721,6 +756,9 @@ def main():
                     help="with --rk-only and one precision: continue the f_c search from this "
                          ".npz file if it exists, and save the search state there after the "
                          "ramp and after every round")
+    ap.add_argument("--rk-compare", nargs=2, metavar=("REF_NPZ", "NPZ"), default=None,
+                    help="compare two saved f_c search states (brackets per line; the first "
+                         "is the reference precision) and print the JSON summary (CPU only)")
     ap.add_argument("--rk-precisions", default="f64,f32_mid",
                     help="table types of the Rosso-Krauth workload (f64, f32, f32_mid)")
     ap.add_argument("--rk-kway", type=int, default=4,
@@ -739,6 +777,9 @@ def main():
                     help="only run a 1 GiB device copy (reference point for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE scaling) and exit")
     args = ap.parse_args()
+    if args.rk_compare:
+        print(json.dumps(rk_compare(*args.rk_compare)), flush=True)
+        return
     if args.count_lines:
         print(json.dumps(layout_report(walk(), L, L, NY)), flush=True)
         return
