@@ -253,11 +253,13 @@ def bench_lbfgs(log, nlines=512, plain_sample=8, with_cpu=True, omp_info=None):
         t_tune = time.perf_counter()
         b.lbfgs(h, lp, dtype=dtype)
         t_tune = time.perf_counter() - t_tune
-        wgs, teams = b.lbfgs_plan()
-        team_env = ",".join(f"{n_}:{k_}" for n_, k_ in teams) or "1:1"
+        wgs, team_arr = b.lbfgs_plan()
+        team_count = {f"{L_}x{k_}": int(np.sum((np.array(lengths) == L_) & (team_arr == k_)))
+                      for L_ in sorted(set(lengths)) for k_ in (1, 2, 4)
+                      if np.any((np.array(lengths) == L_) & (team_arr == k_))}
         h = h0.clone()
         torch.cuda.synchronize()
-        log(f"lbfgs {name}: launching (team policy {team_env}, picked in {t_tune:.2f} s)")
+        log(f"lbfgs {name}: launching (lines x workgroups {team_count}, picked in {t_tune:.2f} s)")
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         t1 = time.perf_counter()
         ev0.record()
@@ -274,11 +276,11 @@ def bench_lbfgs(log, nlines=512, plain_sample=8, with_cpu=True, omp_info=None):
         gmax[name] = np.array([ga[o0:o1].max() for o0, o1 in zip(offs[:-1], offs[1:])])
         its = [r.iterations for r in res]
         results[name] = (res, span)
-        # single lines below run with the batch's team sizes
-        os.environ["QEW_LBFGS_TEAMS"] = team_env
+        # single lines below run on as many workgroups as in the batch
         # the critical path: the line with the most site-iterations, solved
         # alone (on its team size in the batch), against the whole launch
         crit = int(np.argmax(np.array(its) * np.array(lengths)))
+        os.environ["QEW_LBFGS_TEAM"] = str(int(team_arr[crit]))
         b1 = q.Batch([lengths[crit]], [noises[crit]], [params[crit]])
         b1.set_precond_shifts([shifts[crit]])
         h1 = torch.full((lengths[crit],), ny / 2.0, dtype=torch.float64, device="cuda:0")
@@ -295,6 +297,7 @@ def bench_lbfgs(log, nlines=512, plain_sample=8, with_cpu=True, omp_info=None):
         busy_sample = 0.0
         sample = list(range(0, nlines, 8))
         for k in sample:
+            os.environ["QEW_LBFGS_TEAM"] = str(int(team_arr[k]))
             bk = q.Batch([lengths[k]], [noises[k]], [params[k]])
             bk.set_precond_shifts([shifts[k]])
             hk = torch.full((lengths[k],), ny / 2.0, dtype=torch.float64, device="cuda:0")
@@ -304,13 +307,13 @@ def bench_lbfgs(log, nlines=512, plain_sample=8, with_cpu=True, omp_info=None):
             busy_sample += (time.perf_counter() - t3) * bk.lbfgs_workgroups()
             del bk, hk
         busy = busy_sample * nlines / len(sample)
-        del os.environ["QEW_LBFGS_TEAMS"]
+        del os.environ["QEW_LBFGS_TEAM"]
         out[name] = {"minima_per_s": nlines / dt, "seconds": dt, "ftol": ftol,
                      "launch_s": span,
-                     "team_policy": {"rows": teams, "picked_in_s": t_tune,
-                                     "rule": "a line of length >= n runs on k workgroups "
-                                             "(first matching row), else one"},
+                     "teams": {"lines_by_length_x_workgroups": team_count,
+                               "picked_in_s": t_tune},
                      "critical_line": {"line": crit, "L": int(lengths[crit]),
+                                       "workgroups": int(team_arr[crit]),
                                        "iterations": int(its[crit]), "alone_s": alone,
                                        "alone_over_launch": alone / span},
                      "converged": int(sum(r.converged for r in res)),

@@ -325,18 +325,17 @@ qew_status qew_lbfgs_minimize(qew_ctx ctx, qew_noise noise, const qew_params *p,
 qew_status qew_batch_lbfgs(qew_batch batch, int dtype, const qew_lbfgs_params *lp,
                            double *h, qew_lbfgs_result *results);
 /* How the batch's last qew_batch_lbfgs launch ran: its persistent
- * workgroups, and its team policy as *nrows rows (minimum line length,
- * workgroups per line) in rows[2k], rows[2k + 1] (at most max_rows written):
- * a line takes the team size of the first row whose length it reaches, else
- * one workgroup. The first call for a batch (per precision and
- * preconditioner setting) times the candidate policies -- one workgroup per
- * line, teams of 2 or 4 on the longest length, of 2 on the next -- on a copy
- * of the line state and keeps the fastest; QEW_LBFGS_TEAM=k (k workgroups
- * for every line whose length k divides into segments of at least 64 sites)
- * or QEW_LBFGS_TEAMS="n1:k1,n2:k2" set the policy instead. No reference
- * counterpart. */
-qew_status qew_batch_lbfgs_plan(qew_batch batch, int *workgroups, int *rows, int max_rows,
-                                int *nrows);
+ * workgroups, and (team_per_line: host int[nlines], or NULL) how many
+ * workgroups solved each line. The first call for a batch (per precision and
+ * preconditioner setting) times candidates on a copy of the line state and
+ * keeps the fastest: one workgroup per line; teams of 2 or 4 on the longest
+ * length and of 2 on the next; and, from the iteration counts of the first
+ * candidate, teams of 4 and 2 on the lines with the most site-iterations.
+ * QEW_LBFGS_TEAM=k (k workgroups on every line that k divides into segments
+ * of at least 64 sites) or QEW_LBFGS_TEAMS="n1:k1,n2:k2" (k1 on lines of n1
+ * sites and more, else k2 on lines of n2 and more, else 1) set them instead.
+ * No reference counterpart. */
+qew_status qew_batch_lbfgs_plan(qew_batch batch, int *workgroups, int *team_per_line);
 
 /* ---- FIRE (src/fire.h:41-131) ------------------------------------------- */
 

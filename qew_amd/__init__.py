@@ -599,14 +599,13 @@ class Batch:
         return [LbfgsResult.from_c(r) for r in res]
 
     def lbfgs_plan(self):
-        """(workgroups, team policy) of the last lbfgs() launch: the persistent
-        workgroups and a list of (minimum line length, workgroups per line)
-        rows; "n1:k1,..." of the rows sets the same policy through
-        QEW_LBFGS_TEAMS."""
-        wg, nr = C.c_int(0), C.c_int(0)
-        rows = (C.c_int * 16)()
-        check(_lib.load().qew_batch_lbfgs_plan(self.handle, C.byref(wg), rows, 8, C.byref(nr)))
-        return wg.value, [(rows[2 * k], rows[2 * k + 1]) for k in range(min(nr.value, 8))]
+        """(workgroups, team sizes) of the last lbfgs() launch: the persistent
+        workgroups and how many of them solved each line (numpy int array)."""
+        wg = C.c_int(0)
+        t = np.zeros(self.nlines, dtype=np.int32)
+        check(_lib.load().qew_batch_lbfgs_plan(self.handle, C.byref(wg),
+                                               t.ctypes.data_as(C.POINTER(C.c_int))))
+        return wg.value, t
 
     def lbfgs_workgroups(self):
         """Persistent workgroups of the last lbfgs() launch."""

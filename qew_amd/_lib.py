@@ -104,7 +104,7 @@ SIGNATURES = [
     ("qew_lbfgs_minimize", [VP, VP, C.POINTER(Params), C.POINTER(LbfgsParams), I, VP,
                             C.POINTER(LbfgsResult)]),
     ("qew_batch_lbfgs", [VP, I, C.POINTER(LbfgsParams), VP, C.POINTER(LbfgsResult)]),
-    ("qew_batch_lbfgs_plan", [VP, C.POINTER(I), C.POINTER(I), I, C.POINTER(I)]),
+    ("qew_batch_lbfgs_plan", [VP, C.POINTER(I), C.POINTER(I)]),
     ("qew_roughness", [VP, I, I, D, VP, VP, VP, VP]),
     ("qew_fire_minimize", [VP, VP, C.POINTER(Params), C.POINTER(FireParams), I, VP,
                            C.POINTER(FireResult)]),

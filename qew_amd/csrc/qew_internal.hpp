@@ -163,14 +163,14 @@ struct qew_batch_s {
     // 2 per-colour launches
     int rk_path = 0;
     qew_host::DevBuf d_ltile, d_tune;
-    // persistent workgroups of the last qew_batch_lbfgs launch and the team
-    // policy rows it ran with (minimum line length, workgroups per line)
+    // persistent workgroups of the last qew_batch_lbfgs launch and the
+    // workgroups it gave each line
     int lbfgs_workgroups = 0;
-    std::vector<std::pair<int, int>> lbfgs_last_teams;
-    // team policy picked by timing, per (FP32 state, preconditioner) pair
+    std::vector<int> lbfgs_last_team;
+    // workgroups per line picked by timing, per (FP32 state, preconditioner)
     struct TunedTeams {
         bool tuned = false;
-        std::vector<std::pair<int, int>> rows;
+        std::vector<int> team;
     } lbfgs_teams[4];
 };
 

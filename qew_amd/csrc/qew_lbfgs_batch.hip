@@ -1527,6 +1527,33 @@ std::vector<TeamPolicy> lbfgs_team_candidates(const std::vector<LineDesc> &lines
     return out;
 }
 
+// Per-line team sizes from the site-iterations n (iterations + 1) each line
+// took on one workgroup: 4 workgroups on the lines within a factor f4 of the
+// largest, 2 within f2, 1 otherwise (where the size divides the line into
+// segments of at least 64 sites), for (f4, f2) in a few splits.
+std::vector<std::vector<int>> lbfgs_work_team_candidates(const std::vector<LineDesc> &lines,
+                                                         const std::vector<qew_lbfgs_result> &r) {
+    std::vector<double> work(lines.size());
+    double wmax = 0;
+    for (size_t l = 0; l < lines.size(); ++l) {
+        work[l] = (double)lines[l].n * (double)(r[l].iterations + 1);
+        wmax = std::max(wmax, work[l]);
+    }
+    auto fits = [](int n, int k) { return n % k == 0 && n / k >= 64; };
+    std::vector<std::vector<int>> out;
+    for (const auto &f : {std::make_pair(0.7, 0.35), std::make_pair(0.5, 0.3),
+                          std::make_pair(2.0, 0.45)}) {
+        std::vector<int> t(lines.size(), 1);
+        for (size_t l = 0; l < lines.size(); ++l) {
+            const int n = lines[l].n;
+            if (work[l] >= f.first * wmax && fits(n, 4)) t[l] = 4;
+            else if (work[l] >= f.second * wmax && fits(n, 2)) t[l] = 2;
+        }
+        out.push_back(t);
+    }
+    return out;
+}
+
 int lbfgs_team_size(int n, const TeamPolicy &p) {
     auto fits = [&](int k) { return k >= 1 && k <= kMaxTeam && n % k == 0 && n / k >= 64; };
     if (p.forced > 0) return fits(p.forced) ? p.forced : 1;
@@ -1833,9 +1860,13 @@ extern "C" qew_status qew_batch_lbfgs(qew_batch b, int dtype, const qew_lbfgs_pa
         const size_t lds = (size_t)maxN * 2 * es;
         // One launch on line state hh with the team sizes of `policy`;
         // returns its workgroups.
-        auto run = [&](const TeamPolicy &policy, double *hh) {
-            std::vector<int> team(b->nlines), slot_line, slot_member;
+        auto teams_of = [&](const TeamPolicy &policy) {
+            std::vector<int> team(b->nlines);
             for (int l = 0; l < b->nlines; ++l) team[l] = lbfgs_team_size(b->lines[l].n, policy);
+            return team;
+        };
+        auto run = [&](const std::vector<int> &team, double *hh) {
+            std::vector<int> slot_line, slot_member;
             for (const int l : order)
                 for (int k = 0; k < team[l]; ++k) {
                     slot_line.push_back(l);
@@ -1891,59 +1922,74 @@ extern "C" qew_status qew_batch_lbfgs(qew_batch b, int dtype, const qew_lbfgs_pa
         // choice for this precision and preconditioner setting, picked on
         // the first call by timing every candidate on a copy of the line
         // state (each candidate reaches the same minima within the
-        // reductions' rounding; only time differs).
-        TeamPolicy policy = lbfgs_team_policy();
-        const bool forced = policy.forced > 0 || std::getenv("QEW_LBFGS_TEAMS") != nullptr;
+        // reductions' rounding; only time differs). Candidates: the
+        // length-based policies of lbfgs_team_candidates, then, from the
+        // iteration counts the one-workgroup run measured, teams on the lines
+        // with the most site-iterations (the launch's critical path).
+        const TeamPolicy env_policy = lbfgs_team_policy();
+        const bool forced = env_policy.forced > 0 || std::getenv("QEW_LBFGS_TEAMS") != nullptr;
         const int key = (dtype == QEW_F32 ? 1 : 0) + (lp->precondition ? 2 : 0);
-        if (!forced) {
+        std::vector<int> team;
+        if (forced) {
+            team = teams_of(env_policy);
+        } else {
             auto &tuned = b->lbfgs_teams[key];
             if (!tuned.tuned) {
                 const std::vector<TeamPolicy> cands = lbfgs_team_candidates(b->lines);
                 tuned.tuned = true;
-                tuned.rows = cands.front().rows;
+                tuned.team = teams_of(cands.front());
                 if (cands.size() > 1) {
                     const size_t bytes = sizeof(double) * (size_t)b->total;
                     b->d_tune.ensure(bytes);
                     double *copy = b->d_tune.as<double>();
                     double best = 1e300;
-                    for (const TeamPolicy &c : cands) {
+                    auto time_one = [&](const std::vector<int> &t) {
                         QEW_HIP(hipMemcpyAsync(copy, h, bytes, hipMemcpyDeviceToDevice,
                                                ctx->stream));
                         QEW_HIP(hipStreamSynchronize(ctx->stream));
                         const auto t0 = std::chrono::steady_clock::now();
-                        run(c, copy);
-                        const double t = std::chrono::duration<double>(
-                                             std::chrono::steady_clock::now() - t0)
-                                             .count();
-                        if (t < best) {
-                            best = t;
-                            tuned.rows = c.rows;
+                        run(t, copy);
+                        const double dt = std::chrono::duration<double>(
+                                              std::chrono::steady_clock::now() - t0)
+                                              .count();
+                        if (dt < best) {
+                            best = dt;
+                            tuned.team = t;
+                        }
+                    };
+                    std::vector<qew_lbfgs_result> solo(b->nlines);
+                    for (size_t c = 0; c < cands.size(); ++c) {
+                        time_one(teams_of(cands[c]));
+                        if (c == 0) {  // one workgroup per line: the lines' iterations
+                            QEW_HIP(hipMemcpyAsync(solo.data(), a.results,
+                                                   sizeof(qew_lbfgs_result) * b->nlines,
+                                                   hipMemcpyDeviceToHost, ctx->stream));
+                            QEW_HIP(hipStreamSynchronize(ctx->stream));
                         }
                     }
+                    for (const auto &t : lbfgs_work_team_candidates(b->lines, solo))
+                        time_one(t);
                 }
             }
-            policy.rows = tuned.rows;
+            team = tuned.team;
         }
-        const int grid = run(policy, h);
+        const int grid = run(team, h);
         QEW_HIP(hipMemcpyAsync(results, a.results, sizeof(qew_lbfgs_result) * b->nlines,
                                hipMemcpyDeviceToHost, ctx->stream));
         QEW_HIP(hipStreamSynchronize(ctx->stream));
         b->lbfgs_workgroups = grid;
-        b->lbfgs_last_teams.clear();
-        for (const auto &row : policy.rows) b->lbfgs_last_teams.push_back(row);
+        b->lbfgs_last_team = team;
     });
 }
 
-extern "C" qew_status qew_batch_lbfgs_plan(qew_batch b, int *workgroups, int *rows, int max_rows,
-                                           int *nrows) {
+extern "C" qew_status qew_batch_lbfgs_plan(qew_batch b, int *workgroups, int *team_per_line) {
     return guard([&] {
-        QEW_REQUIRE(b && workgroups && nrows && (rows || max_rows == 0), "null pointer");
+        QEW_REQUIRE(b && workgroups, "null pointer");
+        QEW_REQUIRE((int)b->lbfgs_last_team.size() == b->nlines,
+                    "no qew_batch_lbfgs call on this batch yet");
         *workgroups = b->lbfgs_workgroups;
-        *nrows = (int)b->lbfgs_last_teams.size();
-        for (int k = 0; k < std::min(*nrows, max_rows); ++k) {
-            rows[2 * k] = b->lbfgs_last_teams[k].first;
-            rows[2 * k + 1] = b->lbfgs_last_teams[k].second;
-        }
+        if (team_per_line)
+            for (int l = 0; l < b->nlines; ++l) team_per_line[l] = b->lbfgs_last_team[l];
     });
 }
 
