@@ -655,7 +655,7 @@ def bench_rk(log, nlines=64, budget_s=60.0, precisions=("f64", "f32_mid"), k_way
                      "lines_at_1e-10": int((width <= 1e-10).sum())}
         log(f"rk {name}: {json.dumps({k: v for k, v in out[name].items() if k != 'round_log'})}")
     for name in precisions:
-        if name == "f64":
+        if name == "f64" or "f64" not in brackets:
             continue
         lo, hi = brackets[name]
         lo64, hi64 = brackets["f64"]
