@@ -127,21 +127,6 @@ __device__ __forceinline__ void stream(int n, Ld ld, Use use) {
     stream<U>(0, n, ld, use);
 }
 
-// One element per thread at a time over [i0, i1): use(i, mk(i, pre(i))),
-// where pre(i + kLB) (the next element's cheap operands) is issued while
-// mk(i)'s loads (a table gather) are in flight.
-template <typename Pre, typename Mk, typename Use>
-__device__ __forceinline__ void stream_pf(int i0, int i1, Pre pre, Mk mk, Use use) {
-    int i = i0 + (int)threadIdx.x;
-    if (i >= i1) return;
-    auto p = pre(i);
-    for (; i < i1; i += kLB) {
-        const auto it = mk(i, p);
-        if (i + kLB < i1) p = pre(i + kLB);
-        use(i, it);
-    }
-}
-
 template <typename T>
 struct P2 {
     T a, b;
@@ -638,24 +623,21 @@ __device__ __forceinline__ void objective_trials_table(const LineGeom &G, const 
         int iy;
         B c[4][4];
     };
-    // the edge's line values, then its trial-0 cell and held block
-    auto pre = [&](int i) {
-        const int ip = (i + 1 == n) ? 0 : i + 1;
-        return P4<T>{x[i], d[i], QEW_XR(i, ip), QEW_DR(i, ip)};
-    };
-    auto mk = [&](int i, const P4<T> &v) {
-        const int ip = (i + 1 == n) ? 0 : i + 1;
-        Item it;
-        it.v = v;
-        double hi, hip, fx, fy;
-        heights(it.v, 0, hi, hip);
-        int ixn[4], iyn[4];
-        locate64(G.nd, (double)(i + ip) * 0.5 * G.dx, (hip + hi) * 0.5, ixn, iyn, fx, fy);
-        it.iy = iyn[1];
-        gather_block64<B>(G.nd, ixn[1], iyn, it.c);
-        return it;
-    };
-    auto use = [&](int i, const Item &it) {
+    stream<kUT<T>>(
+        (TEAM ? tm.s0 : 0), (TEAM ? tm.s1 : n),
+        [&](int i) {
+            const int ip = (i + 1 == n) ? 0 : i + 1;
+            Item it;
+            it.v = P4<T>{x[i], d[i], QEW_XR(i, ip), QEW_DR(i, ip)};
+            double hi, hip, fx, fy;
+            heights(it.v, 0, hi, hip);
+            int ixn[4], iyn[4];
+            locate64(G.nd, (double)(i + ip) * 0.5 * G.dx, (hip + hi) * 0.5, ixn, iyn, fx, fy);
+            it.iy = iyn[1];
+            gather_block64<B>(G.nd, ixn[1], iyn, it.c);
+            return it;
+        },
+        [&](int i, const Item &it) {
             const int ip = (i + 1 == n) ? 0 : i + 1;
             const double xc = (double)(i + ip) * 0.5 * G.dx;
             // trials in the held cell first; the others are sampled after the
@@ -706,12 +688,7 @@ __device__ __forceinline__ void objective_trials_table(const LineGeom &G, const 
                 }
                 acc[j] += G.dx * (le + vv - G.f * hi);
             }
-        };
-    const int i0 = TEAM ? tm.s0 : 0, i1 = TEAM ? tm.s1 : n;
-    if constexpr (kUT<T> == 1)
-        stream_pf(i0, i1, pre, mk, use);
-    else
-        stream<kUT<T>>(i0, i1, [&](int i) { return mk(i, pre(i)); }, use);
+        });
 }
 
 // Energies at the kLS trial points x + alpha[j] d in one pass over the line.
