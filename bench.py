@@ -348,7 +348,8 @@ def bench_lbfgs(log, nlines=512, plain_sample=8, with_cpu=True, omp_info=None):
         bytes_total = float((its * (2 * m_hist + 8) * n_arr * s_state
                              + (nfe + its + 1) * per_eval).sum())
         ach = bytes_total / span / 1e9
-        out[name]["roofline"] = {"bound": "latency (one workgroup per line)", "achieved": ach,
+        out[name]["roofline"] = {"bound": "latency (the critical line's iteration chain, "
+                                 "one workgroup or a team of them per line)", "achieved": ach,
                                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                  "frac": ach / HBM_PEAK_GBS, "bytes": bytes_total}
     rel_err = np.abs(finals["f32"] - finals["f64"]) / np.abs(finals["f64"])
