@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "qew_div6.hpp"
+
 namespace qew_dev {
 
 constexpr int kWave = 64;
@@ -61,23 +63,27 @@ struct LineDesc {
 // ---------------------------------------------------------------------------
 // FP64 restatement of the reference sampler (src/filtered_noise.h:33-176).
 // ---------------------------------------------------------------------------
+// The weights of src/filtered_noise.h:33-61, bit for bit (the divisions by
+// 6 through div6: the numerators 4 - 6 t^2 + 3 t^3 and 1 + 3 t + ... are at
+// least 1, (1 - t)^3 >= 2^-159 and 3 + 6 t - 9 t^2 is 0 or a normal number,
+// so their quotients are normal; t^3, -12 t + 9 t^2 and 3 t^2 can be tiny).
 __device__ __forceinline__ void bspline_w(double t, double w[4]) {
     const double t2 = t * t;
     const double t3 = t2 * t;
     const double omt = 1.0 - t;
-    w[0] = omt * omt * omt / 6.0;
-    w[1] = (4.0 - 6.0 * t2 + 3.0 * t3) / 6.0;
-    w[2] = (1.0 + 3.0 * t + 3.0 * t2 - 3.0 * t3) / 6.0;
-    w[3] = t3 / 6.0;
+    w[0] = qew_div::div6(omt * omt * omt);
+    w[1] = qew_div::div6(4.0 - 6.0 * t2 + 3.0 * t3);
+    w[2] = qew_div::div6(1.0 + 3.0 * t + 3.0 * t2 - 3.0 * t3);
+    w[3] = qew_div::div6_sub(t3);
 }
 
 __device__ __forceinline__ void bspline_dw(double t, double dw[4]) {
     const double t2 = t * t;
     const double omt = 1.0 - t;
     dw[0] = -omt * omt / 2.0;
-    dw[1] = (-12.0 * t + 9.0 * t2) / 6.0;
-    dw[2] = (3.0 + 6.0 * t - 9.0 * t2) / 6.0;
-    dw[3] = 3.0 * t2 / 6.0;
+    dw[1] = qew_div::div6_sub(-12.0 * t + 9.0 * t2);
+    dw[2] = qew_div::div6(3.0 + 6.0 * t - 9.0 * t2);
+    dw[3] = qew_div::div6_sub(3.0 * t2);
 }
 
 // Workgroups are dispatched round-robin over the part's 8 XCDs (block b on

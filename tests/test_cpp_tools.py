@@ -35,6 +35,15 @@ def test_cli_parser_selftest(tmp_path):
     assert r.returncode == 0, r.stdout
 
 
+def test_division_free_weights_selftest(tmp_path):
+    """tests/cpp/div6_selftest.cpp: the device weights' x / 6 without a
+    division equals the correctly rounded division (host build of the same
+    header)."""
+    exe = compile_cpp("div6_selftest.cpp", str(tmp_path / "div6_selftest"))
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout
+
+
 @pytest.mark.parametrize("name", ["qew_static", "qew_dynamic", "qew_depinning", "qew_benchmark"])
 def test_tools_help_and_option_errors(name):
     exe = tool(name)
