@@ -12,7 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "qew_div6.hpp"
+#include "qew_div.hpp"
 
 namespace qew_dev {
 
@@ -43,6 +43,7 @@ struct NoiseDesc {
     double dx, dy;
     const void *coef;
     double p[4];
+    double rdx, rdy;  // RN(1 / dx), RN(1 / dy)
 };
 
 // One line of a batch: sites [off, off + n) of the batch's line buffers.
@@ -100,6 +101,22 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 // a / b, skipping the division when b is exactly 1 (a / 1.0 == a bit for bit)
 __device__ __forceinline__ double div1(double a, double b) { return b == 1.0 ? a : a / b; }
 
+// a / b for a spacing b > 0 with rb = RN(1 / b) at hand: a itself when b is
+// exactly 1, else the division-free quotient (correctly rounded whenever it
+// is normal or zero). Unlike div1 no division is evaluated on either side.
+__device__ __forceinline__ double divr(double a, double b, double rb) {
+    return b == 1.0 ? a : qew_div::divp(a, b, rb);
+}
+
+// A value the same on every lane, moved to scalar registers (the line
+// constants computed on the vector unit would otherwise hold VGPRs).
+__device__ __forceinline__ double uniform_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // g - floor(g / n) * n. For 0 <= g <= fl(n (1 - 2^-52)) the correctly rounded
 // g / n is at most 1 - 2^-53, so the floor is 0 and g comes back unchanged:
 // the division is skipped there with the same result bit for bit.
@@ -115,7 +132,7 @@ __device__ __forceinline__ void locate64(const NoiseDesc &nd, double x, double y
                                          int iyn[4], double &fx, double &fy) {
     const int nx = nd.nx, ny = nd.ny;
     double gx = (nd.dx == 1.0) ? x : x / nd.dx;
-    double gy = (nd.dy == 1.0) ? y : y / nd.dy;
+    double gy = divr(y, nd.dy, nd.rdy);
     gx = wrap_cells(gx, nx);
     gy = wrap_cells(gy, ny);
     int ix = (int)floor(gx);
@@ -283,7 +300,7 @@ __device__ __forceinline__ void sample_v_dy64(const NoiseDesc &nd, double x, dou
         dd += wx[a] * row_dy;
     }
     v = vv;
-    dvdy = (nd.dy == 1.0) ? dd : dd / nd.dy;
+    dvdy = divr(dd, nd.dy, nd.rdy);
 }
 
 template <typename C>
@@ -315,7 +332,7 @@ __device__ __forceinline__ double sample_dy64(const NoiseDesc &nd, double x, dou
         for (int b = 0; b < 4; ++b) row_dy += c[a][b] * dwy[b];
         d += wx[a] * row_dy;
     }
-    return (nd.dy == 1.0) ? d : d / nd.dy;
+    return divr(d, nd.dy, nd.rdy);
 }
 
 template <typename C>
@@ -618,6 +635,7 @@ struct LineGeom {
     int n;
     int model;
     double dx, lt, f, L;
+    double rdx, rdx2;  // RN(1 / dx), RN(1 / dx^2), dx^2 rounded
     NoiseDesc nd;
     // FP32 path
     YAnchor ya;
@@ -632,6 +650,8 @@ __device__ __forceinline__ LineGeom make_geom(const LineDesc &L, const NoiseDesc
     g.n = L.n;
     g.model = L.model;
     g.dx = L.L / (double)L.n;
+    g.rdx = uniform_f64(1.0 / g.dx);
+    g.rdx2 = uniform_f64(1.0 / (g.dx * g.dx));
     g.lt = L.lt;
     g.f = L.f;
     g.L = L.L;
@@ -745,7 +765,7 @@ __device__ __forceinline__ int mid_column(const NoiseDesc &nd, int e, int ep) {
 __device__ __forceinline__ void mid_locate_y64(const NoiseDesc &nd, double y, int iyn[4],
                                                double &fy) {
     const int ny = nd.ny;
-    double gy = (nd.dy == 1.0) ? y : y / nd.dy;
+    double gy = divr(y, nd.dy, nd.rdy);
     gy = wrap_cells(gy, ny);
     int iy = (int)floor(gy);
     if (iy >= ny) { iy -= ny; gy -= (double)ny; }
@@ -793,7 +813,7 @@ __device__ __forceinline__ double mid_dy64(const NoiseDesc &nd, int ix, double y
     double d = 0;
 #pragma unroll
     for (int b = 0; b < 4; ++b) d += (double)c[b] * dwy[b];
-    return (nd.dy == 1.0) ? d : d / nd.dy;
+    return divr(d, nd.dy, nd.rdy);
 }
 
 // The 4 x 4 window path of an FP32 or BF16 table.

@@ -1,8 +1,8 @@
-// x / 6.0, correctly rounded, without a division: the cubic B-spline weights
-// of src/filtered_noise.h:33-61 divide by 6 seven times per sample, and a
-// double division is a long instruction sequence on the device. Host and
-// device share these definitions (tests/cpp/div6_selftest.cpp checks them
-// against the division on the host).
+// Correctly rounded double quotients without a division instruction
+// sequence (long on the device): x / 6 for the cubic B-spline weights of
+// src/filtered_noise.h:33-61 (seven per sample), and a / b by the grid and
+// line spacings. Host and device share these definitions
+// (tests/cpp/div_selftest.cpp checks them against the division on the host).
 #pragma once
 
 #if defined(__HIPCC__)
@@ -34,6 +34,16 @@ QEW_HD inline double div6_sub(double x) {
     const double e = __builtin_fma(-q, 6.0, x);
     const long long b = __builtin_bit_cast(long long, q);
     return (__builtin_fabs(e) == 0x3p-1074 && (b & 1)) ? __builtin_bit_cast(double, b + 1) : q;
+}
+
+// a / b for b > 0 given rb = RN(1 / b): the same correction step, exact for
+// b = 1 (rb = 1, no correction) and correctly rounded for any b whenever the
+// quotient is normal or zero (Markstein's theorem: rb within half an ulp of
+// 1 / b and a rb within one ulp of a / b).
+QEW_HD inline double divp(double a, double b, double rb) {
+    const double q = a * rb;
+    const double e = __builtin_fma(-q, b, a);
+    return __builtin_copysign(__builtin_fma(e, rb, q), a);
 }
 
 }  // namespace qew_div

@@ -491,7 +491,7 @@ __device__ double wg_objective(const LineGeom &G, const T *x, const T *d, double
                    const int ip = (i + 1 == n) ? 0 : i + 1;
                    const double oi = v.a, oip = v.b;
                    const double hi = G.anchor + oi, hip = G.anchor + oip;
-                   const double dh = div1(oip - oi, G.dx);
+                   const double dh = divr(oip - oi, G.dx, G.rdx);
                    const double s2 = dh * dh;
                    const double le =
                        (G.model == 0) ? G.lt * s2 * 0.5 : G.lt * (s2 / (sqrt(1.0 + s2) + 1.0));
@@ -583,7 +583,7 @@ __device__ __forceinline__ void objective_trials_mid(const LineGeom &G, const T 
                     mid_column4(G.nd, ix, iyn, c);
                     vv = mid_v_from(c, fy);
                 }
-                const double dh = div1(oip - oi, G.dx);
+                const double dh = divr(oip - oi, G.dx, G.rdx);
                 const double s2 = dh * dh;
                 double le;
                 if constexpr (sizeof(T) == 8)  // src/qew_model.h:53-99
@@ -676,13 +676,13 @@ __device__ __forceinline__ void objective_trials_table(const LineGeom &G, const 
                 double le;
                 if constexpr (sizeof(T) == 8) {
                     // src/qew_model.h:53-99 (TrialAt: x0(i) + alpha * d(i))
-                    const double dh = div1(hip - hi, G.dx);
+                    const double dh = divr(hip - hi, G.dx, G.rdx);
                     le = (G.model == 0) ? G.lt * dh * dh * 0.5 : G.lt * (sqrt(1.0 + dh * dh) - 1.0);
                 } else {
                     const float al = (float)alpha[j];
                     const double oi = (double)__builtin_fmaf(al, it.v.b, it.v.a);
                     const double oip = (double)__builtin_fmaf(al, it.v.d, it.v.c);
-                    const double dh = div1(oip - oi, G.dx);
+                    const double dh = divr(oip - oi, G.dx, G.rdx);
                     const double s2 = dh * dh;
                     le = (G.model == 0) ? G.lt * s2 * 0.5 : G.lt * (s2 / (sqrt(1.0 + s2) + 1.0));
                 }
@@ -733,7 +733,7 @@ __device__ __forceinline__ void wg_objective_trials(const LineGeom &G, const T *
                     const double oi = (double)__builtin_fmaf(al, v.b, v.a);
                     const double oip = (double)__builtin_fmaf(al, v.d, v.c);
                     const double hi = G.anchor + oi, hip = G.anchor + oip;
-                    const double dh = div1(oip - oi, G.dx);
+                    const double dh = divr(oip - oi, G.dx, G.rdx);
                     const double s2 = dh * dh;
                     const double le = (G.model == 0) ? G.lt * s2 * 0.5
                                                      : G.lt * (s2 / (sqrt(1.0 + s2) + 1.0));
@@ -767,7 +767,7 @@ __device__ __forceinline__ void wg_gradient(const LineGeom &G, const T *x, T *g,
                const int i = e < 0 ? n - 1 : e;
                const int ip = (i + 1 == n) ? 0 : i + 1;
                if constexpr (sizeof(T) == 8) {
-                   const double dh = div1(v.b - v.a, G.dx);
+                   const double dh = divr(v.b - v.a, G.dx, G.rdx);
                    if constexpr (MID)
                        edy[e] = mid_dy64<double>(G.nd, mid_column(G.nd, i, ip), (v.b + v.a) * 0.5);
                    else
@@ -807,8 +807,8 @@ __device__ __forceinline__ void wg_gradient(const LineGeom &G, const T *x, T *g,
            [&](int i, Nb b) {
                if constexpr (sizeof(T) == 8) {
                    double lf;
-                   if (!arc) lf = div1(-G.lt * (b.xm + b.xp - 2.0 * b.xi), G.dx * G.dx);
-                   else lf = div1(G.lt * (b.sm - b.si), G.dx);
+                   if (!arc) lf = divr(-G.lt * (b.xm + b.xp - 2.0 * b.xi), G.dx * G.dx, G.rdx2);
+                   else lf = divr(G.lt * (b.sm - b.si), G.dx, G.rdx);
                    g[i] = G.dx * (lf + 0.5 * (b.em + b.ei) - G.f);
                } else {
                    float lf;

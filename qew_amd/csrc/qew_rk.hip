@@ -68,7 +68,7 @@ struct MidF32 {};
 
 // Column of the x cell of an edge midpoint (locate64's x part).
 __device__ __forceinline__ int mid_cell_x(const NoiseDesc &nd, double x) {
-    double gx = (nd.dx == 1.0) ? x : x / nd.dx;
+    double gx = divr(x, nd.dx, nd.rdx);
     gx = wrap_cells(gx, nd.nx);
     int ix = (int)floor(gx);
     if (ix >= nd.nx) ix -= nd.nx;
@@ -90,7 +90,7 @@ __device__ __forceinline__ double edge_dy(const NoiseDesc &nd, double x, double 
         double d = 0;
 #pragma unroll
         for (int b = 0; b < 4; ++b) d += (double)c[b] * dwy[b];
-        return (nd.dy == 1.0) ? d : d / nd.dy;
+        return divr(d, nd.dy, nd.rdy);
     } else {
         return sample_dy64<C>(nd, x, y);
     }
@@ -157,7 +157,7 @@ struct EdgeDy {
             for (int b = 0; b < 4; ++b) row_dy += (double)c[a][b] * dwy[b];
             d += wx[a] * row_dy;
         }
-        return (nd.dy == 1.0) ? d : d / nd.dy;
+        return divr(d, nd.dy, nd.rdy);
     }
 };
 
@@ -180,7 +180,7 @@ struct EdgeDy<MidF32> {
         double d = 0;
 #pragma unroll
         for (int b = 0; b < 4; ++b) d += (double)c[b] * dwy[b];
-        return (nd.dy == 1.0) ? d : d / nd.dy;
+        return divr(d, nd.dy, nd.rdy);
     }
 };
 

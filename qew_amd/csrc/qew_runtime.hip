@@ -207,6 +207,8 @@ qew_noise new_noise(qew_ctx ctx, int kind, int dtype, int nx, int ny, double dx,
     nz->desc.ny = ny;
     nz->desc.dx = dx;
     nz->desc.dy = dy;
+    nz->desc.rdx = 1.0 / dx;
+    nz->desc.rdy = 1.0 / dy;
     return nz;
 }
 

@@ -1,15 +1,17 @@
-// The division-free x / 6 of the device B-spline weights (qew_div::div6 for
+// The division-free quotients of the device sampler (qew_div::div6 for
 // numerators whose quotient is normal or zero, qew_div::div6_sub for any
-// numerator) against the correctly rounded division x / 6.0, on the host:
+// numerator, qew_div::divp by a spacing) against the correctly rounded
+// division, on the host:
 // random significands in every binade from the subnormals up, both signs;
 // every subnormal multiple of 2^-1074 below 2^20 units; the seven weight
-// numerators of src/filtered_noise.h:33-61 at random and at tiny t.
+// numerators of src/filtered_noise.h:33-61 at random and at tiny t; divp
+// over random quotients of normal magnitude.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
 
-#include "../../qew_amd/csrc/qew_div6.hpp"
+#include "../../qew_amd/csrc/qew_div.hpp"
 
 static uint64_t s = 88172645463325252ull;
 static uint64_t rnd() {
@@ -62,6 +64,28 @@ int main() {
     }
     check(0.0);
     check(-0.0);
+    // divp: a / b from RN(1 / b) for spacings b > 0, correctly rounded when
+    // the quotient is normal or zero (b = 1 exactly, random b, b with an
+    // all-ones significand)
+    for (long k = 0; k < 2000000; ++k) {
+        double b = std::ldexp(1.0 + (double)(rnd() >> 12) * 0x1p-52, (int)(rnd() % 41) - 20);
+        if (k % 7 == 0) {
+            uint64_t bits;
+            std::memcpy(&bits, &b, 8);
+            bits |= 0x000FFFFFFFFFFFFFull;
+            std::memcpy(&b, &bits, 8);
+        }
+        if (k % 11 == 0) b = 1.0;
+        double a = std::ldexp(1.0 + (double)(rnd() >> 12) * 0x1p-52, (int)(rnd() % 81) - 40);
+        if (k % 3 == 0) a = (double)(int64_t)(rnd() >> 40) * std::ldexp(1.0, -(int)(rnd() % 30));
+        if (k & 1) a = -a;
+        const double d = a / b, q = qew_div::divp(a, b, 1.0 / b);
+        ++n;
+        if (std::memcmp(&d, &q, 8) != 0) {
+            if (bad_normal < 5) std::printf("divp(%a, %a) = %a, a / b = %a\n", a, b, q, d);
+            ++bad_normal;
+        }
+    }
     std::printf("%ld values: %ld div6_sub mismatches, %ld div6 mismatches with a normal quotient\n", n,
                 bad_any, bad_normal);
     return (bad_any || bad_normal) ? 1 : 0;

@@ -36,10 +36,10 @@ def test_cli_parser_selftest(tmp_path):
 
 
 def test_division_free_weights_selftest(tmp_path):
-    """tests/cpp/div6_selftest.cpp: the device weights' x / 6 without a
+    """tests/cpp/div_selftest.cpp: the device weights' x / 6 without a
     division equals the correctly rounded division (host build of the same
     header)."""
-    exe = compile_cpp("div6_selftest.cpp", str(tmp_path / "div6_selftest"))
+    exe = compile_cpp("div_selftest.cpp", str(tmp_path / "div_selftest"))
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout
 
