@@ -17,7 +17,8 @@
 //
 // The preconditioner d = irfft(rfft(q) / eig) runs in LDS: the real line of
 // length n is packed as n/2 complex values, transformed by an in-place
-// Stockham FFT (radix 4, one radix-2 stage for odd log2), unpacked to the
+// Stockham FFT (radix 8 for FP32 state, 4 for FP64, then at most one radix-4
+// or radix-2 stage), unpacked to the
 // half spectrum, divided by eig, repacked and transformed back. Twiddles and
 // the eig sines come from one table of exp(2 pi i j / M), M = 4 * the
 // largest transform length, filled once per launch.
@@ -877,6 +878,39 @@ __device__ __forceinline__ void fft_stage(C2<T> *buf, int N, int Ns, int sign, c
         } else if constexpr (R == 2) {
             buf[idxD] = {u[0].x + u[1].x, u[0].y + u[1].y};
             buf[idxD + Ns] = {u[0].x - u[1].x, u[0].y - u[1].y};
+        } else if constexpr (R == 8) {
+            // radix 2 x 4: a = u_q + u_{q+4} and b = (u_q - u_{q+4}) W_8^q
+            // (q = 0..3) give the even and the odd outputs as 4-point DFTs
+            const T h = (T)0.70710678118654752440;  // cos(pi / 4)
+            auto rot = [&](const C2<T> &e) {  // e times W_4 = e^(sign i pi / 2)
+                return (sign < 0) ? C2<T>{e.y, -e.x} : C2<T>{-e.y, e.x};
+            };
+            C2<T> a[4], b[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                a[q] = {u[q].x + u[q + 4].x, u[q].y + u[q + 4].y};
+                b[q] = {u[q].x - u[q + 4].x, u[q].y - u[q + 4].y};
+            }
+            // W_8 = h (1 + sign i), W_8^2 = W_4, W_8^3 = h (-1 + sign i)
+            {
+                const C2<T> r1 = rot(b[1]);
+                b[1] = {h * (b[1].x + r1.x), h * (b[1].y + r1.y)};
+                b[2] = rot(b[2]);
+                const C2<T> r3 = rot(b[3]);
+                b[3] = {h * (r3.x - b[3].x), h * (r3.y - b[3].y)};
+            }
+            auto dft4 = [&](const C2<T> (&x)[4], int o0) {
+                const C2<T> p = {x[0].x + x[2].x, x[0].y + x[2].y};
+                const C2<T> m = {x[0].x - x[2].x, x[0].y - x[2].y};
+                const C2<T> c = {x[1].x + x[3].x, x[1].y + x[3].y};
+                const C2<T> e = rot(C2<T>{x[1].x - x[3].x, x[1].y - x[3].y});
+                buf[idxD + o0 * Ns] = {p.x + c.x, p.y + c.y};
+                buf[idxD + (o0 + 2) * Ns] = {m.x + e.x, m.y + e.y};
+                buf[idxD + (o0 + 4) * Ns] = {p.x - c.x, p.y - c.y};
+                buf[idxD + (o0 + 6) * Ns] = {m.x - e.x, m.y - e.y};
+            };
+            dft4(a, 0);
+            dft4(b, 1);
         } else {
             const C2<T> a = {u[0].x + u[2].x, u[0].y + u[2].y};
             const C2<T> b = {u[0].x - u[2].x, u[0].y - u[2].y};
@@ -892,16 +926,21 @@ __device__ __forceinline__ void fft_stage(C2<T> *buf, int N, int Ns, int sign, c
     __syncthreads();
 }
 
-// Radices in the order 4, 2 and, in the MIXED instantiation (lengths with
-// factors 3, 5 or 7; its larger butterflies would raise the register pressure
-// of the power-of-two kernels), 3, 5, 7. The host accepts only lengths with
+// Radices in the order 8, 4, 2 (power-of-two kernels, FP32 state), 4, 2
+// (power-of-two kernels, FP64 state: radix 8 measured no faster there, and
+// radix 4 keeps the FP64 iterates as they were) or 4, 2 and then 3, 5, 7 in
+// the MIXED instantiation (lengths with factors 3, 5 or 7; its larger
+// butterflies would raise the register pressure of the power-of-two kernels). The host accepts only lengths with
 // no other prime factor; any other remainder stops the loop rather than spin.
 template <typename T, bool MIXED>
 __device__ void lds_fft(C2<T> *buf, int N, int sign, const C2<T> *tw, int M) {
     int Ns = 1;
     if constexpr (!MIXED) {
         while (Ns < N) {
-            if ((N / Ns) % 4 == 0) {
+            if (sizeof(T) == 4 && (N / Ns) % 8 == 0) {
+                fft_stage<T, 8>(buf, N, Ns, sign, tw, M);
+                Ns *= 8;
+            } else if ((N / Ns) % 4 == 0) {
                 fft_stage<T, 4>(buf, N, Ns, sign, tw, M);
                 Ns *= 4;
             } else {

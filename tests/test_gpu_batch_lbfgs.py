@@ -60,6 +60,38 @@ def test_batch_lbfgs_fp64_matches_oracle(q, precondition):
         assert np.max(np.abs(line - h_ref)) < 1e-5
 
 
+@pytest.mark.parametrize("iters", [1, 2])
+@pytest.mark.parametrize("lengths", [[16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384],
+                                     [14, 96, 640, 1000, 256]], ids=["pow2", "mixed"])
+def test_batch_lbfgs_first_steps_match_oracle(q, iters, lengths):
+    """The first one or two preconditioned steps against the oracle's: the
+    step is alpha P^-1 g with P applied through the LDS FFT, so every radix
+    sequence of the transform (n/2 = 8 ... 8192 complex points on the
+    power-of-two kernel: radix 8 alone, 8 then 2, 8 then 4, ...; and the
+    mixed-radix kernel, which a batch with any other even length uses for all
+    its lines) is checked to rounding, far below what a wrong butterfly or
+    twiddle would give."""
+    ny = 128
+    xis = [2.0 + (k % 3) for k in range(len(lengths))]
+    models = [k % 2 for k in range(len(lengths))]
+    noises, oracle, params, h0 = make_problem(q, lengths, ny, xis, models,
+                                              list(range(31, 31 + len(lengths))), q.F64)
+    lp_kw = dict(m=10, ftol=1e-12, max_iter=iters, precondition=True)
+    b = q.Batch(lengths, noises, params)
+    h = dev(np.concatenate(h0))
+    res = b.lbfgs(h, q.LbfgsParams(**lp_kw, precond_shift=0.5 / 16.0), dtype=q.F64)
+    hh = host(h)
+    for k, n in enumerate(lengths):
+        on, po = oracle[k]
+        h_ref, r_ref = O.lbfgs(h0[k], po, on, O.lbfgs_params(**lp_kw, precond_shift=0.5 / 16.0),
+                               omp=False)
+        assert res[k].iterations == r_ref.iterations == iters, (k, n, res[k], r_ref.iterations)
+        line = hh[b.offsets[k]:b.offsets[k + 1]]
+        moved = np.max(np.abs(h_ref - h0[k]))
+        assert moved > 1e-3, (k, n, moved)
+        assert np.max(np.abs(line - h_ref)) < 1e-11 * max(1.0, moved), (k, n)
+
+
 def test_batch_lbfgs_fp32_energy_error(q):
     lengths = [1024, 2048, 512, 1024, 4096, 256]
     xis = [2.0, 4.0, 8.0, 4.0, 8.0, 2.0]
