@@ -321,6 +321,10 @@ def bench_lbfgs(log, nlines=512, plain_sample=8, with_cpu=True, omp_info=None):
                      "ls_restarts": int(sum(r.n_ls_restarts for r in res)),
                      "slowest_line": int(np.argmax(its)),
                      "iterations_mean": float(np.mean(its)), "iterations_max": int(max(its)),
+                     "objective_evals_per_iteration": float(
+                         sum(r.n_func_evals for r in res) / max(1, sum(its))),
+                     "critical_line_objective_evals_per_iteration": float(
+                         res[crit].n_func_evals / max(1, its[crit])),
                      "workgroup_utilisation": {
                          "value": busy / (wgs * span), "workgroups": wgs,
                          "busy_workgroup_s_estimate": busy,
