@@ -926,11 +926,11 @@ __device__ __forceinline__ void fft_stage(C2<T> *buf, int N, int Ns, int sign, c
     __syncthreads();
 }
 
-// Radices in the order 8, 4, 2 (power-of-two kernels, FP32 state), 4, 2
-// (power-of-two kernels, FP64 state: radix 8 measured no faster there, and
-// radix 4 keeps the FP64 iterates as they were) or 4, 2 and then 3, 5, 7 in
-// the MIXED instantiation (lengths with factors 3, 5 or 7; its larger
-// butterflies would raise the register pressure of the power-of-two kernels). The host accepts only lengths with
+// Radices in the order 8 (FP32 state only: radix 8 measured no faster for
+// FP64 state, and radix 4 keeps the FP64 iterates as they were), 4, 2 and,
+// in the MIXED instantiation (lengths with factors 3, 5 or 7; its larger
+// butterflies would raise the register pressure of the power-of-two
+// kernels), then 3, 5, 7. The host accepts only lengths with
 // no other prime factor; any other remainder stops the loop rather than spin.
 template <typename T, bool MIXED>
 __device__ void lds_fft(C2<T> *buf, int N, int sign, const C2<T> *tw, int M) {
@@ -952,7 +952,10 @@ __device__ void lds_fft(C2<T> *buf, int N, int sign, const C2<T> *tw, int M) {
     }
     while (Ns < N) {
         const int rest = N / Ns;
-        if (rest % 4 == 0) {
+        if (sizeof(T) == 4 && rest % 8 == 0) {
+            fft_stage<T, 8>(buf, N, Ns, sign, tw, M);
+            Ns *= 8;
+        } else if (rest % 4 == 0) {
             fft_stage<T, 4>(buf, N, Ns, sign, tw, M);
             Ns *= 4;
         } else if (rest % 2 == 0) {
