@@ -40,6 +40,35 @@ def test_sampler_bit_exact(q):
         np.testing.assert_array_equal(host(g), r)
 
 
+@pytest.mark.parametrize("spacing", ["unit", "odd"])
+def test_sampler_bit_exact_at_cell_edges_and_tiny_fractions(q, spacing):
+    """Heights and positions on and next to cell boundaries, and tiny
+    positive ones whose fractional cell coordinate t makes t^3 / 6 or
+    3 t^2 / 6 subnormal: the division-free weight and spacing quotients must
+    give the oracle's divisions bit for bit there too (unit spacing, and a
+    table whose spacings are not 1)."""
+    if spacing == "unit":
+        on, gn, _ = table_pair(q, 32, 24, 32.0, 24.0, 5)
+        dx, dy = 1.0, 1.0
+    else:
+        on, gn, _ = table_pair(q, 48, 40, 2.5, 1.3, 17)
+        dx, dy = 2.5 / 48, 1.3 / 40
+    rng = np.random.default_rng(3)
+    k = rng.integers(-60, 60, 300).astype(np.float64)
+    ulp = np.spacing(np.abs(k) * dy + 1.0)
+    y = np.concatenate([k * dy, k * dy + ulp, k * dy - ulp,
+                        [1e-300, 5e-324, 2.0 ** -340 * dy, 2.0 ** -345 * dy, 2.0 ** -520 * dy,
+                         2.0 ** -1000, 3.0 * 2.0 ** -1074, 9.0 * 2.0 ** -1074, 0.0, -0.0]])
+    extra = rng.uniform(2.0 ** -360, 2.0 ** -330, 200) * dy
+    y = np.concatenate([y, extra, -extra])
+    x = np.concatenate([rng.integers(-40, 40, len(y) // 2) * dx,
+                        rng.uniform(-5, 5, len(y) - len(y) // 2)])
+    ref = O.sample_points(on, x, y)
+    got = gn.sample(dev(x), dev(y))
+    for r, g in zip(ref, got):
+        np.testing.assert_array_equal(host(g), r)
+
+
 def test_sampler_analytic(q):
     on = O.OracleNoise(O.SINCOS, p=(0.3, 0.7, 1.3, 0))
     gn = q.AnalyticNoise(0.3, 0.7, 1.3)
