@@ -17,8 +17,8 @@
 //
 // The preconditioner d = irfft(rfft(q) / eig) runs in LDS: the real line of
 // length n is packed as n/2 complex values, transformed by an in-place
-// Stockham FFT (radix 8 for FP32 state, 4 for FP64, then at most one radix-4
-// or radix-2 stage), unpacked to the
+// Stockham FFT (radix 8 for FP32 state on FP32 tables, else radix 4, then at
+// most one radix-4 or radix-2 stage), unpacked to the
 // half spectrum, divided by eig, repacked and transformed back. Twiddles and
 // the eig sines come from one table of exp(2 pi i j / M), M = 4 * the
 // largest transform length, filled once per launch.
@@ -926,18 +926,25 @@ __device__ __forceinline__ void fft_stage(C2<T> *buf, int N, int Ns, int sign, c
     __syncthreads();
 }
 
-// Radices in the order 8 (FP32 state only: radix 8 measured no faster for
-// FP64 state, and radix 4 keeps the FP64 iterates as they were), 4, 2 and,
-// in the MIXED instantiation (lengths with factors 3, 5 or 7; its larger
-// butterflies would raise the register pressure of the power-of-two
-// kernels), then 3, 5, 7. The host accepts only lengths with
+// Radix-8 stages for FP32 state on FP32 tables: there the transform's five
+// rounding stages instead of seven bring the directions close enough to
+// FP64's that the line search needs fewer trials. FP64 state measured no
+// faster with them and keeps its iterates; FP32 state on midpoint tables
+// keeps radix 4, with which every minimum of the batched workload lands in
+// its FP64 basin (with radix 8 one line settles in a nearby one).
+template <typename T, bool MID>
+constexpr bool kR8 = sizeof(T) == 4 && !MID;
+
+// Radices in the order 8 (R8), 4, 2 and, in the MIXED instantiation
+// (lengths with factors 3, 5 or 7; its larger butterflies would raise the
+// register pressure of the power-of-two kernels), then 3, 5, 7. The host accepts only lengths with
 // no other prime factor; any other remainder stops the loop rather than spin.
-template <typename T, bool MIXED>
+template <typename T, bool MIXED, bool R8>
 __device__ void lds_fft(C2<T> *buf, int N, int sign, const C2<T> *tw, int M) {
     int Ns = 1;
     if constexpr (!MIXED) {
         while (Ns < N) {
-            if (sizeof(T) == 4 && (N / Ns) % 8 == 0) {
+            if (R8 && (N / Ns) % 8 == 0) {
                 fft_stage<T, 8>(buf, N, Ns, sign, tw, M);
                 Ns *= 8;
             } else if ((N / Ns) % 4 == 0) {
@@ -952,7 +959,7 @@ __device__ void lds_fft(C2<T> *buf, int N, int sign, const C2<T> *tw, int M) {
     }
     while (Ns < N) {
         const int rest = N / Ns;
-        if (sizeof(T) == 4 && rest % 8 == 0) {
+        if (R8 && rest % 8 == 0) {
             fft_stage<T, 8>(buf, N, Ns, sign, tw, M);
             Ns *= 8;
         } else if (rest % 4 == 0) {
@@ -983,11 +990,11 @@ __device__ void lds_fft(C2<T> *buf, int N, int sign, const C2<T> *tw, int M) {
 // d = irfft(rfft(q) / eig), eig[k] = (kappa/dx) 4 sin^2(pi k / n) + shift
 // (src/lbfgs.h:116-146, 173-179); n = 2N. `buf` holds the packed q on entry
 // and d, as n reals, on exit.
-template <typename T, bool MIXED>
+template <typename T, bool MIXED, bool R8>
 __device__ void wg_precondition(int n, double kdx, double shift, C2<T> *buf, const C2<T> *tw,
                                 int M) {
     const int N = n / 2;
-    lds_fft<T, MIXED>(buf, N, -1, tw, M);
+    lds_fft<T, MIXED, R8>(buf, N, -1, tw, M);
     const int e_w = M / n;        // W_n^k = conj(tw[k e_w])
     const int e_s = M / (2 * n);  // sin(pi k / n) = tw[k e_s].y
     auto eig = [&](int k) {
@@ -1027,7 +1034,7 @@ __device__ void wg_precondition(int n, double kdx, double shift, C2<T> *buf, con
         }
     }
     __syncthreads();
-    lds_fft<T, MIXED>(buf, N, +1, tw, M);
+    lds_fft<T, MIXED, R8>(buf, N, +1, tw, M);
     // complex j is reals 2j, 2j + 1: each thread scales its own entries in place
     const T inv = (T)(1.0 / (double)N);
     for (int j = threadIdx.x; j < N; j += blockDim.x) {
@@ -1041,7 +1048,7 @@ __device__ void wg_precondition(int n, double kdx, double shift, C2<T> *buf, con
 // the line's vector exchange, the leader (member 0) receives the rest of the
 // line into its LDS buffer, transforms it and sends back the others' sites of
 // d, which they receive into their own buffers.
-template <typename T, bool MIXED>
+template <typename T, bool MIXED, bool R8>
 __device__ __forceinline__ void team_precondition(Team &tm, int n, double kdx, double shift,
                                                   C2<T> *buf, const C2<T> *tw, int M) {
     T *fb = reinterpret_cast<T *>(buf);
@@ -1053,7 +1060,7 @@ __device__ __forceinline__ void team_precondition(Team &tm, int n, double kdx, d
             fb[i] = get_elem<T>(tm.bulk, i, tag_q, polls, ok, tm.abort);
         if (!ok) *tm.s_ok = 0;
         __syncthreads();
-        wg_precondition<T, MIXED>(n, kdx, shift, buf, tw, M);
+        wg_precondition<T, MIXED, R8>(n, kdx, shift, buf, tw, M);
         for (int i = tm.s1 + (int)threadIdx.x; i < n; i += kLB) put_elem<T>(tm.bulk, i, tag_d, fb[i]);
     } else {
         for (int i = tm.s0 + (int)threadIdx.x; i < tm.s1; i += kLB)
@@ -1242,9 +1249,9 @@ __device__ __forceinline__ void solve_line(const BatchArgs &a, int line, Red r, 
         if (pre) {
             first_loop(fb);
             if constexpr (TEAM)
-                team_precondition<T, MIXED>(tm, n, kdx, shift, fftbuf, tw, MIXED ? 2 * n : a.tw_m);
+                team_precondition<T, MIXED, kR8<T, MID>>(tm, n, kdx, shift, fftbuf, tw, MIXED ? 2 * n : a.tw_m);
             else
-                wg_precondition<T, MIXED>(n, kdx, shift, fftbuf, tw, MIXED ? 2 * n : a.tw_m);
+                wg_precondition<T, MIXED, kR8<T, MID>>(n, kdx, shift, fftbuf, tw, MIXED ? 2 * n : a.tw_m);
         } else {
             first_loop(q);
             const T g0 = (T)(nslots == 0 ? 1.0 : gamma);
