@@ -56,6 +56,7 @@ __global__ __launch_bounds__(kTile) void langevin64_kernel(StepArgs a, LineDesc 
     const int i = gl < 0 ? n - 1 : gl;
     const double *h = static_cast<const double *>(a.h_in) + L.off;
     const double dx = L.L / (double)n;
+    const double rdx = uniform_f64(1.0 / dx), rdx2 = uniform_f64(1.0 / (dx * dx));
     const double lt = L.lt, f = L.f;
     const bool arc = (L.model == 1);
     double hi = 0, hip = 0;
@@ -68,7 +69,7 @@ __global__ __launch_bounds__(kTile) void langevin64_kernel(StepArgs a, LineDesc 
         s_dvdy[tid] = sample_dy64<C>(N, xc, hc);
         s_h[tid] = hi;
         if (arc) {
-            const double dh = div1(hip - hi, dx);
+            const double dh = divr(hip - hi, dx, rdx);
             s_slope[tid] = dh / sqrt(1.0 + dh * dh);
         }
     }
@@ -77,9 +78,9 @@ __global__ __launch_bounds__(kTile) void langevin64_kernel(StepArgs a, LineDesc 
     double line_force;
     if (!arc) {
         const double d2h = s_h[tid - 1] + hip - 2.0 * hi;
-        line_force = div1(-lt * d2h, dx * dx);
+        line_force = divr(-lt * d2h, dx * dx, rdx2);
     } else {
-        line_force = div1(lt * (s_slope[tid - 1] - s_slope[tid]), dx);
+        line_force = divr(lt * (s_slope[tid - 1] - s_slope[tid]), dx, rdx);
     }
     const double g = dx * (line_force + 0.5 * (s_dvdy[tid - 1] + s_dvdy[tid]) - f);
     double hn;
@@ -232,6 +233,7 @@ __global__ __launch_bounds__(kLvExt) void langevin64_multi(MultiArgs a) {
     const NoiseDesc N = a.noises[L.noise];
     const int n = L.n;
     const double dx = L.L / (double)n;
+    const double rdx = uniform_f64(1.0 / dx), rdx2 = uniform_f64(1.0 / (dx * dx));
     const double lt = L.lt, f = L.f;
     const bool arc = (L.model == 1);
     const int gi = ext_site<K>(tl.y, t, n);
@@ -253,7 +255,7 @@ __global__ __launch_bounds__(kLvExt) void langevin64_multi(MultiArgs a) {
             const double hp = s_h[cur][t + 1];
             s_dvdy[t + 1] = sample_dy64<C>(N, xc, (hp + hi) * 0.5);
             if (arc) {
-                const double dh = div1(hp - hi, dx);
+                const double dh = divr(hp - hi, dx, rdx);
                 s_slope[t + 1] = dh / sqrt(1.0 + dh * dh);
             }
         }
@@ -261,8 +263,8 @@ __global__ __launch_bounds__(kLvExt) void langevin64_multi(MultiArgs a) {
         double hn = hi;
         if (t >= 1 && t + 1 < kLvExt) {
             double line_force;
-            if (!arc) line_force = div1(-lt * (s_h[cur][t - 1] + s_h[cur][t + 1] - 2.0 * hi), dx * dx);
-            else line_force = div1(lt * (s_slope[t] - s_slope[t + 1]), dx);
+            if (!arc) line_force = divr(-lt * (s_h[cur][t - 1] + s_h[cur][t + 1] - 2.0 * hi), dx * dx, rdx2);
+            else line_force = divr(lt * (s_slope[t] - s_slope[t + 1]), dx, rdx);
             const double g = dx * (line_force + 0.5 * (s_dvdy[t] + s_dvdy[t + 1]) - f);
             const double dr = a.drift[s];
             if (a.diff[s] > 0) hn = hi + (-dr * g + s_kick[s % KC][t]);
@@ -433,6 +435,7 @@ __global__ __launch_bounds__(kLvExt) void langevin64_wave(MultiArgs a) {
     const NoiseDesc N = a.noises[L.noise];
     const int n = L.n;
     const double dx = L.L / (double)n;
+    const double rdx = uniform_f64(1.0 / dx), rdx2 = uniform_f64(1.0 / (dx * dx));
     const double lt = L.lt, f = L.f;
     const bool arc = (L.model == 1);
     const int gi = ext_site<K>(tl.y, lane, n);
@@ -452,11 +455,11 @@ __global__ __launch_bounds__(kLvExt) void langevin64_wave(MultiArgs a) {
         const double dyl = __shfl_up(dy, 1);
         double line_force;
         if (!arc) {
-            line_force = div1(-lt * (hm + hp - 2.0 * h), dx * dx);
+            line_force = divr(-lt * (hm + hp - 2.0 * h), dx * dx, rdx2);
         } else {
-            const double dh = div1(hp - h, dx);
+            const double dh = divr(hp - h, dx, rdx);
             const double sl = dh / sqrt(1.0 + dh * dh);
-            line_force = div1(lt * (__shfl_up(sl, 1) - sl), dx);
+            line_force = divr(lt * (__shfl_up(sl, 1) - sl), dx, rdx);
         }
         const double g = dx * (line_force + 0.5 * (dyl + dy) - f);
         const double dr = a.drift[s];
